@@ -47,6 +47,8 @@ def main():
             g = r.geom
             # stored triplet count for COO/SCOO is the nnz field after a fill
             arrays = r.arrays()
+            if fmt == O.DENSE and not r._ptrs()[0]:
+                continue  # empty DENSE block: the reference leaves nnz uninitialised
             x = np.arange(g["n_col"], dtype=np.float64)[g["f_col"]:] if fmt != O.COO else \
                 np.arange(g["n_col"], dtype=np.float64)
             length = g["n_row"] if fmt == O.COO else g["ln_row"]
