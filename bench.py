@@ -1,0 +1,104 @@
+"""Benchmark: FP64 CSR SpMV on the n = 2^22, c = 32, q = 0.1 cqmat (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one SpMV y = A x over the whole matrix with A and x resident in
+HBM (values and x uniform in [0,1), structure seed 0).  The timed region is K
+back-to-back launches between two HIP events on the launch stream.  `value`
+is algorithmic bytes / time: 12 bytes per stored entry (FP64 value + int32
+column), 4 (n+1) for the row pointers, 8 n each for x and y.
+
+Printed: one JSON line with the roofline of the CSR kernel (HBM-bound,
+against 8 TB/s) and the reference's own OpenMP CPU SpMV (oracle/_ref, the
+reference library compiled from its sources) timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SpMV GB/s (% HBM peak); CG time-to-1e-8, learned vs Jacobi vs plain"
+HBM_PEAK_GBS = 8000.0
+N, C, Q = 4194304, 32, 0.1
+
+
+def algorithmic_bytes_csr(stored, n_rows, n_cols):
+    return 12 * stored + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
+
+
+def cpu_baseline(n, c, q, stored, iters, reps, threads):
+    """The reference's CPU SpMV (MatrixCSR::spmv, OpenMP static rows) on this
+    host: oracle/_ref/ref_bench, built from the reference sources."""
+    import oracle
+    exe = oracle.REF_BENCH
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    out = subprocess.run([exe, "CSR", str(n), str(c), str(q), "0", "0", "0", "1", "1",
+                          str(iters), str(reps)], capture_output=True, text=True, env=env,
+                         timeout=600)
+    if out.returncode != 0:
+        return None
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    t = rec["time_op"]
+    gbs = algorithmic_bytes_csr(stored, n, n) / t / 1e9
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": rec["omp_threads"],
+            "kind": "reference",
+            "sample": f"reference MatrixCSR::spmv (OpenMP static) on the full n={n} c={c} "
+                      f"q={q} matrix, median over {reps} repeats of the mean of {iters} calls "
+                      f"({t * 1e3:.2f} ms per SpMV)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 16)))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (one block of the 2-D grid per GPU); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import tbsla_amd as T
+    m = T.Matrix(T.CSR)
+    t0 = time.time()
+    m.fill_cqmat(N, N, C, Q, 0, value_seed=0)
+    t_fill = time.time() - t0
+    g = m.geometry()
+    dx = T.DeviceBuffer(N * 8)
+    T.check(T.lib().tbsla_hip_fill_unit_vector(dx.ptr, N, 1, None))
+    dy = T.DeviceBuffer(N * 8)
+    if args.warmup:
+        m.time_spmv(dy, dx, args.warmup)
+    ms = m.time_spmv(dy, dx, args.steps)
+    per = ms / args.steps
+    nbytes = algorithmic_bytes_csr(g.stored, g.ln_row, g.ln_col)
+    gbs = nbytes / (per * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"k_csr_spmv<{m.get_csr_lanes()}>", "bytes_per_launch": nbytes,
+            "avg_launch_us": round(per * 1e3, 2)}
+    cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
+    line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "csr_spmv_cqmat", "format": "CSR", "n": N, "c": C, "q": Q,
+                       "seed_mult": 0, "values": "U[0,1) seed 0", "x": "U[0,1) seed 1",
+                       "stored_nnz": g.stored, "nnz_field": g.nnz, "lanes_per_row":
+                       m.get_csr_lanes(), "fill_s": round(t_fill, 3)},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * tbsla_hip.h -- C ABI of libtbsla_hip, the MI355X (gfx950) implementation of
+ * TBSLA's sparse hot path.
+ *
+ * One opaque handle holds one matrix block on the GPU in one of TBSLA's five
+ * formats.  It replaces a tbsla::cpp::Matrix object
+ * (src/cpp/include/tbsla/cpp/Matrix.hpp:11-57) and the stale opaque-handle C
+ * interface C_CPP_Matrix<FMT>_* (src/cpp/interface/c/include/tbsla/cpp/
+ * MatrixCSR.h:10-21), which does not compile.
+ *
+ * Conventions
+ *   - Every function returns a status (TBSLA_HIP_SUCCESS = 0); on failure
+ *     tbsla_hip_last_error() describes the cause (per thread).
+ *   - Vector arguments named d_* are device pointers (hipMalloc memory, or
+ *     tbsla_hip_malloc); h_* are host pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL: the null stream).
+ *     Work is enqueued on it; functions that return scalars to the host
+ *     (iteration counts, norms) synchronise that stream.  The library creates
+ *     no streams of its own.
+ *   - Thread-compatible: one handle per thread, or external synchronisation.
+ *   - Geometry follows the reference exactly (n_row .. NC, nnz), including
+ *     the count-pass nnz of CSR fill_cqmat (MatrixCSR.cpp:423-434) that can
+ *     exceed the stored entries; `stored` reports what is held.
+ *   - Device layouts: CSR rowptr/colidx/values; ELL column-major
+ *     (columns[j * ln_row + i], the transpose of the reference's row-major
+ *     layout, so that a wavefront's loads coalesce); COO/SCOO triplets;
+ *     DENSE row-major.  tbsla_hip_export returns the reference's layouts.
+ */
+#ifndef TBSLA_HIP_H
+#define TBSLA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tbsla_hip_matrix_s *tbsla_hip_matrix_t;
+
+/* Formats, in the order of the reference perf app's --format switch
+ * (src/cpp/apps/tbsla_perf.cpp:217-230). */
+enum {
+  TBSLA_HIP_COO = 0,
+  TBSLA_HIP_SCOO = 1,
+  TBSLA_HIP_CSR = 2,
+  TBSLA_HIP_ELL = 3,
+  TBSLA_HIP_DENSE = 4
+};
+
+enum {
+  TBSLA_HIP_SUCCESS = 0,
+  TBSLA_HIP_ERR_INVALID = 1,     /* bad argument or handle */
+  TBSLA_HIP_ERR_HIP = 2,         /* a HIP runtime call failed */
+  TBSLA_HIP_ERR_ALLOC = 3,       /* device allocation failed */
+  TBSLA_HIP_ERR_UNSUPPORTED = 4, /* operation not defined for this format */
+  TBSLA_HIP_ERR_RANGE = 5        /* size beyond int32 indexing */
+};
+
+/* Geometry of a block: the protected fields of tbsla::cpp::Matrix
+ * (Matrix.hpp:54-55) and their getters (:21-31), plus what the GPU holds. */
+typedef struct {
+  int n_row, n_col, f_row, f_col, ln_row, ln_col, pr, pc, NR, NC;
+  long long nnz;    /* the reference's nnz field (get_nnz) */
+  long long stored; /* entries held (CSR rowptr[ln_row]; COO/SCOO triplets) */
+  int max_col;      /* ELL row width */
+  int format;
+} tbsla_hip_geometry;
+
+const char *tbsla_hip_status_string(int status);
+const char *tbsla_hip_last_error(void);
+int tbsla_hip_version(void);
+
+/* ---- memory helpers (for callers without their own allocator) ----------- */
+int tbsla_hip_malloc(void **d_ptr, size_t bytes);
+int tbsla_hip_free(void *d_ptr);
+int tbsla_hip_memcpy_htod(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int tbsla_hip_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int tbsla_hip_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int tbsla_hip_stream_synchronize(void *stream);
+int tbsla_hip_device_count(int *count);
+
+/* ---- handles ------------------------------------------------------------ */
+/* C_CPP_Matrix<FMT>_create / _destroy (cpp/interface/c/include/tbsla/cpp/
+ * MatrixCSR.h:10-11). */
+int tbsla_hip_create(tbsla_hip_matrix_t *out, int format);
+int tbsla_hip_destroy(tbsla_hip_matrix_t m);
+int tbsla_hip_get_geometry(tbsla_hip_matrix_t m, tbsla_hip_geometry *out);
+
+/* ---- generators --------------------------------------------------------- */
+/* Matrix::fill_cdiag (Matrix.hpp:41); per format MatrixCSR.cpp:297-371,
+ * MatrixELL.cpp:261-333, MatrixCOO.cpp:280-318, MatrixSCOO.cpp:228-293,
+ * MatrixDENSE.cpp:129-185. */
+int tbsla_hip_fill_cdiag(tbsla_hip_matrix_t m, int n_row, int n_col, int cdiag, int pr, int pc,
+                         int NR, int NC, void *stream);
+/* Matrix::fill_cqmat (Matrix.hpp:42); per format MatrixCSR.cpp:373-486,
+ * MatrixELL.cpp:335-466, MatrixCOO.cpp:320-354, MatrixSCOO.cpp:295-403,
+ * MatrixDENSE.cpp:187-270.  value_seed < 0 stores 1.0 for every entry, as
+ * the reference; value_seed >= 0 stores V(value_seed, k), the project's
+ * uniform [0,1) value of generator index k (no reference counterpart). */
+int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, int c, double q,
+                         unsigned seed_mult, int pr, int pc, int NR, int NC,
+                         long long value_seed, void *stream);
+/* x_i = V(seed, i) for i < n into a device vector (the synthetic inputs). */
+int tbsla_hip_fill_unit_vector(double *d_x, long long n, long long seed, void *stream);
+
+/* ---- import / export ---------------------------------------------------- */
+/* The arrays of the reference layout, copied to the host (the fields a
+ * Matrix::write would emit, MatrixCSR.cpp:236-261 etc.):
+ *   CSR:   h_a = rowptr (ln_row+1 int), h_b = colidx (stored int), h_v = values
+ *   ELL:   h_a = columns (ln_row*max_col int, row-major), h_v = values (row-major)
+ *   COO/SCOO: h_a = row, h_b = col (stored int), h_v = values
+ *   DENSE: h_v = values (ln_row*ln_col, row-major)
+ * Unused pointers may be NULL. */
+int tbsla_hip_export(tbsla_hip_matrix_t m, int *h_a, int *h_b, double *h_v, void *stream);
+/* Builds a block from host arrays in the reference layout (the state a
+ * Matrix::read restores, MatrixCSR.cpp:263-295; `geometry` supplies
+ * n_row .. NC, nnz, stored and, for ELL, max_col).  Array roles as export. */
+int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry *geometry,
+                     const int *h_a, const int *h_b, const double *h_v, void *stream);
+/* Conversion constructors <FMT>(const MatrixCOO&): MatrixCSR.cpp:49-96,
+ * MatrixELL.cpp:25-79, MatrixSCOO.cpp:75-101, MatrixDENSE.cpp:15-37.  The
+ * (row, col) sort is stable (the reference's std::sort leaves the order of
+ * equal keys unspecified). */
+int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_matrix_t coo, void *stream);
+
+/* ---- SpMV and composites ------------------------------------------------ */
+/* Matrix::Ax (Matrix.hpp:15): CSR and ELL overwrite r (MatrixCSR.cpp:209-233,
+ * MatrixELL.cpp:181-205); COO, SCOO and DENSE add into r
+ * (MatrixCOO.cpp:135-142, MatrixSCOO.cpp:154-164, MatrixDENSE.cpp:77-86).
+ * d_v has ln_col entries (n_col for COO); d_r has ln_row (n_row for COO). */
+int tbsla_hip_Ax(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
+                 void *stream);
+/* Matrix::spmv (Matrix.hpp:14): r = 0, then Ax. */
+int tbsla_hip_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
+                   void *stream);
+/* Matrix::a_axpx_ (src/cpp/matrix/Matrix.cpp:5-14): r = A(Av + v); d_tmp
+ * holds ln_row doubles of scratch. */
+int tbsla_hip_a_axpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_tmp,
+                     int vect_incr, void *stream);
+/* Matrix::AAxpAx / AAxpAxpx (Matrix.cpp:16-31); d_buf is the caller's
+ * buffer, used without zeroing as the reference does. */
+int tbsla_hip_AAxpAx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_buf,
+                     int vect_incr, void *stream);
+int tbsla_hip_AAxpAxpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_buf,
+                       int vect_incr, void *stream);
+/* Times `iters` back-to-back tbsla_hip_spmv calls between two events on
+ * `stream`; *ms = total milliseconds.  For measurement only. */
+int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
+                        float *ms, void *stream);
+/* CSR SpMV schedule: lanes cooperating on one row (2..64, power of two);
+ * 0 restores the choice from the mean row length. */
+int tbsla_hip_set_csr_lanes(tbsla_hip_matrix_t m, int lanes);
+int tbsla_hip_get_csr_lanes(tbsla_hip_matrix_t m, int *lanes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

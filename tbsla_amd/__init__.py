@@ -1,0 +1,311 @@
+"""tbsla_amd -- MI355X (gfx950) implementation of TBSLA's sparse hot path.
+
+The product is ``libtbsla_hip.so`` (HIP kernels behind the C ABI declared in
+``include/tbsla_hip.h``).  This package loads it with ctypes and mirrors the
+reference's ``tbsla::cpp::Matrix`` API (src/cpp/include/tbsla/cpp/Matrix.hpp)
+for Python callers and tests: ``Matrix.fill_cqmat``, ``spmv``, ``Ax``,
+``a_axpx_``, ... with the same argument meaning.
+
+There is no CPU fallback: if the library is missing, or no GPU is visible,
+every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtbsla_hip.so")
+
+COO, SCOO, CSR, ELL, DENSE = 0, 1, 2, 3, 4
+FORMATS = {"COO": COO, "SCOO": SCOO, "CSR": CSR, "ELL": ELL, "DENSE": DENSE}
+GEOM_FIELDS = ("n_row", "n_col", "f_row", "f_col", "ln_row", "ln_col", "pr", "pc", "NR", "NC")
+
+# every entry point of include/tbsla_hip.h: (name, restype, argtypes)
+_vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+
+class Geometry(C.Structure):
+    _fields_ = ([(f, C.c_int) for f in GEOM_FIELDS] +
+                [("nnz", C.c_longlong), ("stored", C.c_longlong), ("max_col", C.c_int),
+                 ("format", C.c_int)])
+
+
+_GP = C.POINTER(Geometry)
+SIGNATURES = [
+    ("tbsla_hip_status_string", C.c_char_p, [C.c_int]),
+    ("tbsla_hip_last_error", C.c_char_p, []),
+    ("tbsla_hip_version", C.c_int, []),
+    ("tbsla_hip_malloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    ("tbsla_hip_free", C.c_int, [_vp]),
+    ("tbsla_hip_memcpy_htod", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("tbsla_hip_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("tbsla_hip_memcpy_dtod", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("tbsla_hip_stream_synchronize", C.c_int, [_vp]),
+    ("tbsla_hip_device_count", C.c_int, [_ip]),
+    ("tbsla_hip_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("tbsla_hip_destroy", C.c_int, [_vp]),
+    ("tbsla_hip_get_geometry", C.c_int, [_vp, _GP]),
+    ("tbsla_hip_fill_cdiag", C.c_int, [_vp] + [C.c_int] * 7 + [_vp]),
+    ("tbsla_hip_fill_cqmat", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, _vp]),
+    ("tbsla_hip_fill_unit_vector", C.c_int, [_vp, C.c_longlong, C.c_longlong, _vp]),
+    ("tbsla_hip_export", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("tbsla_hip_import", C.c_int, [_vp, C.c_int, _GP, _vp, _vp, _vp, _vp]),
+    ("tbsla_hip_convert", C.c_int, [_vp, C.c_int, _vp, _vp]),
+    ("tbsla_hip_Ax", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_a_axpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_AAxpAx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_AAxpAxpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_time_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
+    ("tbsla_hip_set_csr_lanes", C.c_int, [_vp, C.c_int]),
+    ("tbsla_hip_get_csr_lanes", C.c_int, [_vp, _ip]),
+]
+
+_lib = None
+
+
+class TbslaHipError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded libtbsla_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TbslaHipError(f"{LIB_PATH} is not built: run `python -m tbsla_amd.build` "
+                                "(or __graft_entry__.build())")
+        lib_ = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(lib_, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib_
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        L = lib()
+        raise TbslaHipError(f"{L.tbsla_hip_status_string(status).decode()}: "
+                            f"{L.tbsla_hip_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    st = lib().tbsla_hip_device_count(C.byref(n))
+    return n.value if st == 0 else 0
+
+
+class DeviceBuffer:
+    """Device memory owned through tbsla_hip_malloc / tbsla_hip_free."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().tbsla_hip_malloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tbsla_hip_free(self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().tbsla_hip_memcpy_htod(b.ptr, a.ctypes.data, a.nbytes, None))
+        return b
+
+    def to_numpy(self, dtype=np.float64, count: int | None = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().tbsla_hip_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, None))
+        return out
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if a.nbytes:
+            check(lib().tbsla_hip_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, None))
+
+
+def _dev(v, dtype=np.float64) -> DeviceBuffer:
+    return v if isinstance(v, DeviceBuffer) else DeviceBuffer.from_numpy(np.asarray(v, dtype))
+
+
+class Matrix:
+    """A matrix block on the GPU: the reference's tbsla::cpp::Matrix API.
+
+    Vectors may be numpy arrays (copied to the device per call) or
+    DeviceBuffers (used in place).  Results come back as numpy arrays.
+    """
+
+    def __init__(self, fmt: int, handle=None):
+        self.fmt = fmt
+        if handle is None:
+            h = C.c_void_p()
+            check(lib().tbsla_hip_create(C.byref(h), fmt))
+            handle = h.value
+        self.h = handle
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tbsla_hip_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+    # -- generators ---------------------------------------------------------
+    def fill_cqmat(self, n_row, n_col, c, q, seed_mult=1, pr=0, pc=0, NR=1, NC=1,
+                   value_seed=-1, stream=None):
+        check(lib().tbsla_hip_fill_cqmat(self.h, n_row, n_col, c, q, seed_mult, pr, pc, NR, NC,
+                                         value_seed, stream))
+        return self
+
+    def fill_cdiag(self, n_row, n_col, cdiag, pr=0, pc=0, NR=1, NC=1, stream=None):
+        check(lib().tbsla_hip_fill_cdiag(self.h, n_row, n_col, cdiag, pr, pc, NR, NC, stream))
+        return self
+
+    # -- geometry -----------------------------------------------------------
+    def geometry(self) -> Geometry:
+        g = Geometry()
+        check(lib().tbsla_hip_get_geometry(self.h, C.byref(g)))
+        return g
+
+    @property
+    def geom(self) -> dict:
+        g = self.geometry()
+        d = {f: getattr(g, f) for f in GEOM_FIELDS}
+        d["nnz"] = g.nnz
+        if self.fmt == ELL:
+            d["max_col"] = g.max_col
+        return d
+
+    def get_nnz(self) -> int:
+        return self.geometry().nnz
+
+    @property
+    def result_len(self) -> int:
+        g = self.geometry()
+        return g.n_row if self.fmt == COO else g.ln_row
+
+    def arrays(self):
+        """Numpy copies in the reference's layout (as oracle.Matrix.arrays)."""
+        g = self.geometry()
+        L = lib()
+        if self.fmt == CSR:
+            rp = np.zeros(g.ln_row + 1, np.int32)
+            ci = np.zeros(max(g.stored, 0), np.int32)
+            va = np.zeros(max(g.stored, 0), np.float64)
+            check(L.tbsla_hip_export(self.h, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, None))
+            return rp, ci, va
+        if self.fmt in (COO, SCOO):
+            n = max(g.stored, 0)
+            r, c, v = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+            check(L.tbsla_hip_export(self.h, r.ctypes.data, c.ctypes.data, v.ctypes.data, None))
+            return r, c, v
+        if self.fmt == ELL:
+            rows = g.ln_row if g.stored > 0 else 0
+            cols = np.zeros((rows, g.max_col), np.int32)
+            vals = np.zeros((rows, g.max_col))
+            if rows * g.max_col:
+                check(L.tbsla_hip_export(self.h, cols.ctypes.data, None, vals.ctypes.data, None))
+            return cols, vals
+        vals = np.zeros((g.ln_row, g.ln_col))
+        if g.stored > 0:
+            check(L.tbsla_hip_export(self.h, None, None, vals.ctypes.data, None))
+        return (vals,)
+
+    # -- SpMV ---------------------------------------------------------------
+    def spmv(self, v, vect_incr: int = 0) -> np.ndarray:
+        dv = _dev(v)
+        n = self.result_len
+        dr = DeviceBuffer(max(n, 1) * 8)
+        check(lib().tbsla_hip_spmv(self.h, dr.ptr, dv.ptr, vect_incr, None))
+        return dr.to_numpy(np.float64, n)
+
+    def Ax(self, r, v, vect_incr: int = 0) -> np.ndarray:
+        """Matrix::Ax into a copy of r (overwrite or accumulate per format)."""
+        dv = _dev(v)
+        r = np.asarray(r, np.float64)
+        dr = DeviceBuffer.from_numpy(r)
+        check(lib().tbsla_hip_Ax(self.h, dr.ptr, dv.ptr, vect_incr, None))
+        return dr.to_numpy(np.float64, len(r))
+
+    def a_axpx_(self, v, vect_incr: int = 0) -> np.ndarray:
+        dv = _dev(v)
+        n = self.result_len
+        dr, dt = DeviceBuffer(max(n, 1) * 8), DeviceBuffer(max(n, 1) * 8)
+        check(lib().tbsla_hip_a_axpx(self.h, dr.ptr, dv.ptr, dt.ptr, vect_incr, None))
+        return dr.to_numpy(np.float64, n)
+
+    def AAxpAx(self, r, v, buf, vect_incr: int = 0):
+        dr, dv, db = _dev(r), _dev(v), _dev(buf)
+        check(lib().tbsla_hip_AAxpAx(self.h, dr.ptr, dv.ptr, db.ptr, vect_incr, None))
+        return dr.to_numpy(np.float64, len(r)), db.to_numpy(np.float64, len(buf))
+
+    def AAxpAxpx(self, r, v, buf, vect_incr: int = 0):
+        dr, dv, db = _dev(r), _dev(v), _dev(buf)
+        check(lib().tbsla_hip_AAxpAxpx(self.h, dr.ptr, dv.ptr, db.ptr, vect_incr, None))
+        return dr.to_numpy(np.float64, len(r)), db.to_numpy(np.float64, len(buf))
+
+    def spmv_device(self, dr: DeviceBuffer, dv: DeviceBuffer, vect_incr: int = 0, stream=None):
+        check(lib().tbsla_hip_spmv(self.h, dr.ptr, dv.ptr, vect_incr, stream))
+
+    def time_spmv(self, dr: DeviceBuffer, dv: DeviceBuffer, iters: int, stream=None) -> float:
+        ms = C.c_float()
+        check(lib().tbsla_hip_time_spmv(self.h, dr.ptr, dv.ptr, iters, C.byref(ms), stream))
+        return ms.value
+
+    def set_csr_lanes(self, lanes: int) -> None:
+        check(lib().tbsla_hip_set_csr_lanes(self.h, lanes))
+
+    def get_csr_lanes(self) -> int:
+        x = C.c_int()
+        check(lib().tbsla_hip_get_csr_lanes(self.h, C.byref(x)))
+        return x.value
+
+    # -- conversions --------------------------------------------------------
+    def to(self, fmt: int) -> "Matrix":
+        """Conversion constructor <fmt>(const MatrixCOO&); self must be COO."""
+        m = Matrix(fmt)
+        check(lib().tbsla_hip_convert(m.h, fmt, self.h, None))
+        return m
+
+    @classmethod
+    def from_arrays(cls, fmt: int, geom: dict, a=None, b=None, v=None, stored=None) -> "Matrix":
+        """tbsla_hip_import: arrays in the reference layout."""
+        g = Geometry()
+        for f in GEOM_FIELDS:
+            setattr(g, f, int(geom[f]))
+        g.nnz = int(geom["nnz"])
+        g.max_col = int(geom.get("max_col", 0))
+        g.format = fmt
+        keep = []
+
+        def ptr(x, dt):
+            if x is None:
+                return None
+            x = np.ascontiguousarray(x, dt)
+            keep.append(x)
+            return x.ctypes.data
+        if stored is None:
+            stored = len(v) if v is not None and fmt in (COO, SCOO) else 0
+        g.stored = int(stored)
+        m = cls(fmt)
+        check(lib().tbsla_hip_import(m.h, fmt, C.byref(g), ptr(a, np.int32), ptr(b, np.int32),
+                                     ptr(v, np.float64), None))
+        return m

@@ -1,0 +1,369 @@
+// capi.hip -- handle lifecycle, memory helpers, import/export and the SpMV
+// entry points of the C ABI (include/tbsla_hip.h).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tbsla_hip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int status, const std::string &msg) {
+  set_error(msg);
+  return status;
+}
+
+int device_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(TBSLA_HIP_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(bytes) +
+                                         "): " + hipGetErrorString(e));
+  }
+  return TBSLA_HIP_SUCCESS;
+}
+
+int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
+  if (m->scratch_bytes >= bytes) return TBSLA_HIP_SUCCESS;
+  if (m->scratch) {
+    // the previous scratch may still be read by queued work
+    TBSLA_HIP_CHECK(hipDeviceSynchronize());
+    hipFree(m->scratch);
+    m->scratch = nullptr;
+    m->scratch_bytes = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 4096);
+  TBSLA_TRY(device_alloc(&m->scratch, want));
+  m->scratch_bytes = want;
+  return TBSLA_HIP_SUCCESS;
+}
+
+void release_arrays(tbsla_hip_matrix_s *m) {
+  if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32) hipDeviceSynchronize();
+  hipFree(m->rowptr);
+  hipFree(m->colidx);
+  hipFree(m->rowidx);
+  hipFree(m->values);
+  hipFree(m->values32);
+  m->rowptr = m->colidx = m->rowidx = nullptr;
+  m->values = nullptr;
+  m->values32 = nullptr;
+  m->has_arrays = 0;
+  m->stored = 0;
+}
+
+}  // namespace tbsla_hip
+
+using namespace tbsla_hip;
+
+static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+extern "C" {
+
+const char *tbsla_hip_status_string(int status) {
+  switch (status) {
+    case TBSLA_HIP_SUCCESS: return "success";
+    case TBSLA_HIP_ERR_INVALID: return "invalid argument";
+    case TBSLA_HIP_ERR_HIP: return "HIP runtime error";
+    case TBSLA_HIP_ERR_ALLOC: return "device allocation failed";
+    case TBSLA_HIP_ERR_UNSUPPORTED: return "unsupported for this format";
+    case TBSLA_HIP_ERR_RANGE: return "size beyond int32 indexing";
+  }
+  return "unknown status";
+}
+
+const char *tbsla_hip_last_error(void) { return g_last_error.c_str(); }
+
+int tbsla_hip_version(void) { return 1; }
+
+int tbsla_hip_malloc(void **d_ptr, size_t bytes) {
+  if (!d_ptr) return fail(TBSLA_HIP_ERR_INVALID, "null output pointer");
+  return device_alloc(d_ptr, bytes);
+}
+
+int tbsla_hip_free(void *d_ptr) {
+  TBSLA_HIP_CHECK(hipFree(d_ptr));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_memcpy_htod(void *d_dst, const void *h_src, size_t bytes, void *stream) {
+  if (bytes == 0) return TBSLA_HIP_SUCCESS;
+  TBSLA_HIP_CHECK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes, void *stream) {
+  if (bytes == 0) return TBSLA_HIP_SUCCESS;
+  TBSLA_HIP_CHECK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+  if (bytes == 0) return TBSLA_HIP_SUCCESS;
+  TBSLA_HIP_CHECK(
+      hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_stream_synchronize(void *stream) {
+  TBSLA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_device_count(int *count) {
+  if (!count) return fail(TBSLA_HIP_ERR_INVALID, "null output pointer");
+  *count = 0;
+  TBSLA_HIP_CHECK(hipGetDeviceCount(count));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_create(tbsla_hip_matrix_t *out, int format) {
+  if (!out) return fail(TBSLA_HIP_ERR_INVALID, "null output pointer");
+  if (format < TBSLA_HIP_COO || format > TBSLA_HIP_DENSE)
+    return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
+  *out = new tbsla_hip_matrix_s();
+  (*out)->format = format;
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_destroy(tbsla_hip_matrix_t m) {
+  if (!m) return TBSLA_HIP_SUCCESS;
+  release_arrays(m);
+  hipFree(m->scratch);
+  delete m;
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_get_geometry(tbsla_hip_matrix_t m, tbsla_hip_geometry *g) {
+  if (!m || !g) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  g->n_row = m->n_row;
+  g->n_col = m->n_col;
+  g->f_row = m->f_row;
+  g->f_col = m->f_col;
+  g->ln_row = m->ln_row;
+  g->ln_col = m->ln_col;
+  g->pr = m->pr;
+  g->pc = m->pc;
+  g->NR = m->NR;
+  g->NC = m->NC;
+  g->nnz = m->nnz;
+  g->stored = m->stored;
+  g->max_col = m->max_col;
+  g->format = m->format;
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_export(tbsla_hip_matrix_t m, int *h_a, int *h_b, double *h_v, void *stream) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  hipStream_t s = as_stream(stream);
+  auto d2h = [&](void *h, const void *d, size_t bytes) -> int {
+    if (!h || !d || bytes == 0) return TBSLA_HIP_SUCCESS;
+    TBSLA_HIP_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+    return TBSLA_HIP_SUCCESS;
+  };
+  const size_t st = (size_t)std::max(0LL, m->stored);
+  switch (m->format) {
+    case TBSLA_HIP_CSR:
+      if (m->rowptr) TBSLA_TRY(d2h(h_a, m->rowptr, ((size_t)m->ln_row + 1) * sizeof(int)));
+      else if (h_a) std::memset(h_a, 0, ((size_t)m->ln_row + 1) * sizeof(int));
+      TBSLA_TRY(d2h(h_b, m->colidx, st * sizeof(int)));
+      TBSLA_TRY(d2h(h_v, m->values, st * sizeof(double)));
+      break;
+    case TBSLA_HIP_COO:
+    case TBSLA_HIP_SCOO:
+      TBSLA_TRY(d2h(h_a, m->rowidx, st * sizeof(int)));
+      TBSLA_TRY(d2h(h_b, m->colidx, st * sizeof(int)));
+      TBSLA_TRY(d2h(h_v, m->values, st * sizeof(double)));
+      break;
+    case TBSLA_HIP_ELL: {
+      // column-major on the device, row-major (reference layout) on the host
+      if (!m->has_arrays) break;
+      const size_t rows = (size_t)m->ln_row, w = (size_t)m->max_col, slots = rows * w;
+      std::vector<int> cols(slots);
+      std::vector<double> vals(slots);
+      TBSLA_TRY(d2h(cols.data(), m->colidx, slots * sizeof(int)));
+      TBSLA_TRY(d2h(vals.data(), m->values, slots * sizeof(double)));
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      for (size_t i = 0; i < rows; i++)
+        for (size_t j = 0; j < w; j++) {
+          if (h_a) h_a[i * w + j] = cols[j * rows + i];
+          if (h_v) h_v[i * w + j] = vals[j * rows + i];
+        }
+      break;
+    }
+    case TBSLA_HIP_DENSE:
+      if (m->has_arrays)
+        TBSLA_TRY(d2h(h_v, m->values, (size_t)m->ln_row * (size_t)m->ln_col * sizeof(double)));
+      break;
+  }
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry *g,
+                     const int *h_a, const int *h_b, const double *h_v, void *stream) {
+  if (!m || !g) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  if (format < TBSLA_HIP_COO || format > TBSLA_HIP_DENSE)
+    return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
+  if (g->ln_row < 0 || g->ln_col < 0 || g->stored < 0)
+    return fail(TBSLA_HIP_ERR_INVALID, "bad geometry");
+  hipStream_t s = as_stream(stream);
+  release_arrays(m);
+  m->format = format;
+  m->n_row = g->n_row;
+  m->n_col = g->n_col;
+  m->f_row = g->f_row;
+  m->f_col = g->f_col;
+  m->ln_row = g->ln_row;
+  m->ln_col = g->ln_col;
+  m->pr = g->pr;
+  m->pc = g->pc;
+  m->NR = g->NR;
+  m->NC = g->NC;
+  m->nnz = g->nnz;
+  m->max_col = format == TBSLA_HIP_ELL ? g->max_col : 0;
+  m->row_sorted = 1;
+  auto h2d = [&](void **d, const void *h, size_t bytes) -> int {
+    TBSLA_TRY(device_alloc(d, bytes));
+    if (h && bytes) TBSLA_HIP_CHECK(hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, s));
+    return TBSLA_HIP_SUCCESS;
+  };
+  const size_t st = (size_t)g->stored;
+  switch (format) {
+    case TBSLA_HIP_CSR: {
+      if (!h_a) return fail(TBSLA_HIP_ERR_INVALID, "CSR import needs rowptr");
+      m->stored = h_a[g->ln_row];
+      TBSLA_TRY(h2d((void **)&m->rowptr, h_a, ((size_t)g->ln_row + 1) * sizeof(int)));
+      TBSLA_TRY(h2d((void **)&m->colidx, h_b, (size_t)m->stored * sizeof(int)));
+      TBSLA_TRY(h2d((void **)&m->values, h_v, (size_t)m->stored * sizeof(double)));
+      m->has_arrays = 1;
+      break;
+    }
+    case TBSLA_HIP_COO:
+    case TBSLA_HIP_SCOO: {
+      m->stored = g->stored;
+      for (size_t i = 1; h_a && i < st; i++)
+        if (h_a[i] < h_a[i - 1]) {
+          m->row_sorted = 0;
+          break;
+        }
+      TBSLA_TRY(h2d((void **)&m->rowidx, h_a, st * sizeof(int)));
+      TBSLA_TRY(h2d((void **)&m->colidx, h_b, st * sizeof(int)));
+      TBSLA_TRY(h2d((void **)&m->values, h_v, st * sizeof(double)));
+      m->has_arrays = st > 0;
+      break;
+    }
+    case TBSLA_HIP_ELL: {
+      const size_t rows = (size_t)g->ln_row, w = (size_t)std::max(0, g->max_col);
+      const size_t slots = rows * w;
+      std::vector<int> cols(slots);
+      std::vector<double> vals(slots);
+      for (size_t i = 0; i < rows; i++)
+        for (size_t j = 0; j < w; j++) {
+          cols[j * rows + i] = h_a ? h_a[i * w + j] : 0;
+          vals[j * rows + i] = h_v ? h_v[i * w + j] : 0.0;
+        }
+      m->stored = (long long)slots;
+      TBSLA_TRY(h2d((void **)&m->colidx, cols.data(), slots * sizeof(int)));
+      TBSLA_TRY(h2d((void **)&m->values, vals.data(), slots * sizeof(double)));
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      m->has_arrays = 1;
+      break;
+    }
+    case TBSLA_HIP_DENSE: {
+      const size_t cells = (size_t)g->ln_row * (size_t)g->ln_col;
+      m->stored = (long long)cells;
+      TBSLA_TRY(h2d((void **)&m->values, h_v, cells * sizeof(double)));
+      m->has_arrays = h_v != nullptr;
+      break;
+    }
+  }
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_Ax(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
+                 void *stream) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  return spmv_launch(m, d_r, d_v, vect_incr, false, as_stream(stream));
+}
+
+int tbsla_hip_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
+                   void *stream) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  return spmv_launch(m, d_r, d_v, vect_incr, true, as_stream(stream));
+}
+
+// Matrix::a_axpx_ (Matrix.cpp:5-14): tmp = spmv(v); tmp[i] += v[i] over
+// ln_col entries; r = spmv(tmp).
+int tbsla_hip_a_axpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_tmp,
+                     int vect_incr, void *stream) {
+  if (!m || !d_tmp) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  hipStream_t s = as_stream(stream);
+  TBSLA_TRY(spmv_launch(m, d_tmp, d_v, vect_incr, true, s));
+  TBSLA_TRY(launch_axpy_inplace(d_tmp, d_v, m->ln_col, s));
+  return spmv_launch(m, d_r, d_tmp, vect_incr, true, s);
+}
+
+// Matrix::AAxpAx (Matrix.cpp:16-23).
+int tbsla_hip_AAxpAx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_buf,
+                     int vect_incr, void *stream) {
+  if (!m || !d_buf) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  hipStream_t s = as_stream(stream);
+  TBSLA_TRY(spmv_launch(m, d_buf, d_v, vect_incr, false, s));
+  TBSLA_TRY(launch_axpy_inplace(d_buf, d_v, m->ln_col, s));
+  return spmv_launch(m, d_r, d_buf, vect_incr, false, s);
+}
+
+// Matrix::AAxpAxpx (Matrix.cpp:25-31).
+int tbsla_hip_AAxpAxpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, double *d_buf,
+                       int vect_incr, void *stream) {
+  TBSLA_TRY(tbsla_hip_AAxpAx(m, d_r, d_v, d_buf, vect_incr, stream));
+  return launch_axpy_inplace(d_r, d_v, m->ln_col, as_stream(stream));
+}
+
+int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
+                        float *ms, void *stream) {
+  if (!m || !ms || iters < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad argument");
+  hipStream_t s = as_stream(stream);
+  hipEvent_t e0, e1;
+  TBSLA_HIP_CHECK(hipEventCreate(&e0));
+  TBSLA_HIP_CHECK(hipEventCreate(&e1));
+  TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+  int st = TBSLA_HIP_SUCCESS;
+  for (int i = 0; i < iters && st == TBSLA_HIP_SUCCESS; i++)
+    st = spmv_launch(m, d_r, d_v, 0, true, s);
+  TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+  TBSLA_HIP_CHECK(hipEventSynchronize(e1));
+  TBSLA_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return st;
+}
+
+int tbsla_hip_set_csr_lanes(tbsla_hip_matrix_t m, int lanes) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  if (lanes != 0 && (lanes < 2 || lanes > 64 || (lanes & (lanes - 1))))
+    return fail(TBSLA_HIP_ERR_INVALID, "lanes must be 0 or a power of two in 2..64");
+  m->lanes_per_row = lanes;
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_get_csr_lanes(tbsla_hip_matrix_t m, int *lanes) {
+  if (!m || !lanes) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  *lanes = pick_csr_lanes(m);
+  return TBSLA_HIP_SUCCESS;
+}
+
+}  // extern "C"

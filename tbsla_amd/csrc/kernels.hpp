@@ -1,0 +1,30 @@
+// kernels.hpp -- internal interfaces between the .hip translation units.
+#pragma once
+
+#include "common.hpp"
+
+namespace tbsla_hip {
+
+// scan.hip: prefix sum of n values (in may equal out); *d_total (device,
+// nullable) receives the sum.  workspace: scan_workspace_bytes(n) bytes.
+template <typename T>
+int scan(const T *in, T *out, long long n, bool inclusive, long long *d_total, void *workspace,
+         hipStream_t s);
+size_t scan_workspace_bytes(long long n);
+
+// spmv.hip
+int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
+                bool zero_first, hipStream_t s);
+int pick_csr_lanes(const tbsla_hip_matrix_s *m);
+
+// vec.hip
+int launch_fill(double *d, long long n, double v, hipStream_t s);
+int launch_axpy_inplace(double *y, const double *x, long long n, hipStream_t s);  // y += x
+
+// sort.hip: stable LSD radix sort of 64-bit keys carrying a 32-bit payload
+// (the original position); key_bits significant bits.  Buffers of n
+// elements; the result lands in keys/payload.
+int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int key_bits,
+                     hipStream_t s);
+
+}  // namespace tbsla_hip

@@ -1,0 +1,335 @@
+// spmv.hip -- y = A x for the five formats (Matrix::Ax, Matrix.hpp:15).
+//
+// All kernels are HBM-bound streams over the matrix arrays with gathers of x
+// (which stays in L2 / the 256 MB MALL):
+//   CSR    L lanes per row (L = 2..64), loads of values/colidx coalesced
+//          across the group and marked non-temporal; shuffle reduction.
+//   ELL    one lane per row over column-major slots: every slot load is a
+//          contiguous 64-row stripe.
+//   COO/SCOO  row-sorted triplets: each wavefront reduces a 512-entry chunk
+//          with a segmented scan; rows complete inside a chunk are written
+//          directly, rows crossing chunk borders are summed in chunk order by
+//          a fix-up kernel -- deterministic.  Unsorted triplets use FP64
+//          atomics (unordered, as the reference's OpenMP array reduction).
+//   DENSE  one wavefront per row (GEMV).
+// Floating point: FMA accumulation; the per-row order differs from the
+// reference's sequential loop only in the CSR/COO/DENSE lane split.
+#include <climits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tbsla_hip {
+
+namespace {
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_csr_spmv(long long n_rows, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ colidx,
+                                                     const double *__restrict__ vals,
+                                                     const double *__restrict__ x, int f_col,
+                                                     double *__restrict__ y) {
+  const long long row = ((long long)blockIdx.x * kBlock + threadIdx.x) / L;
+  const int lane = threadIdx.x & (L - 1);
+  if (row >= n_rows) return;
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  double sum = 0.0;
+  for (int j = beg + lane; j < end; j += L)
+    sum = fma(ldg_stream(vals + j), x[ldg_stream(colidx + j) - f_col], sum);
+  sum = group_sum<L>(sum);
+  if (lane == 0) y[row] = sum;
+}
+
+// MatrixELL::Ax (MatrixELL.cpp:181-205): a negative local column (padding
+// of a block with f_col > 0) reads x[0], multiplied by the zero padding.
+__global__ __launch_bounds__(kBlock) void k_ell_spmv(long long n_rows, int max_col,
+                                                     const int *__restrict__ cols,
+                                                     const double *__restrict__ vals,
+                                                     const double *__restrict__ x, int f_col,
+                                                     double *__restrict__ y) {
+  const long long row = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n_rows) return;
+  double sum = 0.0;
+#pragma unroll 4
+  for (int j = 0; j < max_col; j++) {
+    const size_t s = (size_t)j * n_rows + row;
+    int idx = ldg_stream(cols + s) - f_col;
+    idx = idx < 0 ? 0 : idx;
+    sum = fma(ldg_stream(vals + s), x[idx], sum);
+  }
+  y[row] = sum;
+}
+
+constexpr int kSegSteps = 8;
+constexpr int kSegChunk = kWave * kSegSteps;
+
+// Segmented reduction of row-sorted triplets; r[row + row_off] +=
+// val * x[col + col_off].  One wavefront per chunk of kSegChunk entries.
+// Each chunk reports its first segment (head) and, when it has more than one,
+// its last (tail); the fix-up kernel completes those.
+__global__ __launch_bounds__(kBlock) void k_coo_seg(long long nnz, const int *__restrict__ rows,
+                                                    const int *__restrict__ cols,
+                                                    const double *__restrict__ vals,
+                                                    const double *__restrict__ x, int row_off,
+                                                    int col_off, double *__restrict__ y,
+                                                    int *__restrict__ head_key,
+                                                    double *__restrict__ head_val,
+                                                    int *__restrict__ tail_key,
+                                                    double *__restrict__ tail_val) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long long chunk = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const long long base = chunk * kSegChunk;
+  if (base >= nnz) return;
+  const long long end = min(nnz, base + kSegChunk);
+  const int first_key = rows[base];
+  int carry_key = -1;
+  double carry_val = 0.0;
+  bool head_written = false;
+  for (int t = 0; t < kSegSteps; t++) {
+    const long long e = base + (long long)t * kWave + lane;
+    const bool valid = e < end;
+    if (!__any(valid)) break;
+    const int key = valid ? rows[e] : INT_MAX;
+    double p = valid ? ldg_stream(vals + e) * x[ldg_stream(cols + e) + col_off] : 0.0;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const double o = __shfl_up(p, off, kWave);
+      const int ok = __shfl_up(key, off, kWave);
+      if (lane >= off && ok == key) p += o;
+    }
+    // the carried segment either continues into this step or is complete
+    const int key0 = __shfl(key, 0, kWave);
+    if (carry_key >= 0) {
+      if (key0 == carry_key) {
+        if (key == carry_key) p += carry_val;
+      } else if (lane == 0) {
+        if (carry_key == first_key) {
+          head_key[chunk] = carry_key;
+          head_val[chunk] = carry_val;
+        } else {
+          y[carry_key + row_off] += carry_val;
+        }
+      }
+      if (key0 != carry_key && carry_key == first_key) head_written = true;
+    }
+    const int next_key = __shfl_down(key, 1, kWave);
+    const bool seg_end = valid && lane != kWave - 1 && next_key != key;
+    if (seg_end) {
+      if (key == first_key) {
+        head_key[chunk] = key;
+        head_val[chunk] = p;
+      } else {
+        y[key + row_off] += p;
+      }
+    }
+    if (__any(seg_end && key == first_key)) head_written = true;
+    carry_key = __shfl(key, kWave - 1, kWave);
+    carry_val = __shfl(p, kWave - 1, kWave);
+    if (carry_key == INT_MAX) carry_key = -1;  // partial last step: nothing carried
+  }
+  if (lane == 0) {
+    int tk = -1;
+    double tv = 0.0;
+    if (carry_key >= 0) {
+      if (carry_key == first_key && !head_written) {
+        head_key[chunk] = carry_key;
+        head_val[chunk] = carry_val;
+      } else {
+        tk = carry_key;
+        tv = carry_val;
+      }
+    }
+    tail_key[chunk] = tk;
+    tail_val[chunk] = tv;
+  }
+}
+
+// Completes the rows that cross chunk borders, each summed by one thread in
+// chunk order.
+__global__ __launch_bounds__(kBlock) void k_coo_fixup(long long nchunks, const int *__restrict__ head_key,
+                                                      const double *__restrict__ head_val,
+                                                      const int *__restrict__ tail_key,
+                                                      const double *__restrict__ tail_val,
+                                                      int row_off, double *__restrict__ y) {
+  const long long c = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nchunks) return;
+  const int hk = head_key[c];
+  const int prev_last = c > 0 ? (tail_key[c - 1] >= 0 ? tail_key[c - 1] : head_key[c - 1]) : -1;
+  if (hk != prev_last) {
+    double sum = head_val[c];
+    long long cc = c;
+    while (tail_key[cc] < 0 && cc + 1 < nchunks && head_key[cc + 1] == hk) {
+      cc++;
+      sum += head_val[cc];
+    }
+    y[hk + row_off] += sum;
+  }
+  const int tk = tail_key[c];
+  if (tk >= 0) {
+    double sum = tail_val[c];
+    long long cc = c;
+    while (cc + 1 < nchunks && head_key[cc + 1] == tk) {
+      cc++;
+      sum += head_val[cc];
+      if (tail_key[cc] >= 0) break;
+    }
+    y[tk + row_off] += sum;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_coo_atomic(long long nnz, const int *__restrict__ rows,
+                                                       const int *__restrict__ cols,
+                                                       const double *__restrict__ vals,
+                                                       const double *__restrict__ x, int row_off,
+                                                       int col_off, double *__restrict__ y) {
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < nnz;
+       e += (long long)gridDim.x * kBlock)
+    atomicAdd(y + rows[e] + row_off, vals[e] * x[cols[e] + col_off]);
+}
+
+// MatrixDENSE::Ax (MatrixDENSE.cpp:77-86): r[i] += A[i,:] . x.
+__global__ __launch_bounds__(kBlock) void k_dense_gemv(long long n_rows, long long n_cols,
+                                                       const double *__restrict__ A,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+  const long long row = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= n_rows) return;
+  const double *a = A + (size_t)row * n_cols;
+  double sum = 0.0;
+  for (long long j = lane; j < n_cols; j += kWave) sum = fma(ldg_stream(a + j), x[j], sum);
+  sum = group_sum<kWave>(sum);
+  if (lane == 0) y[row] += sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ d, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    d[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_inplace(double *__restrict__ y,
+                                                        const double *__restrict__ x, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    y[i] += x[i];
+}
+
+unsigned stride_grid(long long n) {
+  long long g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <int L>
+void launch_csr(const tbsla_hip_matrix_s *m, double *y, const double *x, hipStream_t s) {
+  long long threads = (long long)m->ln_row * L;
+  k_csr_spmv<L><<<grid_for(threads), kBlock, 0, s>>>(m->ln_row, m->rowptr, m->colidx, m->values,
+                                                      x, m->f_col, y);
+}
+
+}  // namespace
+
+int launch_fill(double *d, long long n, double v, hipStream_t s) {
+  if (n <= 0) return TBSLA_HIP_SUCCESS;
+  k_fill<<<stride_grid(n), kBlock, 0, s>>>(d, n, v);
+  TBSLA_HIP_LAUNCH_CHECK();
+  return TBSLA_HIP_SUCCESS;
+}
+
+int launch_axpy_inplace(double *y, const double *x, long long n, hipStream_t s) {
+  if (n <= 0) return TBSLA_HIP_SUCCESS;
+  k_add_inplace<<<stride_grid(n), kBlock, 0, s>>>(y, x, n);
+  TBSLA_HIP_LAUNCH_CHECK();
+  return TBSLA_HIP_SUCCESS;
+}
+
+// Lanes per row from the mean row length (half a row per lane pass,
+// clamped to 2..64); tbsla_hip_set_csr_lanes overrides it.
+int pick_csr_lanes(const tbsla_hip_matrix_s *m) {
+  if (m->lanes_per_row > 0) return m->lanes_per_row;
+  double mean = m->ln_row > 0 ? (double)m->stored / m->ln_row : 1.0;
+  int L = 2;
+  while (L < 64 && L < mean / 2) L <<= 1;
+  return L;
+}
+
+int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
+                bool zero_first, hipStream_t s) {
+  switch (m->format) {
+    case TBSLA_HIP_CSR: {
+      // MatrixCSR::Ax returns when nnz == 0 (MatrixCSR.cpp:210-211)
+      if (m->nnz == 0 || m->ln_row == 0) {
+        if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
+        return TBSLA_HIP_SUCCESS;
+      }
+      switch (pick_csr_lanes(m)) {
+        case 2: launch_csr<2>(m, d_r, d_v, s); break;
+        case 4: launch_csr<4>(m, d_r, d_v, s); break;
+        case 8: launch_csr<8>(m, d_r, d_v, s); break;
+        case 16: launch_csr<16>(m, d_r, d_v, s); break;
+        case 32: launch_csr<32>(m, d_r, d_v, s); break;
+        default: launch_csr<64>(m, d_r, d_v, s); break;
+      }
+      TBSLA_HIP_LAUNCH_CHECK();
+      return TBSLA_HIP_SUCCESS;
+    }
+    case TBSLA_HIP_ELL: {
+      // MatrixELL.cpp:182-183
+      if (m->nnz == 0 || m->max_col == 0 || !m->has_arrays || m->ln_row == 0) {
+        if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
+        return TBSLA_HIP_SUCCESS;
+      }
+      k_ell_spmv<<<grid_for(m->ln_row), kBlock, 0, s>>>(m->ln_row, m->max_col, m->colidx,
+                                                         m->values, d_v, m->f_col, d_r);
+      TBSLA_HIP_LAUNCH_CHECK();
+      return TBSLA_HIP_SUCCESS;
+    }
+    case TBSLA_HIP_COO:
+    case TBSLA_HIP_SCOO: {
+      const bool coo = m->format == TBSLA_HIP_COO;
+      const long long len = coo ? m->n_row : m->ln_row;
+      if (zero_first) TBSLA_TRY(launch_fill(d_r, len, 0.0, s));
+      // SCOO returns when nnz == 0 (MatrixSCOO.cpp:155-157); COO loops over nnz
+      const long long n = coo ? m->nnz : (m->nnz == 0 ? 0 : m->stored);
+      if (n <= 0 || !m->has_arrays) return TBSLA_HIP_SUCCESS;
+      const int row_off = coo ? vect_incr : vect_incr - m->f_row;
+      const int col_off = coo ? 0 : -m->f_col;
+      if (!m->row_sorted) {
+        k_coo_atomic<<<stride_grid(n), kBlock, 0, s>>>(n, m->rowidx, m->colidx, m->values, d_v,
+                                                        row_off, col_off, d_r);
+        TBSLA_HIP_LAUNCH_CHECK();
+        return TBSLA_HIP_SUCCESS;
+      }
+      const long long nchunks = (n + kSegChunk - 1) / kSegChunk;
+      TBSLA_TRY(ensure_scratch(m, (size_t)nchunks * 24 + 256));
+      char *sc = static_cast<char *>(m->scratch);
+      int *head_key = reinterpret_cast<int *>(sc);
+      int *tail_key = head_key + nchunks;
+      size_t off = ((size_t)nchunks * 8 + 15) & ~(size_t)15;
+      double *head_val = reinterpret_cast<double *>(sc + off);
+      double *tail_val = head_val + nchunks;
+      k_coo_seg<<<grid_for(nchunks * kWave), kBlock, 0, s>>>(n, m->rowidx, m->colidx, m->values,
+                                                             d_v, row_off, col_off, d_r, head_key,
+                                                             head_val, tail_key, tail_val);
+      TBSLA_HIP_LAUNCH_CHECK();
+      k_coo_fixup<<<grid_for(nchunks), kBlock, 0, s>>>(nchunks, head_key, head_val, tail_key,
+                                                       tail_val, row_off, d_r);
+      TBSLA_HIP_LAUNCH_CHECK();
+      return TBSLA_HIP_SUCCESS;
+    }
+    case TBSLA_HIP_DENSE: {
+      if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
+      // MatrixDENSE.cpp:78-79
+      if (m->nnz == 0 || !m->has_arrays || m->ln_row == 0) return TBSLA_HIP_SUCCESS;
+      k_dense_gemv<<<grid_for((long long)m->ln_row * kWave), kBlock, 0, s>>>(
+          m->ln_row, m->ln_col, m->values, d_v, d_r);
+      TBSLA_HIP_LAUNCH_CHECK();
+      return TBSLA_HIP_SUCCESS;
+    }
+  }
+  return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
+}
+
+}  // namespace tbsla_hip
