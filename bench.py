@@ -85,7 +85,7 @@ def main():
     gbs = nbytes / (per * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_csr_spmv<{m.get_csr_lanes()}>", "bytes_per_launch": nbytes,
+            "kernel": "k_csr_spmv<%d,%d>" % m.get_csr_schedule(), "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
@@ -94,8 +94,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": "csr_spmv_cqmat", "format": "CSR", "n": N, "c": C, "q": Q,
                        "seed_mult": 0, "values": "U[0,1) seed 0", "x": "U[0,1) seed 1",
-                       "stored_nnz": g.stored, "nnz_field": g.nnz, "lanes_per_row":
-                       m.get_csr_lanes(), "fill_s": round(t_fill, 3)},
+                       "stored_nnz": g.stored, "nnz_field": g.nnz, "csr_schedule":
+                       list(m.get_csr_schedule()), "fill_s": round(t_fill, 3)},
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line))
 

@@ -124,6 +124,12 @@ int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry 
  * equal keys unspecified). */
 int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_matrix_t coo, void *stream);
 
+/* Device pointers of the block's arrays (owned by the handle; valid until
+ * the next fill/import/convert or destroy), in the device layout: CSR
+ * a = rowptr, b = colidx; ELL b = columns (column-major); COO/SCOO a = row,
+ * b = col; v = values.  For zero-copy interoperation. */
+int tbsla_hip_device_arrays(tbsla_hip_matrix_t m, void **d_a, void **d_b, void **d_v);
+
 /* ---- SpMV and composites ------------------------------------------------ */
 /* Matrix::Ax (Matrix.hpp:15): CSR and ELL overwrite r (MatrixCSR.cpp:209-233,
  * MatrixELL.cpp:181-205); COO, SCOO and DENSE add into r
@@ -148,10 +154,16 @@ int tbsla_hip_AAxpAxpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, dou
  * `stream`; *ms = total milliseconds.  For measurement only. */
 int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                         float *ms, void *stream);
-/* CSR SpMV schedule: lanes cooperating on one row (2..64, power of two);
- * 0 restores the choice from the mean row length. */
-int tbsla_hip_set_csr_lanes(tbsla_hip_matrix_t m, int lanes);
-int tbsla_hip_get_csr_lanes(tbsla_hip_matrix_t m, int *lanes);
+/* CSR SpMV schedule (no reference counterpart): `lanes` cooperating on one
+ * row (1..64, power of two) and `rows` each lane group keeps in flight
+ * (1, 2 or 4); 0 restores the choice from the mean row length. */
+int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows);
+int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows);
+/* Autotuner (no reference counterpart): times every CSR schedule on this
+ * block with d_v / d_r (`iters` launches each) and keeps the fastest; other
+ * formats have a single schedule.  *best_us receives its time per SpMV. */
+int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
+                       float *best_us, void *stream);
 
 #ifdef __cplusplus
 }

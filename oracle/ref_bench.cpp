@@ -72,12 +72,16 @@ int main(int argc, char **argv) {
   }
   std::sort(per.begin(), per.end());
   double med = per[per.size() / 2];
+  std::fflush(stdout);
   std::printf(
       "{\"format\":\"%s\",\"n\":%d,\"c\":%d,\"q\":%g,\"NR\":%d,\"NC\":%d,\"nnz_field\":%ld,"
       "\"time_op\":%.9g,\"time_fill\":%.6g,\"iters\":%d,\"reps\":%d,\"omp_threads\":%d,"
       "\"checksum\":%.17g}\n",
       fmt.c_str(), n, c, q, NR, NC, m->get_nnz(), med, t_fill, iters, reps, omp_get_max_threads(),
       checksum);
-  delete m;
+  std::fflush(stdout);
+  // tbsla::cpp::Matrix has no virtual destructor (Matrix.hpp:11-57)
+  if (auto *x = dynamic_cast<tbsla::cpp::MatrixCSR *>(m)) delete x;
+  else if (auto *x = dynamic_cast<tbsla::cpp::MatrixELL *>(m)) delete x;
   return 0;
 }

@@ -55,14 +55,17 @@ SIGNATURES = [
     ("tbsla_hip_export", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("tbsla_hip_import", C.c_int, [_vp, C.c_int, _GP, _vp, _vp, _vp, _vp]),
     ("tbsla_hip_convert", C.c_int, [_vp, C.c_int, _vp, _vp]),
+    ("tbsla_hip_device_arrays", C.c_int, [_vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p)]),
     ("tbsla_hip_Ax", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_a_axpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_AAxpAx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_AAxpAxpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_time_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
-    ("tbsla_hip_set_csr_lanes", C.c_int, [_vp, C.c_int]),
-    ("tbsla_hip_get_csr_lanes", C.c_int, [_vp, _ip]),
+    ("tbsla_hip_set_csr_schedule", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("tbsla_hip_get_csr_schedule", C.c_int, [_vp, _ip, _ip]),
+    ("tbsla_hip_autotune", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
 ]
 
 _lib = None
@@ -270,13 +273,24 @@ class Matrix:
         check(lib().tbsla_hip_time_spmv(self.h, dr.ptr, dv.ptr, iters, C.byref(ms), stream))
         return ms.value
 
-    def set_csr_lanes(self, lanes: int) -> None:
-        check(lib().tbsla_hip_set_csr_lanes(self.h, lanes))
+    def device_arrays(self):
+        a, b, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().tbsla_hip_device_arrays(self.h, C.byref(a), C.byref(b), C.byref(v)))
+        return a.value, b.value, v.value
 
-    def get_csr_lanes(self) -> int:
-        x = C.c_int()
-        check(lib().tbsla_hip_get_csr_lanes(self.h, C.byref(x)))
-        return x.value
+    def set_csr_schedule(self, lanes: int, rows: int = 0) -> None:
+        check(lib().tbsla_hip_set_csr_schedule(self.h, lanes, rows))
+
+    def get_csr_schedule(self) -> tuple[int, int]:
+        a, b = C.c_int(), C.c_int()
+        check(lib().tbsla_hip_get_csr_schedule(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, dr: DeviceBuffer, dv: DeviceBuffer, iters: int = 10,
+                 stream=None) -> float:
+        us = C.c_float()
+        check(lib().tbsla_hip_autotune(self.h, dr.ptr, dv.ptr, iters, C.byref(us), stream))
+        return us.value
 
     # -- conversions --------------------------------------------------------
     def to(self, fmt: int) -> "Matrix":

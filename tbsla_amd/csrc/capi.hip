@@ -293,6 +293,14 @@ int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry 
   return TBSLA_HIP_SUCCESS;
 }
 
+int tbsla_hip_device_arrays(tbsla_hip_matrix_t m, void **d_a, void **d_b, void **d_v) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  if (d_a) *d_a = m->format == TBSLA_HIP_CSR ? (void *)m->rowptr : (void *)m->rowidx;
+  if (d_b) *d_b = m->colidx;
+  if (d_v) *d_v = m->values;
+  return TBSLA_HIP_SUCCESS;
+}
+
 int tbsla_hip_Ax(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
                  void *stream) {
   if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
@@ -352,17 +360,51 @@ int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, in
   return st;
 }
 
-int tbsla_hip_set_csr_lanes(tbsla_hip_matrix_t m, int lanes) {
+int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows) {
   if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
-  if (lanes != 0 && (lanes < 2 || lanes > 64 || (lanes & (lanes - 1))))
-    return fail(TBSLA_HIP_ERR_INVALID, "lanes must be 0 or a power of two in 2..64");
+  if (lanes != 0 && (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))))
+    return fail(TBSLA_HIP_ERR_INVALID, "lanes must be 0 or a power of two in 1..64");
+  if (rows != 0 && rows != 1 && rows != 2 && rows != 4)
+    return fail(TBSLA_HIP_ERR_INVALID, "rows must be 0, 1, 2 or 4");
   m->lanes_per_row = lanes;
+  m->rows_per_group = rows;
   return TBSLA_HIP_SUCCESS;
 }
 
-int tbsla_hip_get_csr_lanes(tbsla_hip_matrix_t m, int *lanes) {
-  if (!m || !lanes) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
-  *lanes = pick_csr_lanes(m);
+int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  int L = pick_csr_lanes(m);
+  if (lanes) *lanes = L;
+  if (rows) *rows = pick_csr_rows(m, L);
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
+                       float *best_us, void *stream) {
+  if (!m || iters < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad argument");
+  float ms = 0;
+  if (m->format != TBSLA_HIP_CSR) {
+    TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+    if (best_us) *best_us = ms * 1000.0f / iters;
+    return TBSLA_HIP_SUCCESS;
+  }
+  int bestL = 0, bestR = 0;
+  float best = 0;
+  for (int L = 1; L <= 64; L <<= 1)
+    for (int R : {1, 2, 4}) {
+      m->lanes_per_row = L;
+      m->rows_per_group = R;
+      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
+      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+      if (bestL == 0 || ms < best) {
+        best = ms;
+        bestL = L;
+        bestR = R;
+      }
+    }
+  m->lanes_per_row = bestL;
+  m->rows_per_group = bestR;
+  if (best_us) *best_us = best * 1000.0f / iters;
   return TBSLA_HIP_SUCCESS;
 }
 

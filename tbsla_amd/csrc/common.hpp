@@ -198,6 +198,7 @@ struct tbsla_hip_matrix_s {
   float *values32 = nullptr; // FP32 copy of values for the mixed-precision forms
   // SpMV schedule
   int lanes_per_row = 0;     // CSR: 0 = pick from the mean row length
+  int rows_per_group = 0;    // CSR: rows each lane group keeps in flight (0 = pick)
   // scratch owned by the record (reused across calls; grows on demand)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -16,6 +16,7 @@ size_t scan_workspace_bytes(long long n);
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
                 bool zero_first, hipStream_t s);
 int pick_csr_lanes(const tbsla_hip_matrix_s *m);
+int pick_csr_rows(const tbsla_hip_matrix_s *m, int lanes);
 
 // vec.hip
 int launch_fill(double *d, long long n, double v, hipStream_t s);

@@ -2,8 +2,9 @@
 //
 // All kernels are HBM-bound streams over the matrix arrays with gathers of x
 // (which stays in L2 / the 256 MB MALL):
-//   CSR    L lanes per row (L = 2..64), loads of values/colidx coalesced
-//          across the group and marked non-temporal; shuffle reduction.
+//   CSR    L lanes per row (L = 1..64), R rows per lane group in flight;
+//          loads of values/colidx coalesced and non-temporal; shuffle
+//          reduction.
 //   ELL    one lane per row over column-major slots: every slot load is a
 //          contiguous 64-row stripe.
 //   COO/SCOO  row-sorted triplets: each wavefront reduces a 512-entry chunk
@@ -23,21 +24,60 @@ namespace tbsla_hip {
 
 namespace {
 
-template <int L>
+// L lanes per row, R rows per lane group: a wavefront owns 64/L * R
+// consecutive rows; at step r its groups take the next 64/L rows, so every
+// step's loads cover one contiguous stretch of colidx/values, and the R steps'
+// loads are all in flight before the first x gather is consumed.
+template <int L, int R>
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(long long n_rows, const int *__restrict__ rowptr,
                                                      const int *__restrict__ colidx,
                                                      const double *__restrict__ vals,
                                                      const double *__restrict__ x, int f_col,
                                                      double *__restrict__ y) {
-  const long long row = ((long long)blockIdx.x * kBlock + threadIdx.x) / L;
+  constexpr int G = kWave / L;
   const int lane = threadIdx.x & (L - 1);
-  if (row >= n_rows) return;
-  const int beg = rowptr[row], end = rowptr[row + 1];
-  double sum = 0.0;
-  for (int j = beg + lane; j < end; j += L)
-    sum = fma(ldg_stream(vals + j), x[ldg_stream(colidx + j) - f_col], sum);
-  sum = group_sum<L>(sum);
-  if (lane == 0) y[row] = sum;
+  const int g = (threadIdx.x & (kWave - 1)) / L;
+  const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const long long base = wave * (G * R);
+  if (base >= n_rows) return;
+  int beg[R], end[R];
+  int len = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const long long row = base + r * G + g;
+    if (row < n_rows) {
+      beg[r] = rowptr[row];
+      end[r] = rowptr[row + 1];
+    } else {
+      beg[r] = end[r] = 0;
+    }
+    len = max(len, end[r] - beg[r]);
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, kWave));
+  const int trips = (len + L - 1) / L;
+  double sum[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) sum[r] = 0.0;
+  for (int t = 0; t < trips; t++) {
+    int c[R];
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int j = beg[r] + t * L + lane;
+      const bool in = j < end[r];
+      c[r] = in ? ldg_stream(colidx + j) - f_col : 0;
+      v[r] = in ? ldg_stream(vals + j) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) sum[r] = fma(v[r], x[c[r]], sum[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const double s = group_sum<L>(sum[r]);
+    const long long row = base + r * G + g;
+    if (lane == 0 && row < n_rows) y[row] = s;
+  }
 }
 
 // MatrixELL::Ax (MatrixELL.cpp:181-205): a negative local column (padding
@@ -222,11 +262,21 @@ unsigned stride_grid(long long n) {
   return (unsigned)g;
 }
 
+template <int L, int R>
+void launch_csr_lr(const tbsla_hip_matrix_s *m, double *y, const double *x, hipStream_t s) {
+  const long long rows_per_wave = (long long)(kWave / L) * R;
+  const long long waves = (m->ln_row + rows_per_wave - 1) / rows_per_wave;
+  k_csr_spmv<L, R><<<grid_for(waves * kWave), kBlock, 0, s>>>(m->ln_row, m->rowptr, m->colidx,
+                                                               m->values, x, m->f_col, y);
+}
+
 template <int L>
-void launch_csr(const tbsla_hip_matrix_s *m, double *y, const double *x, hipStream_t s) {
-  long long threads = (long long)m->ln_row * L;
-  k_csr_spmv<L><<<grid_for(threads), kBlock, 0, s>>>(m->ln_row, m->rowptr, m->colidx, m->values,
-                                                      x, m->f_col, y);
+void launch_csr(const tbsla_hip_matrix_s *m, int R, double *y, const double *x, hipStream_t s) {
+  switch (R) {
+    case 1: launch_csr_lr<L, 1>(m, y, x, s); break;
+    case 2: launch_csr_lr<L, 2>(m, y, x, s); break;
+    default: launch_csr_lr<L, 4>(m, y, x, s); break;
+  }
 }
 
 }  // namespace
@@ -245,14 +295,19 @@ int launch_axpy_inplace(double *y, const double *x, long long n, hipStream_t s) 
   return TBSLA_HIP_SUCCESS;
 }
 
-// Lanes per row from the mean row length (half a row per lane pass,
-// clamped to 2..64); tbsla_hip_set_csr_lanes overrides it.
+// Lanes per row from the mean row length (two entries per lane per row,
+// clamped to 1..64) unless set; rows per group from the lanes.
 int pick_csr_lanes(const tbsla_hip_matrix_s *m) {
   if (m->lanes_per_row > 0) return m->lanes_per_row;
   double mean = m->ln_row > 0 ? (double)m->stored / m->ln_row : 1.0;
-  int L = 2;
+  int L = 1;
   while (L < 64 && L < mean / 2) L <<= 1;
   return L;
+}
+
+int pick_csr_rows(const tbsla_hip_matrix_s *m, int L) {
+  if (m->rows_per_group > 0) return m->rows_per_group;
+  return L >= 64 ? 1 : (L >= 32 ? 2 : 4);
 }
 
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
@@ -264,13 +319,15 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
         return TBSLA_HIP_SUCCESS;
       }
-      switch (pick_csr_lanes(m)) {
-        case 2: launch_csr<2>(m, d_r, d_v, s); break;
-        case 4: launch_csr<4>(m, d_r, d_v, s); break;
-        case 8: launch_csr<8>(m, d_r, d_v, s); break;
-        case 16: launch_csr<16>(m, d_r, d_v, s); break;
-        case 32: launch_csr<32>(m, d_r, d_v, s); break;
-        default: launch_csr<64>(m, d_r, d_v, s); break;
+      const int L = pick_csr_lanes(m), R = pick_csr_rows(m, L);
+      switch (L) {
+        case 1: launch_csr<1>(m, R, d_r, d_v, s); break;
+        case 2: launch_csr<2>(m, R, d_r, d_v, s); break;
+        case 4: launch_csr<4>(m, R, d_r, d_v, s); break;
+        case 8: launch_csr<8>(m, R, d_r, d_v, s); break;
+        case 16: launch_csr<16>(m, R, d_r, d_v, s); break;
+        case 32: launch_csr<32>(m, R, d_r, d_v, s); break;
+        default: launch_csr<64>(m, R, d_r, d_v, s); break;
       }
       TBSLA_HIP_LAUNCH_CHECK();
       return TBSLA_HIP_SUCCESS;

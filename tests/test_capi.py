@@ -52,8 +52,9 @@ def test_status_strings_and_bad_arguments():
     assert L.tbsla_hip_create(ctypes.byref(h), 99) == 1
     assert b"format" in L.tbsla_hip_last_error()
     assert L.tbsla_hip_create(ctypes.byref(h), T.CSR) == 0
-    assert L.tbsla_hip_set_csr_lanes(h, 3) == 1
-    assert L.tbsla_hip_set_csr_lanes(h, 16) == 0
+    assert L.tbsla_hip_set_csr_schedule(h, 3, 0) == 1
+    assert L.tbsla_hip_set_csr_schedule(h, 16, 3) == 1
+    assert L.tbsla_hip_set_csr_schedule(h, 16, 4) == 0
     assert L.tbsla_hip_fill_cqmat(h, 10, 10, -1, 0.1, 1, 0, 0, 1, 1, -1, None) == 1
     assert L.tbsla_hip_fill_cqmat(h, 10, 10, 3, 0.1, 1, 2, 0, 1, 1, -1, None) == 1
     assert L.tbsla_hip_destroy(h) == 0

@@ -158,13 +158,39 @@ def test_cfg1_ten_spmvs():
         close(dy.to_numpy(), y_ref)
 
 
-@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
 def test_csr_schedules_agree(lanes):
+    """Every (lanes, rows) schedule gives the oracle's result within 1e-12,
+    on regular rows and on rows of very different lengths."""
     n = 20000
     o = O.fill_cqmat(O.CSR, n, n, 40, 0.2, 0, value_seed=2)
     g = T.Matrix(T.CSR).fill_cqmat(n, n, 40, 0.2, 0, value_seed=2)
-    g.set_csr_lanes(lanes)
+    rng = np.random.default_rng(lanes)
+    lens = rng.integers(0, 300, 3000)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = rng.integers(0, 3000, rp[-1]).astype(np.int32)
+    vals = rng.random(rp[-1])
+    geom = dict(n_row=3000, n_col=3000, f_row=0, f_col=0, ln_row=3000, ln_col=3000, pr=0, pc=0,
+                NR=1, NC=1, nnz=int(rp[-1]))
+    irr = T.Matrix.from_arrays(T.CSR, geom, rp, cols, vals)
+    oirr = O.csr_from_arrays(3000, 3000, rp, cols, vals)
     x = O.unit_vector(1, n)
+    for rows in (1, 2, 4):
+        g.set_csr_schedule(lanes, rows)
+        close(g.spmv(x), o.spmv(x))
+        irr.set_csr_schedule(lanes, rows)
+        close(irr.spmv(x[:3000]), oirr.spmv(x[:3000]))
+
+
+def test_autotune_keeps_results():
+    n = 50000
+    o = O.fill_cqmat(O.CSR, n, n, 24, 0.1, 0, value_seed=2)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, 24, 0.1, 0, value_seed=2)
+    x = O.unit_vector(1, n)
+    dx, dy = T.DeviceBuffer.from_numpy(x), T.DeviceBuffer(n * 8)
+    us = g.autotune(dy, dx, 3)
+    L, R = g.get_csr_schedule()
+    assert us > 0 and L in (1, 2, 4, 8, 16, 32, 64) and R in (1, 2, 4)
     close(g.spmv(x), o.spmv(x))
 
 
