@@ -27,6 +27,20 @@ HBM_PEAK_GBS = 8000.0
 N, C, Q = 4194304, 32, 0.1
 
 
+CAL_BYTES = 256 << 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "csr_spmv_cqmat_n4194304_c32_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the CSR kernel from the committed rocprofv3
+    --pmc summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
+    calibration copy), or None when it has not been measured."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def algorithmic_bytes_csr(stored, n_rows, n_cols):
     return 12 * stored + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
 
@@ -87,6 +101,15 @@ def main():
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "k_csr_spmv<%d,%d>" % m.get_csr_schedule(), "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
+    # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+    # give the scale factor applied to the SpMV kernel's counters
+    cal = T.DeviceBuffer(CAL_BYTES), T.DeviceBuffer(CAL_BYTES)
+    T.check(T.lib().tbsla_hip_memcpy_dtod(cal[1].ptr, cal[0].ptr, CAL_BYTES, None))
+    T.check(T.lib().tbsla_hip_stream_synchronize(None))
+    del cal
+    traffic = measured_traffic()
+    if traffic is not None:
+        roof["traffic"] = traffic
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),

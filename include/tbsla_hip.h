@@ -154,6 +154,50 @@ int tbsla_hip_AAxpAxpx(tbsla_hip_matrix_t m, double *d_r, const double *d_v, dou
  * `stream`; *ms = total milliseconds.  For measurement only. */
 int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                         float *ms, void *stream);
+/* ---- column normalisation, PageRank, CG --------------------------------- */
+/* Matrix::get_col_sums / normalize_cols / make_stochastic (Matrix.hpp:46-49,
+ * src/cpp/matrix/Matrix.cpp:33-38): CSR MatrixCSR.cpp:1192-1232, ELL
+ * MatrixELL.cpp:803-833 (positive values only), SCOO MatrixSCOO.cpp:713-738;
+ * COO and DENSE are no-ops as in the reference.  d_s (ln_col doubles) is
+ * accumulated into, as the reference does; the caller zeroes it.  Column
+ * sums are formed in the reference's serial loop order (stable sort by
+ * column), so they are bit-identical to a serial run. */
+int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *stream);
+int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s, void *stream);
+int tbsla_hip_make_stochastic(tbsla_hip_matrix_t m, double *d_s, void *stream);
+
+/* tbsla::mpi::Matrix::page_rank (src/mpi/matrix/Matrix.cpp:540-602) on this
+ * block: b starts at 1; per iteration y = A b_t, b = (beta y + S(1-beta)/n)/S
+ * with S = sum(b_t), error = sum|b - b_t| / NC; stops when error < epsilon
+ * or after max_iterations; finally b /= sum(b).  d_b receives ln_col
+ * doubles; *nb_iterations and *final_error as the reference reports them. */
+int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double epsilon, int max_iterations,
+                        double *d_b, int *nb_iterations, double *final_error, void *stream);
+
+enum { TBSLA_HIP_CG_FP64 = 0, TBSLA_HIP_CG_MIXED = 1 };
+enum { TBSLA_HIP_PRECOND_NONE = 0, TBSLA_HIP_PRECOND_JACOBI = 1 };
+
+typedef struct {
+  int max_iterations;    /* reference default 10000 */
+  double tolerance;      /* stop when |r| < tolerance (x |rhs| if relative); CG.cpp:34 */
+  int relative;          /* 0: absolute test as the reference; 1: relative */
+  int precision;         /* TBSLA_HIP_CG_FP64, or _MIXED: FP32 matrix values, FP32 SpMV
+                            accumulation, FP64 vectors and dot products (CSR only) */
+  int preconditioner;    /* TBSLA_HIP_PRECOND_NONE or _JACOBI (CSR only) */
+  int replace_every;     /* > 0: replace r by the true FP64 residual every k iterations */
+  int use_initial_guess; /* 0: x starts at 0 (as the reference app); 1: from d_x */
+} tbsla_hip_cg_options;
+
+/* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call
+ * corrected (:61 passes output and input swapped, so the reference never
+ * moves): alpha = r.r / max(p.Ap, 1e-10), x += alpha p, r -= alpha Ap, stop
+ * on |r|, beta = r.r / max(r_old.r_old, 1e-10), p = r + beta p.  opt NULL:
+ * the reference's settings.  *nb_iterations counts the iterations that did
+ * not converge (:71); *res_norm is |r| of the recurrence at exit. */
+int tbsla_hip_cg(tbsla_hip_matrix_t m, const double *d_rhs, double *d_x,
+                 const tbsla_hip_cg_options *opt, int *nb_iterations, double *res_norm,
+                 void *stream);
+
 /* CSR SpMV schedule (no reference counterpart): `lanes` cooperating on one
  * row (1..64, power of two) and `rows` each lane group keeps in flight
  * (1, 2 or 4); 0 restores the choice from the mean row length. */

@@ -1033,7 +1033,7 @@ int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_i
     for (int z = 0; z < length; z++) b[r_start + z] = buf[s_start + z];
     double sum = 0;
     for (int k = 0; k < ln_col; k++) sum += b_t[k];
-    double t_sum = sum * (1 - beta) / g->n_col;
+    double t_sum = sum * ((1 - beta) / g->n_col); /* t_sum *= (1-beta)/n_col, :123 */
     for (int k = 0; k < ln_col; k++) b[k] = beta * b[k] + t_sum;
     error = 0;
     for (int k = 0; k < ln_col; k++) {

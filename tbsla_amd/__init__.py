@@ -27,6 +27,16 @@ GEOM_FIELDS = ("n_row", "n_col", "f_row", "f_col", "ln_row", "ln_col", "pr", "pc
 _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 
 
+class CGOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("tolerance", C.c_double), ("relative", C.c_int),
+                ("precision", C.c_int), ("preconditioner", C.c_int), ("replace_every", C.c_int),
+                ("use_initial_guess", C.c_int)]
+
+
+CG_FP64, CG_MIXED = 0, 1
+PRECOND_NONE, PRECOND_JACOBI = 0, 1
+
+
 class Geometry(C.Structure):
     _fields_ = ([(f, C.c_int) for f in GEOM_FIELDS] +
                 [("nnz", C.c_longlong), ("stored", C.c_longlong), ("max_col", C.c_int),
@@ -63,6 +73,12 @@ SIGNATURES = [
     ("tbsla_hip_AAxpAx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_AAxpAxpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_time_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
+    ("tbsla_hip_get_col_sums", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_normalize_cols", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_make_stochastic", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
+                                      C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_set_csr_schedule", C.c_int, [_vp, C.c_int, C.c_int]),
     ("tbsla_hip_get_csr_schedule", C.c_int, [_vp, _ip, _ip]),
     ("tbsla_hip_autotune", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
@@ -291,6 +307,44 @@ class Matrix:
         us = C.c_float()
         check(lib().tbsla_hip_autotune(self.h, dr.ptr, dv.ptr, iters, C.byref(us), stream))
         return us.value
+
+    # -- column normalisation, PageRank, CG ----------------------------------
+    def get_col_sums(self, s=None) -> np.ndarray:
+        n = self.geometry().ln_col
+        ds = _dev(np.zeros(n) if s is None else s)
+        check(lib().tbsla_hip_get_col_sums(self.h, ds.ptr, None))
+        return ds.to_numpy(np.float64, n)
+
+    def normalize_cols(self, s) -> None:
+        ds = _dev(s)
+        check(lib().tbsla_hip_normalize_cols(self.h, ds.ptr, None))
+
+    def make_stochastic(self, s=None) -> np.ndarray:
+        n = self.geometry().ln_col
+        ds = _dev(np.zeros(n) if s is None else s)
+        check(lib().tbsla_hip_make_stochastic(self.h, ds.ptr, None))
+        return ds.to_numpy(np.float64, n)
+
+    def page_rank(self, beta=0.85, epsilon=1e-10, max_iterations=10000):
+        n = self.geometry().ln_col
+        db = DeviceBuffer(max(n, 1) * 8)
+        it, err = C.c_int(), C.c_double()
+        check(lib().tbsla_hip_page_rank(self.h, beta, epsilon, max_iterations, db.ptr,
+                                        C.byref(it), C.byref(err), None))
+        return db.to_numpy(np.float64, n), it.value, err.value
+
+    def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False,
+           precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None):
+        """Corrected tbsla::mpi::Matrix::CG; returns (x, iterations, |r|)."""
+        drhs = _dev(rhs)
+        n = self.geometry().ln_col
+        dx = _dev(np.zeros(n) if x0 is None else x0)
+        o = CGOptions(max_iterations, tolerance, int(relative), precision, preconditioner,
+                      replace_every, int(x0 is not None))
+        it, rn = C.c_int(), C.c_double()
+        check(lib().tbsla_hip_cg(self.h, drhs.ptr, dx.ptr, C.byref(o), C.byref(it),
+                                 C.byref(rn), None))
+        return dx.to_numpy(np.float64, n), it.value, rn.value
 
     # -- conversions --------------------------------------------------------
     def to(self, fmt: int) -> "Matrix":

@@ -13,8 +13,10 @@ int scan(const T *in, T *out, long long n, bool inclusive, long long *d_total, v
 size_t scan_workspace_bytes(long long n);
 
 // spmv.hip
+// d_done (device, nullable): CSR and ELL launches return at once when it
+// holds a nonzero value (converged solver iterations queued in a batch).
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
-                bool zero_first, hipStream_t s);
+                bool zero_first, hipStream_t s, const int *d_done = nullptr);
 int pick_csr_lanes(const tbsla_hip_matrix_s *m);
 int pick_csr_rows(const tbsla_hip_matrix_s *m, int lanes);
 

@@ -33,7 +33,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(long long n_rows, const int
                                                      const int *__restrict__ colidx,
                                                      const double *__restrict__ vals,
                                                      const double *__restrict__ x, int f_col,
-                                                     double *__restrict__ y) {
+                                                     double *__restrict__ y,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
   constexpr int G = kWave / L;
   const int lane = threadIdx.x & (L - 1);
   const int g = (threadIdx.x & (kWave - 1)) / L;
@@ -86,7 +88,9 @@ __global__ __launch_bounds__(kBlock) void k_ell_spmv(long long n_rows, int max_c
                                                      const int *__restrict__ cols,
                                                      const double *__restrict__ vals,
                                                      const double *__restrict__ x, int f_col,
-                                                     double *__restrict__ y) {
+                                                     double *__restrict__ y,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
   const long long row = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (row >= n_rows) return;
   double sum = 0.0;
@@ -263,19 +267,21 @@ unsigned stride_grid(long long n) {
 }
 
 template <int L, int R>
-void launch_csr_lr(const tbsla_hip_matrix_s *m, double *y, const double *x, hipStream_t s) {
+void launch_csr_lr(const tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
+                   hipStream_t s) {
   const long long rows_per_wave = (long long)(kWave / L) * R;
   const long long waves = (m->ln_row + rows_per_wave - 1) / rows_per_wave;
   k_csr_spmv<L, R><<<grid_for(waves * kWave), kBlock, 0, s>>>(m->ln_row, m->rowptr, m->colidx,
-                                                               m->values, x, m->f_col, y);
+                                                               m->values, x, m->f_col, y, done);
 }
 
 template <int L>
-void launch_csr(const tbsla_hip_matrix_s *m, int R, double *y, const double *x, hipStream_t s) {
+void launch_csr(const tbsla_hip_matrix_s *m, int R, double *y, const double *x, const int *done,
+                hipStream_t s) {
   switch (R) {
-    case 1: launch_csr_lr<L, 1>(m, y, x, s); break;
-    case 2: launch_csr_lr<L, 2>(m, y, x, s); break;
-    default: launch_csr_lr<L, 4>(m, y, x, s); break;
+    case 1: launch_csr_lr<L, 1>(m, y, x, done, s); break;
+    case 2: launch_csr_lr<L, 2>(m, y, x, done, s); break;
+    default: launch_csr_lr<L, 4>(m, y, x, done, s); break;
   }
 }
 
@@ -311,7 +317,7 @@ int pick_csr_rows(const tbsla_hip_matrix_s *m, int L) {
 }
 
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
-                bool zero_first, hipStream_t s) {
+                bool zero_first, hipStream_t s, const int *d_done) {
   switch (m->format) {
     case TBSLA_HIP_CSR: {
       // MatrixCSR::Ax returns when nnz == 0 (MatrixCSR.cpp:210-211)
@@ -321,13 +327,13 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       }
       const int L = pick_csr_lanes(m), R = pick_csr_rows(m, L);
       switch (L) {
-        case 1: launch_csr<1>(m, R, d_r, d_v, s); break;
-        case 2: launch_csr<2>(m, R, d_r, d_v, s); break;
-        case 4: launch_csr<4>(m, R, d_r, d_v, s); break;
-        case 8: launch_csr<8>(m, R, d_r, d_v, s); break;
-        case 16: launch_csr<16>(m, R, d_r, d_v, s); break;
-        case 32: launch_csr<32>(m, R, d_r, d_v, s); break;
-        default: launch_csr<64>(m, R, d_r, d_v, s); break;
+        case 1: launch_csr<1>(m, R, d_r, d_v, d_done, s); break;
+        case 2: launch_csr<2>(m, R, d_r, d_v, d_done, s); break;
+        case 4: launch_csr<4>(m, R, d_r, d_v, d_done, s); break;
+        case 8: launch_csr<8>(m, R, d_r, d_v, d_done, s); break;
+        case 16: launch_csr<16>(m, R, d_r, d_v, d_done, s); break;
+        case 32: launch_csr<32>(m, R, d_r, d_v, d_done, s); break;
+        default: launch_csr<64>(m, R, d_r, d_v, d_done, s); break;
       }
       TBSLA_HIP_LAUNCH_CHECK();
       return TBSLA_HIP_SUCCESS;
@@ -339,7 +345,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         return TBSLA_HIP_SUCCESS;
       }
       k_ell_spmv<<<grid_for(m->ln_row), kBlock, 0, s>>>(m->ln_row, m->max_col, m->colidx,
-                                                         m->values, d_v, m->f_col, d_r);
+                                                         m->values, d_v, m->f_col, d_r, d_done);
       TBSLA_HIP_LAUNCH_CHECK();
       return TBSLA_HIP_SUCCESS;
     }

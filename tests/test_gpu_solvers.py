@@ -1,0 +1,180 @@
+"""Column normalisation, PageRank and CG on the GPU against the oracle.
+
+* get_col_sums / make_stochastic: bit-exact with the oracle's serial loop
+  (the GPU sums each column in the reference's loop order).
+* PageRank (page_rank semantics): same iteration count, iterate within a
+  relative 1e-12 (max-norm) of the oracle.
+* CG (corrected): early iterates within a relative 1e-12 of the oracle's
+  (the iteration amplifies the last-bit differences of the dot products
+  only slowly there), the converged solution against a direct solve.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tbsla_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def to_gpu_csr(m):
+    """Import an oracle CSR block into a GPU CSR handle."""
+    rp, ci, va = m.arrays()
+    return T.Matrix.from_arrays(T.CSR, m.geom, rp, ci, va)
+
+
+def laplacian_1d(n):
+    A = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    return A
+
+
+def sym_cqmat(n, c, q, seed):
+    """A = (B + B^T)/2 with the diagonal set to the row's off-diagonal
+    absolute sum + 1 (strictly diagonally dominant, SPD)."""
+    B = O.fill_cqmat(O.CSR, n, n, c, q, seed, value_seed=seed)
+    rp, ci, va = B.arrays()
+    D = np.zeros((n, n))
+    for i in range(n):
+        np.add.at(D[i], ci[rp[i]:rp[i + 1]], va[rp[i]:rp[i + 1]])
+    A = (D + D.T) / 2
+    np.fill_diagonal(A, 0.0)
+    np.fill_diagonal(A, np.abs(A).sum(axis=1) + 1.0)
+    return A
+
+
+@pytest.mark.parametrize("fmt", [T.CSR, T.ELL, T.SCOO])
+def test_col_sums_and_normalisation_bit_exact(fmt):
+    for (n, c, q, grid) in [(3000, 12, 0.2, (0, 0, 1, 1)), (5000, 20, 0.5, (1, 2, 3, 4)),
+                            (20000, 32, 0.1, (0, 0, 1, 1))]:
+        o = O.fill_cqmat(fmt, n, n, c, q, 0, *grid, value_seed=9)
+        g = T.Matrix(fmt).fill_cqmat(n, n, c, q, 0, *grid, value_seed=9)
+        so = o.get_col_sums()
+        sg = g.get_col_sums()
+        assert np.array_equal(sg, so), (fmt, n)
+        so = O.fill_cqmat(fmt, n, n, c, q, 0, *grid, value_seed=9)
+        s1 = so.make_stochastic()
+        s2 = g.make_stochastic()
+        assert np.array_equal(s2, s1)
+        for a, b in zip(g.arrays(), so.arrays()):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fmt", [T.COO, T.DENSE])
+def test_col_sums_noop_formats(fmt):
+    g = T.Matrix(fmt).fill_cqmat(500, 500, 8, 0.2, 0, value_seed=1)
+    before = g.arrays()
+    s = g.make_stochastic(np.full(500, 3.0))
+    assert np.array_equal(s, np.full(500, 3.0))
+    for a, b in zip(g.arrays(), before):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fmt", [T.CSR, T.ELL, T.SCOO, T.COO])
+def test_page_rank_matches_oracle(fmt):
+    n, c, q = 60000, 16, 0.1
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, value_seed=0)
+    o.make_stochastic()
+    bo, ito, eo = o.page_rank(0.85, 1e-10, 10000)
+    g = T.Matrix(fmt).fill_cqmat(n, n, c, q, 0, value_seed=0)
+    if fmt == T.COO:
+        # COO's make_stochastic is a no-op in the reference: import the
+        # oracle's normalised CSR entries as triplets instead
+        rp, ci, va = o.arrays()
+        rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+        g = T.Matrix.from_arrays(T.COO, o.geom, rows, ci, va)
+    else:
+        g.make_stochastic()
+    bg, itg, eg = g.page_rank(0.85, 1e-10, 10000)
+    assert itg == ito, (itg, ito)
+    assert rel_max(bg, bo) < 1e-12
+    assert abs(bg.sum() - 1.0) < 1e-12
+
+
+def test_page_rank_max_iterations_and_determinism():
+    n = 30000
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, 8, 0.3, 2, value_seed=4)
+    g.make_stochastic()
+    b1, it1, e1 = g.page_rank(0.85, 1e-14, 37)
+    b2, it2, e2 = g.page_rank(0.85, 1e-14, 37)
+    assert it1 == 37 and np.array_equal(b1, b2) and e1 == e2
+    o = O.fill_cqmat(O.CSR, n, n, 8, 0.3, 2, value_seed=4)
+    o.make_stochastic()
+    bo, ito, _ = o.page_rank(0.85, 1e-14, 37)
+    assert ito == 37 and rel_max(b1, bo) < 1e-12
+
+
+def test_cg_early_iterates_match_oracle():
+    n = 4000
+    A = sym_cqmat(n, 9, 0.1, 3)
+    o = O.csr_from_dense(A)
+    g = to_gpu_csr(o)
+    rhs = 2 * O.unit_vector(7, n) - 1
+    for k in (1, 3, 10, 30):
+        xo, ito, ro = o.cg(rhs, max_iterations=k, tolerance=1e-30)
+        xg, itg, rg = g.cg(rhs, max_iterations=k, tolerance=1e-30)
+        assert itg == ito == k
+        assert rel_max(xg, xo) < 1e-12, (k, rel_max(xg, xo))
+        assert abs(rg - ro) <= 1e-10 * ro
+
+
+def test_cg_converges_like_oracle():
+    n = 2000
+    A = laplacian_1d(n) + np.diag(np.linspace(0.01, 1.0, n))
+    o = O.csr_from_dense(A)
+    g = to_gpu_csr(o)
+    rhs = 2 * O.unit_vector(11, n) - 1
+    exact = np.linalg.solve(A, rhs)
+    xo, ito, ro = o.cg(rhs, tolerance=1e-10)
+    xg, itg, rg = g.cg(rhs, tolerance=1e-10)
+    assert abs(itg - ito) <= 2, (itg, ito)
+    assert rg < 1e-10
+    assert np.linalg.norm(A @ xg - rhs) < 2e-10
+    assert rel_max(xg, exact) < 1e-8
+
+
+def test_cg_relative_jacobi_and_mixed():
+    n = 20000
+    A = None
+    # symmetric strongly diagonally dominant matrix with varying diagonal,
+    # built as COO on the host and imported
+    rng = np.random.default_rng(5)
+    i = np.arange(n)
+    rows = np.concatenate([i, i[1:], i[:-1], i[2:], i[:-2]])
+    cols = np.concatenate([i, i[:-1], i[1:], i[:-2], i[2:]])
+    off = -rng.random(n - 1)
+    off2 = -0.5 * rng.random(n - 2)
+    vals_off = np.concatenate([off, off, off2, off2])
+    diag = np.zeros(n)
+    np.add.at(diag, rows[n:], -vals_off)
+    diag = diag * (1 + 100 * rng.random(n)) + 1e-3
+    vals = np.concatenate([diag, vals_off])
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp)
+    o = O.csr_from_arrays(n, n, rp, cols, vals)
+    g = to_gpu_csr(o)
+    rhs = 2 * O.unit_vector(3, n) - 1
+    x0, it0, _ = g.cg(rhs, tolerance=1e-8, relative=True)
+    xj, itj, _ = g.cg(rhs, tolerance=1e-8, relative=True, preconditioner=T.PRECOND_JACOBI)
+    xm, itm, _ = g.cg(rhs, tolerance=1e-8, relative=True, precision=T.CG_MIXED, replace_every=50)
+    xmj, itmj, _ = g.cg(rhs, tolerance=1e-8, relative=True, precision=T.CG_MIXED,
+                        preconditioner=T.PRECOND_JACOBI, replace_every=50)
+
+    def true_rel(x):
+        y = o.spmv(x)
+        return np.linalg.norm(rhs - y) / np.linalg.norm(rhs)
+    assert true_rel(x0) < 1.5e-8
+    assert true_rel(xj) < 1.5e-8
+    assert itj < it0  # the varying diagonal is what Jacobi removes
+    # mixed precision: convergence is judged on the FP64 true residual
+    assert true_rel(xm) < 1.5e-8
+    assert true_rel(xmj) < 1.5e-8
+    # the oracle's plain CG agrees on the iteration count within a couple
+    xo, ito, _ = o.cg(rhs, tolerance=1e-8, relative=True)
+    assert abs(ito - it0) <= 2
