@@ -1,0 +1,385 @@
+// tbsla/hip/Matrix.hpp -- drop-in C++ classes for TBSLA's matrix API on the GPU.
+//
+// Header-only layer over the C ABI of libtbsla_hip (include/tbsla_hip.h).
+// tbsla::hip::Matrix keeps the interface of tbsla::cpp::Matrix
+// (src/cpp/include/tbsla/cpp/Matrix.hpp:11-57): the same method names,
+// argument meanings, host-pointer ownership rules (spmv and a_axpx_ return
+// new[] host arrays the caller deletes; Ax writes a caller buffer,
+// overwriting for CSR/ELL and accumulating for COO/SCOO/DENSE), getters and
+// the binary write/read format; tbsla::mpi::Matrix's page_rank and CG
+// (src/mpi/include/tbsla/mpi/Matrix.hpp:31-33) are available on one block.
+// Device-resident overloads (*_device) take device pointers and a stream.
+//
+// Errors: a failing library call throws std::runtime_error with the
+// library's message (the reference throws MatrixFormatReadException and
+// std::runtime_error for unsupported operations).
+#ifndef TBSLA_HIP_MATRIX_HPP
+#define TBSLA_HIP_MATRIX_HPP
+
+#include <cstddef>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../tbsla_hip.h"
+
+namespace tbsla {
+namespace hip {
+
+inline void check(int status) {
+  if (status != TBSLA_HIP_SUCCESS)
+    throw std::runtime_error(std::string("tbsla_hip: ") + tbsla_hip_status_string(status) +
+                             ": " + tbsla_hip_last_error());
+}
+
+// Device buffer owned by the caller's scope.
+template <typename T>
+class DeviceArray {
+ public:
+  explicit DeviceArray(size_t n = 0) : n_(n) {
+    if (n) check(tbsla_hip_malloc(reinterpret_cast<void **>(&p_), n * sizeof(T)));
+  }
+  DeviceArray(const T *host, size_t n) : DeviceArray(n) {
+    if (n) check(tbsla_hip_memcpy_htod(p_, host, n * sizeof(T), nullptr));
+  }
+  ~DeviceArray() {
+    if (p_) tbsla_hip_free(p_);
+  }
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  void to_host(T *host, size_t n) const {
+    if (n) check(tbsla_hip_memcpy_dtoh(host, p_, n * sizeof(T), nullptr));
+  }
+  T *get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  T *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+class Matrix {
+ public:
+  explicit Matrix(int format) { check(tbsla_hip_create(&h_, format)); }
+  virtual ~Matrix() { tbsla_hip_destroy(h_); }
+  Matrix(const Matrix &) = delete;
+  Matrix &operator=(const Matrix &) = delete;
+
+  friend std::ostream &operator<<(std::ostream &os, const Matrix &m) { return m.print(os); }
+
+  // ---- SpMV (Matrix.hpp:14-15) -------------------------------------------
+  double *spmv(const double *v, int vect_incr = 0) const {
+    const tbsla_hip_geometry g = geometry();
+    DeviceArray<double> dv(v, in_len(g)), dr(out_len(g));
+    check(tbsla_hip_spmv(h_, dr.get(), dv.get(), vect_incr, nullptr));
+    double *r = new double[out_len(g)]();
+    dr.to_host(r, out_len(g));
+    return r;
+  }
+  void Ax(double *r, const double *v, int vect_incr = 0) const {
+    const tbsla_hip_geometry g = geometry();
+    DeviceArray<double> dv(v, in_len(g)), dr(r, out_len(g));
+    check(tbsla_hip_Ax(h_, dr.get(), dv.get(), vect_incr, nullptr));
+    dr.to_host(r, out_len(g));
+  }
+  void Ax_device(double *d_r, const double *d_v, int vect_incr = 0, void *stream = nullptr) const {
+    check(tbsla_hip_Ax(h_, d_r, d_v, vect_incr, stream));
+  }
+  void spmv_device(double *d_r, const double *d_v, int vect_incr = 0,
+                   void *stream = nullptr) const {
+    check(tbsla_hip_spmv(h_, d_r, d_v, vect_incr, stream));
+  }
+
+  // ---- composites (src/cpp/matrix/Matrix.cpp:5-38) -----------------------
+  double *a_axpx_(const double *v, int vect_incr = 0) const {
+    const tbsla_hip_geometry g = geometry();
+    DeviceArray<double> dv(v, in_len(g)), dr(out_len(g)), dt(out_len(g));
+    check(tbsla_hip_a_axpx(h_, dr.get(), dv.get(), dt.get(), vect_incr, nullptr));
+    double *r = new double[out_len(g)]();
+    dr.to_host(r, out_len(g));
+    return r;
+  }
+  void AAxpAx(double *r, double *v, double *buffer, int vect_incr = 0) const {
+    const tbsla_hip_geometry g = geometry();
+    DeviceArray<double> dv(v, in_len(g)), dr(r, out_len(g)), db(buffer, out_len(g));
+    check(tbsla_hip_AAxpAx(h_, dr.get(), dv.get(), db.get(), vect_incr, nullptr));
+    dr.to_host(r, out_len(g));
+    db.to_host(buffer, out_len(g));
+  }
+  void AAxpAxpx(double *r, double *v, double *buffer, int vect_incr = 0) const {
+    const tbsla_hip_geometry g = geometry();
+    DeviceArray<double> dv(v, in_len(g)), dr(r, out_len(g)), db(buffer, out_len(g));
+    check(tbsla_hip_AAxpAxpx(h_, dr.get(), dv.get(), db.get(), vect_incr, nullptr));
+    dr.to_host(r, out_len(g));
+    db.to_host(buffer, out_len(g));
+  }
+  void make_stochastic(double *s) {
+    const size_t n = (size_t)geometry().ln_col;
+    DeviceArray<double> ds(s, n);
+    check(tbsla_hip_make_stochastic(h_, ds.get(), nullptr));
+    ds.to_host(s, n);
+  }
+  void get_col_sums(double *s) {
+    const size_t n = (size_t)geometry().ln_col;
+    DeviceArray<double> ds(s, n);
+    check(tbsla_hip_get_col_sums(h_, ds.get(), nullptr));
+    ds.to_host(s, n);
+  }
+  void normalize_cols(double *s) {
+    const size_t n = (size_t)geometry().ln_col;
+    DeviceArray<double> ds(s, n);
+    check(tbsla_hip_normalize_cols(h_, ds.get(), nullptr));
+  }
+
+  // ---- drivers (src/mpi/include/tbsla/mpi/Matrix.hpp:31-33), one block ---
+  double *page_rank(double beta, double epsilon, int max_iterations, int &nb_iterations_done) {
+    const size_t n = (size_t)geometry().ln_col;
+    DeviceArray<double> db(n);
+    double err = 0;
+    check(tbsla_hip_page_rank(h_, beta, epsilon, max_iterations, db.get(), &nb_iterations_done,
+                              &err, nullptr));
+    double *b = new double[n];
+    db.to_host(b, n);
+    return b;
+  }
+  // CG(comm, v, r, ...) of the reference with v the right-hand side and r
+  // the solution (CG.cpp:32); the Ax argument order is corrected.
+  void CG(double *v, double *r, int max_iterations, int &nb_iterations_done,
+          const tbsla_hip_cg_options *opt = nullptr) {
+    const size_t n = (size_t)geometry().n_col;
+    DeviceArray<double> dv(v, n), dr(r, n);
+    tbsla_hip_cg_options o{};
+    if (opt) o = *opt;
+    else {
+      o.tolerance = 1.0e-10;
+      o.use_initial_guess = 0;
+    }
+    o.max_iterations = max_iterations;
+    double rn = 0;
+    check(tbsla_hip_cg(h_, dv.get(), dr.get(), &o, &nb_iterations_done, &rn, nullptr));
+    dr.to_host(r, n);
+  }
+
+  // ---- generators (Matrix.hpp:41-42) -------------------------------------
+  void fill_cdiag(int n_row, int n_col, int cdiag, int pr = 0, int pc = 0, int NR = 1,
+                  int NC = 1) {
+    check(tbsla_hip_fill_cdiag(h_, n_row, n_col, cdiag, pr, pc, NR, NC, nullptr));
+  }
+  void fill_cqmat(int n_row, int n_col, int c, double q, unsigned int seed_mult = 1, int pr = 0,
+                  int pc = 0, int NR = 1, int NC = 1) {
+    check(tbsla_hip_fill_cqmat(h_, n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, -1, nullptr));
+  }
+  // fill_cqmat with values V(value_seed, k) uniform in [0,1) instead of 1.0
+  void fill_cqmat_values(int n_row, int n_col, int c, double q, unsigned int seed_mult,
+                         long long value_seed, int pr = 0, int pc = 0, int NR = 1, int NC = 1) {
+    check(tbsla_hip_fill_cqmat(h_, n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed,
+                               nullptr));
+  }
+
+  // ---- getters (Matrix.hpp:21-32) ----------------------------------------
+  int get_n_row() const { return geometry().n_row; }
+  int get_n_col() const { return geometry().n_col; }
+  int get_f_row() const { return geometry().f_row; }
+  int get_f_col() const { return geometry().f_col; }
+  int get_ln_row() const { return geometry().ln_row; }
+  int get_ln_col() const { return geometry().ln_col; }
+  int get_pr() const { return geometry().pr; }
+  int get_pc() const { return geometry().pc; }
+  int get_NR() const { return geometry().NR; }
+  int get_NC() const { return geometry().NC; }
+  long int get_nnz() const { return (long int)geometry().nnz; }
+  std::string get_vectorization() const { return "HIP gfx950"; }
+
+  // ---- binary format (MatrixCSR.cpp:236-295 and the per-format twins) ----
+  // Header: n_row n_col ln_row ln_col f_row f_col (int32), nnz (int64),
+  // pr pc NR NC (int32), ELL max_col (int32); then each array preceded by
+  // its size_t length.
+  std::ostream &write(std::ostream &os) const {
+    const tbsla_hip_geometry g = geometry();
+    const int hdr1[6] = {g.n_row, g.n_col, g.ln_row, g.ln_col, g.f_row, g.f_col};
+    const long int nnz = (long int)g.nnz;
+    const int hdr2[4] = {g.pr, g.pc, g.NR, g.NC};
+    os.write(reinterpret_cast<const char *>(hdr1), sizeof(hdr1));
+    os.write(reinterpret_cast<const char *>(&nnz), sizeof(nnz));
+    os.write(reinterpret_cast<const char *>(hdr2), sizeof(hdr2));
+    std::vector<int> a, b;
+    std::vector<double> v;
+    exported(g, a, b, v);
+    auto arr = [&os](const void *p, size_t n, size_t elem) {
+      os.write(reinterpret_cast<const char *>(&n), sizeof(n));
+      if (n) os.write(reinterpret_cast<const char *>(p), (std::streamsize)(n * elem));
+    };
+    switch (g.format) {
+      case TBSLA_HIP_CSR: {
+        // values and colidx are written with the nnz field's length, as
+        // the reference does; entries beyond rowptr[ln_row] are zero here
+        v.resize((size_t)std::max<long long>(g.nnz, (long long)v.size()), 0.0);
+        b.resize(v.size(), 0);
+        arr(v.data(), (size_t)g.nnz, sizeof(double));
+        arr(a.data(), (size_t)g.ln_row + 1, sizeof(int));
+        arr(b.data(), (size_t)g.nnz, sizeof(int));
+        break;
+      }
+      case TBSLA_HIP_COO:
+      case TBSLA_HIP_SCOO:
+        arr(v.data(), (size_t)g.nnz, sizeof(double));
+        arr(a.data(), (size_t)g.nnz, sizeof(int));
+        arr(b.data(), (size_t)g.nnz, sizeof(int));
+        break;
+      case TBSLA_HIP_ELL: {
+        os.write(reinterpret_cast<const char *>(&g.max_col), sizeof(int));
+        const size_t slots = (size_t)g.ln_row * (size_t)g.max_col;
+        v.resize(slots, 0.0);
+        a.resize(slots, 0);
+        arr(v.data(), slots, sizeof(double));
+        arr(a.data(), slots, sizeof(int));
+        break;
+      }
+      case TBSLA_HIP_DENSE: {
+        const size_t cells = (size_t)g.ln_row * (size_t)g.ln_col;
+        v.resize(cells, 0.0);
+        arr(v.data(), cells, sizeof(double));
+        break;
+      }
+    }
+    return os;
+  }
+
+  std::istream &read(std::istream &is, std::size_t pos = 0, std::size_t n = 1) {
+    (void)pos;
+    (void)n;
+    tbsla_hip_geometry g{};
+    int hdr1[6], hdr2[4];
+    long int nnz = 0;
+    is.read(reinterpret_cast<char *>(hdr1), sizeof(hdr1));
+    is.read(reinterpret_cast<char *>(&nnz), sizeof(nnz));
+    is.read(reinterpret_cast<char *>(hdr2), sizeof(hdr2));
+    g.n_row = hdr1[0];
+    g.n_col = hdr1[1];
+    g.ln_row = hdr1[2];
+    g.ln_col = hdr1[3];
+    g.f_row = hdr1[4];
+    g.f_col = hdr1[5];
+    g.nnz = nnz;
+    g.pr = hdr2[0];
+    g.pc = hdr2[1];
+    g.NR = hdr2[2];
+    g.NC = hdr2[3];
+    const int fmt = geometry().format;
+    g.format = fmt;
+    if (fmt == TBSLA_HIP_ELL) is.read(reinterpret_cast<char *>(&g.max_col), sizeof(int));
+    auto arr = [&is](auto &vec) {
+      size_t len = 0;
+      is.read(reinterpret_cast<char *>(&len), sizeof(len));
+      vec.resize(len);
+      if (len) is.read(reinterpret_cast<char *>(vec.data()), (std::streamsize)(len * sizeof(vec[0])));
+    };
+    std::vector<int> a, b;
+    std::vector<double> v;
+    switch (fmt) {
+      case TBSLA_HIP_CSR: arr(v); arr(a); arr(b); break;
+      case TBSLA_HIP_COO:
+      case TBSLA_HIP_SCOO: arr(v); arr(a); arr(b); g.stored = (long long)v.size(); break;
+      case TBSLA_HIP_ELL: arr(v); arr(a); break;
+      case TBSLA_HIP_DENSE: arr(v); break;
+    }
+    if (!is) throw std::runtime_error("tbsla_hip: truncated matrix stream");
+    check(tbsla_hip_import(h_, fmt, &g, a.empty() ? nullptr : a.data(),
+                           b.empty() ? nullptr : b.data(), v.empty() ? nullptr : v.data(),
+                           nullptr));
+    return is;
+  }
+
+  std::ostream &print(std::ostream &os) const {
+    const tbsla_hip_geometry g = geometry();
+    static const char *names[] = {"COO", "SCOO", "CSR", "ELL", "DENSE"};
+    os << "------ " << names[g.format] << " (HIP) ------\n"
+       << "n_row " << g.n_row << " n_col " << g.n_col << " ln_row " << g.ln_row << " ln_col "
+       << g.ln_col << " f_row " << g.f_row << " f_col " << g.f_col << " nnz " << g.nnz
+       << " pr " << g.pr << " pc " << g.pc << " NR " << g.NR << " NC " << g.NC;
+    if (g.format == TBSLA_HIP_ELL) os << " max_col " << g.max_col;
+    return os << "\n";
+  }
+
+  tbsla_hip_geometry geometry() const {
+    tbsla_hip_geometry g{};
+    check(tbsla_hip_get_geometry(h_, &g));
+    return g;
+  }
+  tbsla_hip_matrix_t handle() const { return h_; }
+
+ protected:
+  tbsla_hip_matrix_t h_ = nullptr;
+
+  static size_t in_len(const tbsla_hip_geometry &g) { return (size_t)g.ln_col; }
+  static size_t out_len(const tbsla_hip_geometry &g) {
+    return (size_t)(g.format == TBSLA_HIP_COO ? g.n_row : g.ln_row);
+  }
+  void exported(const tbsla_hip_geometry &g, std::vector<int> &a, std::vector<int> &b,
+                std::vector<double> &v) const {
+    switch (g.format) {
+      case TBSLA_HIP_CSR:
+        a.resize((size_t)g.ln_row + 1);
+        b.resize((size_t)g.stored);
+        v.resize((size_t)g.stored);
+        break;
+      case TBSLA_HIP_COO:
+      case TBSLA_HIP_SCOO:
+        a.resize((size_t)g.stored);
+        b.resize((size_t)g.stored);
+        v.resize((size_t)g.stored);
+        break;
+      case TBSLA_HIP_ELL:
+        a.resize((size_t)g.stored);
+        v.resize((size_t)g.stored);
+        break;
+      case TBSLA_HIP_DENSE:
+        v.resize((size_t)g.stored);
+        break;
+    }
+    check(tbsla_hip_export(h_, a.empty() ? nullptr : a.data(), b.empty() ? nullptr : b.data(),
+                           v.empty() ? nullptr : v.data(), nullptr));
+  }
+};
+
+class MatrixCOO : public Matrix {
+ public:
+  MatrixCOO() : Matrix(TBSLA_HIP_COO) {}
+};
+
+// Conversion constructors <FMT>(const MatrixCOO&) (MatrixCSR.cpp:49 etc.).
+class MatrixSCOO : public Matrix {
+ public:
+  MatrixSCOO() : Matrix(TBSLA_HIP_SCOO) {}
+  explicit MatrixSCOO(const MatrixCOO &m) : Matrix(TBSLA_HIP_SCOO) {
+    check(tbsla_hip_convert(h_, TBSLA_HIP_SCOO, m.handle(), nullptr));
+  }
+};
+class MatrixCSR : public Matrix {
+ public:
+  MatrixCSR() : Matrix(TBSLA_HIP_CSR) {}
+  explicit MatrixCSR(const MatrixCOO &m) : Matrix(TBSLA_HIP_CSR) {
+    check(tbsla_hip_convert(h_, TBSLA_HIP_CSR, m.handle(), nullptr));
+  }
+};
+class MatrixELL : public Matrix {
+ public:
+  MatrixELL() : Matrix(TBSLA_HIP_ELL) {}
+  explicit MatrixELL(const MatrixCOO &m) : Matrix(TBSLA_HIP_ELL) {
+    check(tbsla_hip_convert(h_, TBSLA_HIP_ELL, m.handle(), nullptr));
+  }
+};
+class MatrixDENSE : public Matrix {
+ public:
+  MatrixDENSE() : Matrix(TBSLA_HIP_DENSE) {}
+  explicit MatrixDENSE(const MatrixCOO &m) : Matrix(TBSLA_HIP_DENSE) {
+    check(tbsla_hip_convert(h_, TBSLA_HIP_DENSE, m.handle(), nullptr));
+  }
+};
+
+}  // namespace hip
+}  // namespace tbsla
+
+#endif

@@ -198,6 +198,23 @@ int tbsla_hip_cg(tbsla_hip_matrix_t m, const double *d_rhs, double *d_x,
                  const tbsla_hip_cg_options *opt, int *nb_iterations, double *res_norm,
                  void *stream);
 
+/* ---- SpMM and personalised PageRank ------------------------------------- */
+/* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283; block form
+ * src/mpi/matrix/MatrixCSR.cpp:11-30): C = A B, B ln_col x k and C
+ * ln_row x k row-major device arrays, k <= 256.  CSR only.  Products are
+ * accumulated in the reference's order without fused multiply-add, so C is
+ * bit-identical to a serial run of the reference loop. */
+int tbsla_hip_spmm(tbsla_hip_matrix_t m, const double *d_B, double *d_C, int k, void *stream);
+/* Personalised PageRank for 64 one-hot teleport vectors solved together as
+ * one SpMM per iteration (no reference counterpart): column j follows the
+ * page_rank iteration of tbsla_hip_page_rank with the teleport mass
+ * S_j(1-beta) placed on vertex h_sources[j] (global index).  d_X receives
+ * ln_col x 64 doubles, row-major, each column normalised to sum 1.  Stops
+ * when every column's L1 change is below epsilon. */
+int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double beta, double epsilon,
+                                     int max_iterations, const int *h_sources, double *d_X,
+                                     int *nb_iterations, double *max_error, void *stream);
+
 /* CSR SpMV schedule (no reference counterpart): `lanes` cooperating on one
  * row (1..64, power of two) and `rows` each lane group keeps in flight
  * (1, 2 or 4); 0 restores the choice from the mean row length. */

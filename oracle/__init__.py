@@ -114,6 +114,8 @@ def lib() -> C.CDLL:
         _lib.orc_make_stochastic.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_page_rank.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, dp,
                                        dp]
+        _lib.orc_page_rank_source.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
+                                              C.c_int, C.c_int, dp, dp]
         _lib.orc_cg.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int, dp]
         _lib.orc_csr_spmm.argtypes = [C.c_void_p, dp, dp, C.c_int]
     return _lib
@@ -273,6 +275,14 @@ class Matrix:
         err = C.c_double()
         it = lib().orc_page_rank(self.fmt, self.ptr, beta, epsilon, max_iterations, _dp(b),
                                  C.byref(err))
+        return b[: self.s.g.ln_col], it, err.value
+
+    def page_rank_source(self, source, beta=0.85, epsilon=1e-10, max_iterations=10000):
+        """Personalised PageRank: teleport to `source` only."""
+        b = np.zeros(max(self.s.g.ln_col, 1))
+        err = C.c_double()
+        it = lib().orc_page_rank_source(self.fmt, self.ptr, beta, epsilon, max_iterations, source,
+                                        _dp(b), C.byref(err))
         return b[: self.s.g.ln_col], it, err.value
 
     def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False, x0=None):

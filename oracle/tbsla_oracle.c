@@ -1003,6 +1003,14 @@ void orc_make_stochastic(int fmt, void *m, double *s) {
  * b /= sum(b) (pagerank_norma_end :149-171).  Returns the iteration count. */
 int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_iterations,
                   double *b, double *err_out) {
+  return orc_page_rank_source(fmt, m, beta, epsilon, max_iterations, -1, b, err_out);
+}
+
+/* The same iteration with the teleport mass S(1 - beta) put on one vertex
+ * (`source` >= 0) instead of spread uniformly: personalised PageRank, one
+ * column of the multi-vector form.  No reference counterpart. */
+int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, int max_iterations,
+                         int source, double *b, double *err_out) {
   const orc_geom *g = geom_of(fmt, m);
   int ln_col = g->ln_col, ln_row = g->ln_row;
   double *b_t = (double *)zalloc((size_t)ln_col, sizeof(double));
@@ -1033,8 +1041,13 @@ int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_i
     for (int z = 0; z < length; z++) b[r_start + z] = buf[s_start + z];
     double sum = 0;
     for (int k = 0; k < ln_col; k++) sum += b_t[k];
-    double t_sum = sum * ((1 - beta) / g->n_col); /* t_sum *= (1-beta)/n_col, :123 */
-    for (int k = 0; k < ln_col; k++) b[k] = beta * b[k] + t_sum;
+    if (source < 0) {
+      double t_sum = sum * ((1 - beta) / g->n_col); /* t_sum *= (1-beta)/n_col, :123 */
+      for (int k = 0; k < ln_col; k++) b[k] = beta * b[k] + t_sum;
+    } else {
+      double tel = sum * (1 - beta);
+      for (int k = 0; k < ln_col; k++) b[k] = beta * b[k] + (k + g->f_col == source ? tel : 0.0);
+    }
     error = 0;
     for (int k = 0; k < ln_col; k++) {
       b[k] /= sum;
@@ -1110,14 +1123,16 @@ int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterati
 /* SpMM                                                                      */
 /* ------------------------------------------------------------------------ */
 
-/* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283): C = A B with B
- * (n_col x k) and C (n_row x k) row-major; C is zeroed first. */
+/* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283) in its block form
+ * (src/mpi/matrix/MatrixCSR.cpp:11-30): C = A B with B (ln_col x k) and C
+ * (ln_row x k) row-major, columns outside the block skipped; C is zeroed. */
 void orc_csr_spmm(const orc_csr *m, const double *B, double *C, int k) {
   const orc_geom *g = &m->g;
   memset(C, 0, sizeof(double) * (size_t)g->ln_row * (size_t)k);
   for (int i = 0; i < g->ln_row; i++)
     for (int j = m->rowptr[i]; j < m->rowptr[i + 1]; j++) {
       int col = m->colidx[j] - g->f_col;
+      if (col < 0 || col >= g->ln_col) continue; /* src/mpi/matrix/MatrixCSR.cpp:19 */
       double v = m->values[j];
       for (int kk = 0; kk < k; kk++) C[(size_t)i * k + kk] += v * B[(size_t)col * k + kk];
     }

@@ -120,6 +120,8 @@ void orc_normalize_cols(int fmt, void *m, const double *s);
 void orc_make_stochastic(int fmt, void *m, double *s);
 int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_iterations,
                   double *b, double *err_out);
+int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, int max_iterations,
+                         int source, double *b, double *err_out);
 int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
            double tolerance, int relative, double *res_norm_out);
 void orc_csr_spmm(const orc_csr *m, const double *B, double *C, int k);

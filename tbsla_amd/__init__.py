@@ -79,6 +79,9 @@ SIGNATURES = [
     ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
                                       C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_spmm", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_personalized_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _vp,
+                                                   _ip, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_set_csr_schedule", C.c_int, [_vp, C.c_int, C.c_int]),
     ("tbsla_hip_get_csr_schedule", C.c_int, [_vp, _ip, _ip]),
     ("tbsla_hip_autotune", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
@@ -332,6 +335,29 @@ class Matrix:
         check(lib().tbsla_hip_page_rank(self.h, beta, epsilon, max_iterations, db.ptr,
                                         C.byref(it), C.byref(err), None))
         return db.to_numpy(np.float64, n), it.value, err.value
+
+    def spmm(self, B) -> np.ndarray:
+        """MatrixCSR::dense_multiply: C = A B (B row-major ln_col x k)."""
+        B = np.ascontiguousarray(B, np.float64)
+        g = self.geometry()
+        k = B.shape[1]
+        dB, dC = DeviceBuffer.from_numpy(B), DeviceBuffer(max(g.ln_row * k, 1) * 8)
+        check(lib().tbsla_hip_spmm(self.h, dB.ptr, dC.ptr, k, None))
+        return dC.to_numpy(np.float64, g.ln_row * k).reshape(g.ln_row, k)
+
+    def personalized_page_rank(self, sources, beta=0.85, epsilon=1e-10, max_iterations=10000,
+                               out: "DeviceBuffer | None" = None):
+        """64 one-hot teleports at once; returns (X n x 64, iterations, max error)."""
+        src = np.ascontiguousarray(sources, np.int32)
+        assert len(src) == 64
+        n = self.geometry().ln_col
+        dX = out if out is not None else DeviceBuffer(n * 64 * 8)
+        it, err = C.c_int(), C.c_double()
+        check(lib().tbsla_hip_personalized_page_rank(
+            self.h, beta, epsilon, max_iterations, src.ctypes.data_as(_ip), dX.ptr, C.byref(it),
+            C.byref(err), None))
+        X = None if out is not None else dX.to_numpy(np.float64, n * 64).reshape(n, 64)
+        return X, it.value, err.value
 
     def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False,
            precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None):

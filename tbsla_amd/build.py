@@ -39,6 +39,28 @@ def _compile(src: str) -> str:
     return obj
 
 
+CPP_PROGRAMS = [
+    # (source, output, rpath relative to the output)
+    (os.path.join(PKG, "apps", "tbsla_perf_hip.cpp"), os.path.join(PKG, "bin", "tbsla_perf_hip"),
+     "$ORIGIN/.."),
+    (os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"),
+     os.path.join(ROOT, "tests", "cpp", "test_dropin"), "$ORIGIN/../../tbsla_amd"),
+]
+
+
+def build_programs() -> None:
+    """Host programs over the C ABI (plain g++, no HIP headers needed)."""
+    for src, out, rpath in CPP_PROGRAMS:
+        if not os.path.exists(src):
+            continue
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+               "-L" + PKG, "-ltbsla_hip", "-Wl,-rpath," + rpath]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {src}:\n{r.stdout}\n{r.stderr}")
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -50,6 +72,7 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_programs()
     if verbose:
         print(f"built {LIB} from {len(sources)} sources")
     return LIB

@@ -219,6 +219,40 @@ int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry 
   if (g->ln_row < 0 || g->ln_col < 0 || g->stored < 0)
     return fail(TBSLA_HIP_ERR_INVALID, "bad geometry");
   hipStream_t s = as_stream(stream);
+  // every index must lie inside the block: kernels index x, y and the
+  // scatter targets with them unchecked
+  {
+    const long long st = g->stored;
+    const long long r0 = g->f_row, r1 = (long long)g->f_row + g->ln_row;
+    const long long c0 = g->f_col, c1 = (long long)g->f_col + g->ln_col;
+    if (format == TBSLA_HIP_COO || format == TBSLA_HIP_SCOO) {
+      if (st > 0 && (!h_a || !h_b || !h_v)) return fail(TBSLA_HIP_ERR_INVALID, "missing arrays");
+      const bool coo = format == TBSLA_HIP_COO;
+      for (long long i = 0; i < st; i++) {
+        const long long r = h_a[i], c = h_b[i];
+        if (coo ? (r < 0 || r >= g->n_row || c < 0 || c >= g->n_col)
+                : (r < r0 || r >= r1 || c < c0 || c >= c1))
+          return fail(TBSLA_HIP_ERR_INVALID, "triplet " + std::to_string(i) + " outside the block");
+      }
+    } else if (format == TBSLA_HIP_CSR) {
+      if (!h_a) return fail(TBSLA_HIP_ERR_INVALID, "CSR import needs rowptr");
+      if (h_a[0] != 0) return fail(TBSLA_HIP_ERR_INVALID, "rowptr[0] must be 0");
+      for (long long i = 0; i < g->ln_row; i++)
+        if (h_a[i + 1] < h_a[i]) return fail(TBSLA_HIP_ERR_INVALID, "rowptr not monotone");
+      const long long n = h_a[g->ln_row];
+      if (n > 0 && (!h_b || !h_v)) return fail(TBSLA_HIP_ERR_INVALID, "missing arrays");
+      for (long long j = 0; j < n; j++)
+        if (h_b[j] < c0 || h_b[j] >= c1)
+          return fail(TBSLA_HIP_ERR_INVALID, "column " + std::to_string(j) + " outside the block");
+    } else if (format == TBSLA_HIP_ELL) {
+      const long long slots = (long long)g->ln_row * std::max(0, g->max_col);
+      for (long long j = 0; h_a && j < slots; j++) {
+        const bool pad = h_a[j] == 0 && (!h_v || h_v[j] == 0.0);
+        if (!pad && (h_a[j] < c0 || h_a[j] >= c1))
+          return fail(TBSLA_HIP_ERR_INVALID, "column " + std::to_string(j) + " outside the block");
+      }
+    }
+  }
   release_arrays(m);
   m->format = format;
   m->n_row = g->n_row;

@@ -58,24 +58,36 @@ unsigned red_grid(long long n) {
 // ---- column sums --------------------------------------------------------
 
 // key = local column, payload = the entry's position in the reference's loop
-// order (CSR/SCOO: storage order; ELL: row-major slot i*max_col+j).
+// order (CSR/SCOO: storage order; ELL: row-major slot i*max_col+j).  Entries
+// the reference skips -- ELL slots whose value is not positive
+// (MatrixELL.cpp:812), which include the padding (column 0, possibly left of
+// the block) -- and any column outside [0, ln_col) get the key ln_col, a
+// bucket after the last column that no sum reads.
 __global__ __launch_bounds__(kBlock) void k_colkeys(long long n, int fmt, long long ln_row,
-                                                    int max_col, const int *__restrict__ cols,
-                                                    int f_col, unsigned long long *__restrict__ keys,
+                                                    long long ln_col, int max_col,
+                                                    const int *__restrict__ cols,
+                                                    const double *__restrict__ vals, int f_col,
+                                                    unsigned long long *__restrict__ keys,
                                                     int *__restrict__ pos) {
   long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
   long long src = p;
   if (fmt == TBSLA_HIP_ELL) src = (p % max_col) * ln_row + p / max_col;
-  keys[p] = (unsigned long long)(unsigned)(cols[src] - f_col);
+  long long c = (long long)cols[src] - f_col;
+  bool keep = c >= 0 && c < ln_col && (fmt != TBSLA_HIP_ELL || vals[src] > 0);
+  keys[p] = keep ? (unsigned long long)c : (unsigned long long)ln_col;
   pos[p] = (int)p;
 }
 
+// counts[key]++ for keys <= ln_col (the last bucket collects skipped entries)
 __global__ __launch_bounds__(kBlock) void k_count_cols(const unsigned long long *__restrict__ keys,
-                                                       long long n, int *__restrict__ counts) {
+                                                       long long n, long long ln_col,
+                                                       int *__restrict__ counts) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock)
-    atomicAdd(counts + keys[i], 1);
+       i += (long long)gridDim.x * kBlock) {
+    unsigned long long k = keys[i];
+    if (k <= (unsigned long long)ln_col) atomicAdd(counts + k, 1);
+  }
 }
 
 // s[c] += the column's values in reference order (one thread per column):
@@ -105,15 +117,20 @@ __global__ __launch_bounds__(kBlock) void k_col_sums(long long ln_col, int fmt, 
 // normalize_cols: CSR (MatrixCSR.cpp:1221-1231) and SCOO
 // (MatrixSCOO.cpp:731-736) divide where s > 0; ELL also needs value > 0
 // (MatrixELL.cpp:824-832).
-__global__ __launch_bounds__(kBlock) void k_normalize_cols(long long n, int ell,
+// The value is tested before s is read, so padding slots (column 0, which
+// lies left of a block with f_col > 0) never index s.
+__global__ __launch_bounds__(kBlock) void k_normalize_cols(long long n, int ell, long long ln_col,
                                                            const int *__restrict__ cols, int f_col,
                                                            double *__restrict__ vals,
                                                            const double *__restrict__ s) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
-    double sv = s[cols[i] - f_col];
     double v = vals[i];
-    if (sv > 0 && (!ell || v > 0)) vals[i] = v / sv;
+    if (ell && !(v > 0)) continue;
+    long long c = (long long)cols[i] - f_col;
+    if (c < 0 || c >= ln_col) continue;
+    double sv = s[c];
+    if (sv > 0) vals[i] = v / sv;
   }
 }
 
@@ -498,20 +515,21 @@ extern "C" int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *s
   DevBuf<int> pos, colptr;
   TBSLA_TRY(keys.alloc(n));
   TBSLA_TRY(pos.alloc(n));
-  TBSLA_TRY(colptr.alloc(ln_col + 1));
-  k_colkeys<<<grid_for(n), kBlock, 0, s>>>(n, fmt, m->ln_row, m->max_col, m->colidx, m->f_col,
-                                           keys.p, pos.p);
+  TBSLA_TRY(colptr.alloc(ln_col + 2));  // ln_col columns + the skipped-entry bucket
+  k_colkeys<<<grid_for(n), kBlock, 0, s>>>(n, fmt, m->ln_row, ln_col, m->max_col, m->colidx,
+                                           m->values, m->f_col, keys.p, pos.p);
   TBSLA_HIP_LAUNCH_CHECK();
   int bits = 1;
-  while (bits < 62 && (1LL << bits) <= ln_col) bits++;
+  while (bits < 62 && (1LL << bits) <= ln_col) bits++;  // keys <= ln_col < 2^bits
   TBSLA_TRY(radix_sort_pairs(keys.p, pos.p, n, bits, s));
-  TBSLA_HIP_CHECK(hipMemsetAsync(colptr.p, 0, (size_t)(ln_col + 1) * 4, s));
-  k_count_cols<<<stride_grid(n), kBlock, 0, s>>>(keys.p, n, colptr.p + 1);
+  TBSLA_HIP_CHECK(hipMemsetAsync(colptr.p, 0, (size_t)(ln_col + 2) * 4, s));
+  k_count_cols<<<stride_grid(n), kBlock, 0, s>>>(keys.p, n, ln_col, colptr.p + 1);
   TBSLA_HIP_LAUNCH_CHECK();
   DevBuf<char> ws;
-  TBSLA_TRY(ws.alloc((long long)scan_workspace_bytes(ln_col) + 64));
-  TBSLA_TRY(scan<int>(colptr.p + 1, colptr.p + 1, ln_col, true,
-                      reinterpret_cast<long long *>(ws.p + scan_workspace_bytes(ln_col)), ws.p, s));
+  TBSLA_TRY(ws.alloc((long long)scan_workspace_bytes(ln_col + 1) + 64));
+  TBSLA_TRY(scan<int>(colptr.p + 1, colptr.p + 1, ln_col + 1, true,
+                      reinterpret_cast<long long *>(ws.p + scan_workspace_bytes(ln_col + 1)), ws.p,
+                      s));
   if (ln_col > 0) {
     k_col_sums<<<grid_for(ln_col), kBlock, 0, s>>>(ln_col, fmt, m->ln_row, m->max_col, colptr.p,
                                                    pos.p, m->values, d_s);
@@ -529,8 +547,8 @@ extern "C" int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s,
   if (m->nnz == 0 || !m->has_arrays) return TBSLA_HIP_SUCCESS;
   const long long n = fmt == TBSLA_HIP_ELL ? (long long)m->ln_row * m->max_col : m->stored;
   if (n > 0) {
-    k_normalize_cols<<<stride_grid(n), kBlock, 0, s>>>(n, fmt == TBSLA_HIP_ELL, m->colidx,
-                                                       m->f_col, m->values, d_s);
+    k_normalize_cols<<<stride_grid(n), kBlock, 0, s>>>(n, fmt == TBSLA_HIP_ELL, m->ln_col,
+                                                       m->colidx, m->f_col, m->values, d_s);
     TBSLA_HIP_LAUNCH_CHECK();
   }
   if (m->values32) {  // keep the FP32 copy in step

@@ -239,7 +239,7 @@ int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int ke
 // Stable permutation of COO triplets by (row, col) (csr::compare_row,
 // src/cpp/utils/csr.cpp:3-8).  perm receives n ints.
 static int coo_order(const tbsla_hip_matrix_s *coo, int *perm, hipStream_t s) {
-  const long long n = coo->nnz;
+  const long long n = std::min(coo->nnz, coo->stored);
   unsigned long long *keys = nullptr;
   TBSLA_TRY(device_alloc((void **)&keys, (size_t)std::max(n, 1LL) * 8));
   const int col_bits = bits_for(std::max(coo->n_col, 1));
@@ -281,7 +281,8 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
   dst->nnz = coo->nnz;
   dst->max_col = 0;
   dst->row_sorted = coo->row_sorted;
-  const long long n = coo->has_arrays ? coo->nnz : 0;
+  // the reference converts nnz triplets; never more than are held
+  const long long n = coo->has_arrays ? std::min(coo->nnz, coo->stored) : 0;
   const long long rows = dst->n_row;
 
   if (format == TBSLA_HIP_SCOO) {

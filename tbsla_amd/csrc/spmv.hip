@@ -355,7 +355,8 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       const long long len = coo ? m->n_row : m->ln_row;
       if (zero_first) TBSLA_TRY(launch_fill(d_r, len, 0.0, s));
       // SCOO returns when nnz == 0 (MatrixSCOO.cpp:155-157); COO loops over nnz
-      const long long n = coo ? m->nnz : (m->nnz == 0 ? 0 : m->stored);
+      // the triplets held (an import may carry a larger nnz field than arrays)
+      const long long n = (!coo && m->nnz == 0) ? 0 : m->stored;
       if (n <= 0 || !m->has_arrays) return TBSLA_HIP_SUCCESS;
       const int row_off = coo ? vect_incr : vect_incr - m->f_row;
       const int col_off = coo ? 0 : -m->f_col;

@@ -85,7 +85,8 @@ def test_page_rank_matches_oracle(fmt):
         # oracle's normalised CSR entries as triplets instead
         rp, ci, va = o.arrays()
         rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
-        g = T.Matrix.from_arrays(T.COO, o.geom, rows, ci, va)
+        geom = dict(o.geom, nnz=len(rows))
+        g = T.Matrix.from_arrays(T.COO, geom, rows, ci, va)
     else:
         g.make_stochastic()
     bg, itg, eg = g.page_rank(0.85, 1e-10, 10000)
