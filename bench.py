@@ -28,17 +28,21 @@ N, C, Q = 4194304, 32, 0.1
 
 
 CAL_BYTES = 256 << 20
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "csr_spmv_cqmat_n4194304_c32_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "csr_spmv_cqmat_n4194304_c32_traffic.txt")
 
 
 def measured_traffic():
     """HBM bytes per launch of the CSR kernel from the committed rocprofv3
     --pmc summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
-    calibration copy), or None when it has not been measured."""
+    calibration copy): its `traffic_bytes_per_launch:` line, or None when it
+    has not been measured."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def algorithmic_bytes_csr(stored, n_rows, n_cols):
@@ -104,7 +108,7 @@ def main():
     # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
     # give the scale factor applied to the SpMV kernel's counters
     cal = T.DeviceBuffer(CAL_BYTES), T.DeviceBuffer(CAL_BYTES)
-    T.check(T.lib().tbsla_hip_memcpy_dtod(cal[1].ptr, cal[0].ptr, CAL_BYTES, None))
+    T.check(T.lib().tbsla_hip_copy_kernel(cal[1].ptr, cal[0].ptr, CAL_BYTES, None))
     T.check(T.lib().tbsla_hip_stream_synchronize(None))
     del cal
     traffic = measured_traffic()

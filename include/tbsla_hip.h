@@ -78,6 +78,9 @@ int tbsla_hip_memcpy_htod(void *d_dst, const void *h_src, size_t bytes, void *st
 int tbsla_hip_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes, void *stream);
 int tbsla_hip_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int tbsla_hip_stream_synchronize(void *stream);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and
+ * stores), e.g. as a calibration dispatch for hardware counters. */
+int tbsla_hip_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int tbsla_hip_device_count(int *count);
 
 /* ---- handles ------------------------------------------------------------ */
@@ -186,6 +189,9 @@ typedef struct {
   int preconditioner;    /* TBSLA_HIP_PRECOND_NONE or _JACOBI (CSR only) */
   int replace_every;     /* > 0: replace r by the true FP64 residual every k iterations */
   int use_initial_guess; /* 0: x starts at 0 (as the reference app); 1: from d_x */
+  double near_zero;      /* clamp of alpha's and beta's divisors (CG.cpp:35): 0 selects the
+                            reference's 1e-10 in its own setting (absolute test, FP64, no
+                            preconditioner) and none otherwise; > 0 that value; < 0 none */
 } tbsla_hip_cg_options;
 
 /* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call

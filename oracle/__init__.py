@@ -117,6 +117,8 @@ def lib() -> C.CDLL:
         _lib.orc_page_rank_source.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
                                               C.c_int, C.c_int, dp, dp]
         _lib.orc_cg.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int, dp]
+        _lib.orc_cg_clamp.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int,
+                                      C.c_double, dp]
         _lib.orc_csr_spmm.argtypes = [C.c_void_p, dp, dp, C.c_int]
     return _lib
 
@@ -285,12 +287,16 @@ class Matrix:
                                         _dp(b), C.byref(err))
         return b[: self.s.g.ln_col], it, err.value
 
-    def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False, x0=None):
+    def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False, x0=None,
+           near_zero=0.0):
+        """near_zero: divisor clamp; 0 = the reference's 1e-10 for the
+        absolute test, none for the relative one; < 0 none."""
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         x = np.zeros_like(rhs) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
         rn = C.c_double()
-        it = lib().orc_cg(self.fmt, self.ptr, _dp(rhs), _dp(x), max_iterations, tolerance,
-                          int(relative), C.byref(rn))
+        nz = near_zero if near_zero > 0 else (1e-300 if (near_zero < 0 or relative) else 1e-10)
+        it = lib().orc_cg_clamp(self.fmt, self.ptr, _dp(rhs), _dp(x), max_iterations, tolerance,
+                                int(relative), nz, C.byref(rn))
         return x, it, rn.value
 
     def spmm(self, B: np.ndarray) -> np.ndarray:

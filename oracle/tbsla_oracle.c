@@ -1086,7 +1086,16 @@ static double dot(const double *a, const double *b, int n) {
  * that did not converge). */
 int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
            double tolerance, int relative, double *res_norm_out) {
-  const double NEARZERO = 1.0e-10;
+  return orc_cg_clamp(fmt, m, rhs, x, max_iterations, tolerance, relative,
+                      relative ? 1e-300 : 1.0e-10, res_norm_out);
+}
+
+/* The same iteration with the divisor clamp (NEARZERO, CG.cpp:35) given:
+ * the reference's absolute 1e-10 stalls the iteration once r.r drops below
+ * it, so the relative mode uses a guard against exact zero only. */
+int orc_cg_clamp(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
+                 double tolerance, int relative, double near_zero, double *res_norm_out) {
+  const double NEARZERO = near_zero;
   const orc_geom *g = geom_of(fmt, m);
   int n = g->n_col;
   double *R = (double *)zalloc((size_t)n, sizeof(double));

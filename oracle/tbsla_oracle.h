@@ -124,6 +124,8 @@ int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, in
                          int source, double *b, double *err_out);
 int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
            double tolerance, int relative, double *res_norm_out);
+int orc_cg_clamp(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
+                 double tolerance, int relative, double near_zero, double *res_norm_out);
 void orc_csr_spmm(const orc_csr *m, const double *B, double *C, int k);
 
 #ifdef __cplusplus

@@ -30,7 +30,7 @@ _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 class CGOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("tolerance", C.c_double), ("relative", C.c_int),
                 ("precision", C.c_int), ("preconditioner", C.c_int), ("replace_every", C.c_int),
-                ("use_initial_guess", C.c_int)]
+                ("use_initial_guess", C.c_int), ("near_zero", C.c_double)]
 
 
 CG_FP64, CG_MIXED = 0, 1
@@ -54,6 +54,7 @@ SIGNATURES = [
     ("tbsla_hip_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     ("tbsla_hip_memcpy_dtod", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     ("tbsla_hip_stream_synchronize", C.c_int, [_vp]),
+    ("tbsla_hip_copy_kernel", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     ("tbsla_hip_device_count", C.c_int, [_ip]),
     ("tbsla_hip_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
     ("tbsla_hip_destroy", C.c_int, [_vp]),
@@ -360,13 +361,14 @@ class Matrix:
         return X, it.value, err.value
 
     def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False,
-           precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None):
+           precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None,
+           near_zero=0.0):
         """Corrected tbsla::mpi::Matrix::CG; returns (x, iterations, |r|)."""
         drhs = _dev(rhs)
         n = self.geometry().ln_col
         dx = _dev(np.zeros(n) if x0 is None else x0)
         o = CGOptions(max_iterations, tolerance, int(relative), precision, preconditioner,
-                      replace_every, int(x0 is not None))
+                      replace_every, int(x0 is not None), near_zero)
         it, rn = C.c_int(), C.c_double()
         check(lib().tbsla_hip_cg(self.h, drhs.ptr, dx.ptr, C.byref(o), C.byref(it),
                                  C.byref(rn), None))

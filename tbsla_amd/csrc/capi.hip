@@ -67,6 +67,22 @@ using namespace tbsla_hip;
 
 static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
+typedef double v2d_t __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(kBlock) void k_copy16(const v2d_t *__restrict__ a,
+                                                   v2d_t *__restrict__ b, long long n16) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16;
+       i += (long long)gridDim.x * kBlock)
+    b[i] = __builtin_nontemporal_load(a + i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy1(const char *__restrict__ a, char *__restrict__ b,
+                                                  long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    b[i] = a[i];
+}
+
 extern "C" {
 
 const char *tbsla_hip_status_string(int status) {
@@ -113,6 +129,26 @@ int tbsla_hip_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *st
   if (bytes == 0) return TBSLA_HIP_SUCCESS;
   TBSLA_HIP_CHECK(
       hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+  if (bytes == 0) return TBSLA_HIP_SUCCESS;
+  if (!d_dst || !d_src) return fail(TBSLA_HIP_ERR_INVALID, "null pointer");
+  hipStream_t s = as_stream(stream);
+  const bool aligned = ((uintptr_t)d_dst % 16 == 0) && ((uintptr_t)d_src % 16 == 0);
+  const long long n16 = aligned ? (long long)(bytes / 16) : 0;
+  if (n16 > 0) {
+    k_copy16<<<8192, kBlock, 0, s>>>(static_cast<const v2d_t *>(d_src), static_cast<v2d_t *>(d_dst),
+                                     n16);
+    TBSLA_HIP_LAUNCH_CHECK();
+  }
+  const long long rest = (long long)bytes - 16 * n16;
+  if (rest > 0) {
+    k_copy1<<<grid_for(rest), kBlock, 0, s>>>(static_cast<const char *>(d_src) + 16 * n16,
+                                              static_cast<char *>(d_dst) + 16 * n16, rest);
+    TBSLA_HIP_LAUNCH_CHECK();
+  }
   return TBSLA_HIP_SUCCESS;
 }
 

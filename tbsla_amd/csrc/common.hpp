@@ -170,6 +170,10 @@ __device__ inline double group_sum(double v) {
   return v;
 }
 
+// Inside a kernel that must round a*b+c twice, as the reference's loops do
+// (HIP compiles with fp-contract=fast, which would fuse them).
+#define TBSLA_NO_FMA _Pragma("clang fp contract(off)")
+
 template <typename T>
 __device__ inline T ldg_stream(const T *p) {
   return __builtin_nontemporal_load(p);

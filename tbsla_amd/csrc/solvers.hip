@@ -22,6 +22,7 @@ constexpr int kRedBlocks = 512;  // reduction grid (fixed: the partial layout is
 constexpr double kNearZero = 1.0e-10;  // CG.cpp:35
 
 struct SolverState {
+  double near_zero;  // CG: clamp of alpha's and beta's divisors
   double S;      // PageRank: sum of the current iterate
   double err;    // PageRank: last L1 change
   double rr;     // CG: r.r (or r.z with a preconditioner) of the current residual
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_update(long long ln_col, const do
                                                       const SolverState *__restrict__ st,
                                                       double *__restrict__ p_err,
                                                       double *__restrict__ p_sum) {
+  TBSLA_NO_FMA
   if (st->done) return;
   const double S = st->S;
   const double t_sum = S * ((1 - beta) / n_col);  // t_sum *= (1-beta)/n_col
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(SolverState *__restrict__ s
                                                      const double *__restrict__ part, int nb) {
   if (st->done) return;
   double pap = sum_partials(part, nb);
-  if (threadIdx.x == 0) st->alpha = st->rr / fmax(pap, kNearZero);
+  if (threadIdx.x == 0) st->alpha = st->rr / fmax(pap, st->near_zero);
 }
 
 // x += alpha p; r -= alpha Ap (CG.cpp:64-65, products rounded before the
@@ -241,6 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(long long n, double *__res
                                                       const SolverState *__restrict__ st,
                                                       double *__restrict__ p_rr,
                                                       double *__restrict__ p_rz) {
+  TBSLA_NO_FMA
   if (st->done) return;
   const double alpha = st->alpha;
   double rr = 0.0, rz = 0.0;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_beta(SolverState *__restrict__ st
       return;
     }
     double next = precond ? rz : rr;
-    st->beta = next / fmax(st->rr, kNearZero);
+    st->beta = next / fmax(st->rr, st->near_zero);
     st->rr = next;
     st->iters += 1;
   }
@@ -290,6 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(long long n, double *__restrict
                                                  const double *__restrict__ dinv,
                                                  float *__restrict__ p32,
                                                  const SolverState *__restrict__ st) {
+  TBSLA_NO_FMA
   if (st->done) return;
   const double beta = st->beta;
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
@@ -330,10 +334,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(long long n, const double *
 __global__ __launch_bounds__(kBlock) void k_cg_init_state(SolverState *__restrict__ st,
                                                           const double *__restrict__ p_rr,
                                                           const double *__restrict__ p_rz, int nb,
-                                                          int precond) {
+                                                          int precond, double near_zero) {
   double rr = sum_partials(p_rr, nb);
   double rz = precond ? sum_partials(p_rz, nb) : 0.0;
   if (threadIdx.x == 0) {
+    st->near_zero = near_zero;
     st->rr = precond ? rz : rr;
     st->rnorm = sqrt(rr);
     st->iters = 0;
@@ -354,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_check_true(SolverState *__restric
     st->rnorm = rnorm;
     if (rnorm >= stop) {
       double next = precond ? rz : rr;
-      st->beta = next / fmax(st->rr, kNearZero);
+      st->beta = next / fmax(st->rr, st->near_zero);
       st->rr = next;
       st->iters += 1;
       st->done = 0;
@@ -367,6 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p_unguarded(long long n, double *
                                                            const double *__restrict__ dinv,
                                                            float *__restrict__ p32,
                                                            const SolverState *__restrict__ st) {
+  TBSLA_NO_FMA
   if (st->done) return;
   const double beta = st->beta;
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
@@ -678,7 +684,14 @@ extern "C" int tbsla_hip_cg(tbsla_hip_matrix_t m, const double *d_rhs, double *d
     TBSLA_TRY(launch_fill(AP.p, n, 0.0, s));
     k_residual<<<nb, kBlock, 0, s>>>(n, d_rhs, AP.p, R.p, dinv.p, nullptr, p_rr, p_rz);
   }
-  k_cg_init_state<<<1, kBlock, 0, s>>>(st.p, p_rr, p_rz, (int)nb, jacobi ? 1 : 0);
+  // The reference clamps both divisors at an absolute 1e-10 (CG.cpp:35,63,69),
+  // which stalls the iteration once r.r drops below it; it is kept for the
+  // reference's own setting (absolute test, FP64, no preconditioner) and
+  // otherwise replaced by a guard against exact zero unless set explicitly.
+  const bool reference_setting = !o.relative && !mixed && !jacobi;
+  double near_zero = o.near_zero > 0 ? o.near_zero
+                     : (o.near_zero < 0 ? 1e-300 : (reference_setting ? kNearZero : 1e-300));
+  k_cg_init_state<<<1, kBlock, 0, s>>>(st.p, p_rr, p_rz, (int)nb, jacobi ? 1 : 0, near_zero);
   k_cg_init_p<<<stride_grid(n), kBlock, 0, s>>>(n, P.p, R.p, dinv.p, P32.p);
   TBSLA_HIP_LAUNCH_CHECK();
   double stop = o.tolerance;

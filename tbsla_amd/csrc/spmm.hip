@@ -26,6 +26,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, in
                                                      const double *__restrict__ vals,
                                                      const double *__restrict__ B,
                                                      double *__restrict__ C) {
+  TBSLA_NO_FMA
   const long long row = __builtin_amdgcn_readfirstlane(
       (int)(((long long)blockIdx.x * kBlock + threadIdx.x) / kWave));
   const int lane = threadIdx.x & (kWave - 1);
@@ -74,6 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_step(long long n_rows, int f_col
                                                      const PprState *__restrict__ st,
                                                      double *__restrict__ p_err,
                                                      double *__restrict__ p_sum) {
+  TBSLA_NO_FMA
   if (st->done) return;
   __shared__ double lds_e[kBlock / kWave][kWave];
   __shared__ double lds_s[kBlock / kWave][kWave];

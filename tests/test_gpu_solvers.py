@@ -123,18 +123,37 @@ def test_cg_early_iterates_match_oracle():
 
 
 def test_cg_converges_like_oracle():
+    """Without the divisor clamp: same iteration count as the oracle within
+    two, the solution of a direct solve."""
     n = 2000
     A = laplacian_1d(n) + np.diag(np.linspace(0.01, 1.0, n))
     o = O.csr_from_dense(A)
     g = to_gpu_csr(o)
     rhs = 2 * O.unit_vector(11, n) - 1
     exact = np.linalg.solve(A, rhs)
-    xo, ito, ro = o.cg(rhs, tolerance=1e-10)
-    xg, itg, rg = g.cg(rhs, tolerance=1e-10)
+    xo, ito, ro = o.cg(rhs, tolerance=1e-10, near_zero=-1)
+    xg, itg, rg = g.cg(rhs, tolerance=1e-10, near_zero=-1)
     assert abs(itg - ito) <= 2, (itg, ito)
     assert rg < 1e-10
     assert np.linalg.norm(A @ xg - rhs) < 2e-10
     assert rel_max(xg, exact) < 1e-8
+
+
+def test_cg_reference_clamp_like_oracle():
+    """The reference's absolute 1e-10 clamp (the default for the absolute
+    test): the GPU follows the oracle's clamped iteration, which needs
+    several times the unclamped count (656 against 140 to 1e-10 here)."""
+    n = 2000
+    A = laplacian_1d(n) + np.diag(np.linspace(0.01, 1.0, n))
+    o = O.csr_from_dense(A)
+    g = to_gpu_csr(o)
+    rhs = 2 * O.unit_vector(11, n) - 1
+    xo, ito, ro = o.cg(rhs, max_iterations=3000, tolerance=1e-6)
+    xg, itg, rg = g.cg(rhs, max_iterations=3000, tolerance=1e-6)
+    assert abs(itg - ito) <= max(2, 0.02 * ito), (itg, ito)
+    assert rg < 1e-6 and ro < 1e-6
+    _, it_free, _ = g.cg(rhs, max_iterations=3000, tolerance=1e-6, near_zero=-1)
+    assert it_free < itg / 2
 
 
 def test_cg_relative_jacobi_and_mixed():
