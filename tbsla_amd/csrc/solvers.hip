@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_update(long long ln_col, const do
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < ln_col;
        k += (long long)gridDim.x * kBlock) {
     double v = (k >= r_start && k < r_start + length) ? y[s_start + k - r_start] : 0.0;
-    v = __dadd_rn(__dmul_rn(beta, v), t_sum);
+    v = beta * v + t_sum;
     v = v / S;
     b[k] = v;
     e += fabs(v - bt[k]);
@@ -249,8 +249,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(long long n, double *__res
   double rr = 0.0, rz = 0.0;
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
        k += (long long)gridDim.x * kBlock) {
-    x[k] = __dadd_rn(x[k], __dmul_rn(alpha, p[k]));
-    double rk = __dadd_rn(r[k], __dmul_rn(-alpha, ap[k]));
+    x[k] = x[k] + alpha * p[k];
+    double rk = r[k] + (-alpha) * ap[k];
     r[k] = rk;
     rr = fma(rk, rk, rr);
     if (dinv) rz = fma(rk, rk * dinv[k], rz);
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(long long n, double *__restrict
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
        k += (long long)gridDim.x * kBlock) {
     double z = dinv ? r[k] * dinv[k] : r[k];
-    double v = __dadd_rn(z, __dmul_rn(beta, p[k]));
+    double v = z + beta * p[k];
     p[k] = v;
     if (p32) p32[k] = (float)v;
   }
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p_unguarded(long long n, double *
   for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
        k += (long long)gridDim.x * kBlock) {
     double z = dinv ? r[k] * dinv[k] : r[k];
-    double v = __dadd_rn(z, __dmul_rn(beta, p[k]));
+    double v = z + beta * p[k];
     p[k] = v;
     if (p32) p32[k] = (float)v;
   }

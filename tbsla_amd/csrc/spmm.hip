@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, in
 #pragma unroll
     for (int t = 0; t < KT; t++) {
       const int col = lane + t * kWave;
-      if (col < k) acc[t] = __dadd_rn(acc[t], __dmul_rn(v, b[col]));
+      if (col < k) acc[t] = acc[t] + v * b[col];
     }
   }
   double *out = C + (size_t)row * k;
@@ -94,9 +94,9 @@ __global__ __launch_bounds__(kBlock) void k_ppr_step(long long n_rows, int f_col
     for (int j = beg; j < end; j++) {
       const int c = colidx[j] - f_col;
       if (c < 0 || c >= ln_col) continue;
-      acc = __dadd_rn(acc, __dmul_rn(vals[j], X[(size_t)c * kWave + lane]));
+      acc = acc + vals[j] * X[(size_t)c * kWave + lane];
     }
-    double v = __dadd_rn(__dmul_rn(beta, acc), (r + f_col == sk) ? tel : 0.0);
+    double v = beta * acc + ((r + f_col == sk) ? tel : 0.0);
     v = v / Sk;
     const size_t o = (size_t)r * kWave + lane;
     e += fabs(v - X[o]);
