@@ -54,7 +54,8 @@ enum {
   TBSLA_HIP_ERR_HIP = 2,         /* a HIP runtime call failed */
   TBSLA_HIP_ERR_ALLOC = 3,       /* device allocation failed */
   TBSLA_HIP_ERR_UNSUPPORTED = 4, /* operation not defined for this format */
-  TBSLA_HIP_ERR_RANGE = 5        /* size beyond int32 indexing */
+  TBSLA_HIP_ERR_RANGE = 5,       /* size beyond int32 indexing */
+  TBSLA_HIP_ERR_BREAKDOWN = 6    /* CG: r.z <= 0 or not finite, or no progress (see stall_window) */
 };
 
 /* Geometry of a block: the protected fields of tbsla::cpp::Matrix
@@ -106,6 +107,40 @@ int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, int c, doub
                          long long value_seed, void *stream);
 /* x_i = V(seed, i) for i < n into a device vector (the synthetic inputs). */
 int tbsla_hip_fill_unit_vector(double *d_x, long long n, long long seed, void *stream);
+/* d_v[i] = lo + (hi - lo) V(seed, i) (lo = 0, hi = 1 equals fill_unit_vector). */
+int tbsla_hip_fill_uniform_vector(double *d_v, long long n, unsigned long long seed, double lo,
+                                  double hi, void *stream);
+
+/* ---- SPD problems (benchmark inputs; no counterpart in the reference) ----
+ * The reference's CG driver (src/mpi/matrix/CG.cpp:32-73) is fed cdiag /
+ * cqmat blocks; these generators give it SPD systems.  All fill a square
+ * single-block CSR handle with columns sorted within each row. */
+/* -div(k grad u) on an nx*ny*nz cell grid, 7-point, Dirichlet boundaries;
+ * cell (x,y,z) is row x + nx (y + ny z).  Face coefficient 2 k_a k_b /
+ * (k_a + k_b); a boundary face adds k_cell to the diagonal.  d_kappa (device,
+ * nx*ny*nz, nullable): NULL means k = 1 (diagonal 6, off-diagonal -1). */
+int tbsla_hip_fill_poisson3d(tbsla_hip_matrix_t m, int nx, int ny, int nz, const double *d_kappa,
+                             void *stream);
+/* k = exp(mu + sigma z) on the grid: z is standard normal noise (Box-Muller
+ * on V(seed, .)) under a separable Gaussian filter of standard deviation
+ * corr_len cells (radius ceil(3 corr_len), reflecting boundaries), scaled to
+ * unit variance away from the boundary. */
+int tbsla_hip_fill_lognormal_field(double *d_kappa, int nx, int ny, int nz, double mu,
+                                   double sigma, double corr_len, unsigned long long seed,
+                                   void *stream);
+/* A = (B + B^T)/2 with diag(A) = sum_{j != i} |A_ij| + 1, where B is
+ * fill_cqmat(n, n, c, q, seed_mult, 0, 0, 1, 1, value_seed) as CSR.  Each
+ * A_ij = (sum of B_ij + sum of B_ji) * 0.5, each sum in generation order. */
+int tbsla_hip_fill_sym_cqmat(tbsla_hip_matrix_t m, int n, int c, double q, unsigned seed_mult,
+                             long long value_seed, void *stream);
+/* dst = src^T for a CSR block (geometry mirrored; rows keep source-row
+ * order).  MatrixCSR has no transpose in the reference. */
+int tbsla_hip_transpose(tbsla_hip_matrix_t dst, tbsla_hip_matrix_t src, void *stream);
+/* dst = diag(parts[0], ..., parts[k-1]) as one CSR block (local columns of
+ * each part shifted by the columns before it): the layout of the batched
+ * solvers. */
+int tbsla_hip_block_diag(tbsla_hip_matrix_t dst, const tbsla_hip_matrix_t *parts, int k,
+                         void *stream);
 
 /* ---- import / export ---------------------------------------------------- */
 /* The arrays of the reference layout, copied to the host (the fields a
@@ -178,31 +213,66 @@ int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double epsilon, int m
                         double *d_b, int *nb_iterations, double *final_error, void *stream);
 
 enum { TBSLA_HIP_CG_FP64 = 0, TBSLA_HIP_CG_MIXED = 1 };
-enum { TBSLA_HIP_PRECOND_NONE = 0, TBSLA_HIP_PRECOND_JACOBI = 1 };
+enum { TBSLA_HIP_PRECOND_NONE = 0, TBSLA_HIP_PRECOND_JACOBI = 1, TBSLA_HIP_PRECOND_FACTORED = 2 };
 
 typedef struct {
   int max_iterations;    /* reference default 10000 */
   double tolerance;      /* stop when |r| < tolerance (x |rhs| if relative); CG.cpp:34 */
   int relative;          /* 0: absolute test as the reference; 1: relative */
-  int precision;         /* TBSLA_HIP_CG_FP64, or _MIXED: FP32 matrix values, FP32 SpMV
-                            accumulation, FP64 vectors and dot products (CSR only) */
-  int preconditioner;    /* TBSLA_HIP_PRECOND_NONE or _JACOBI (CSR only) */
+  int precision;         /* TBSLA_HIP_CG_FP64, or _MIXED: FP32 matrix and preconditioner
+                            values with FP32 accumulation; FP64 vectors and dot products */
+  int preconditioner;    /* TBSLA_HIP_PRECOND_NONE, _JACOBI (D^-1) or _FACTORED (M = G G^T) */
   int replace_every;     /* > 0: replace r by the true FP64 residual every k iterations */
   int use_initial_guess; /* 0: x starts at 0 (as the reference app); 1: from d_x */
   double near_zero;      /* clamp of alpha's and beta's divisors (CG.cpp:35): 0 selects the
                             reference's 1e-10 in its own setting (absolute test, FP64, no
                             preconditioner) and none otherwise; > 0 that value; < 0 none */
+  tbsla_hip_matrix_t factor;   /* _FACTORED: G, CSR, same shape as the matrix */
+  tbsla_hip_matrix_t factor_t; /* _FACTORED: G^T as CSR, or NULL (built per call) */
+  int graph_batch;       /* iterations per captured hipGraph: 0 picks (16, or a multiple
+                            of replace_every); < 0 launches every kernel directly */
+  int lanes_per_row;     /* lanes per matrix row in the iteration's SpMV (power of two
+                            <= 64); 0 picks from the mean row length */
+  int stall_window;      /* > 0: breakdown when |r| >= |r0| after that many iterations */
 } tbsla_hip_cg_options;
 
 /* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call
  * corrected (:61 passes output and input swapped, so the reference never
- * moves): alpha = r.r / max(p.Ap, 1e-10), x += alpha p, r -= alpha Ap, stop
- * on |r|, beta = r.r / max(r_old.r_old, 1e-10), p = r + beta p.  opt NULL:
- * the reference's settings.  *nb_iterations counts the iterations that did
- * not converge (:71); *res_norm is |r| of the recurrence at exit. */
+ * moves): alpha = r.z / max(p.Ap, nz), x += alpha p, r -= alpha Ap, z = M r,
+ * stop on |r|, beta = r.z / max(r_old.z_old, nz), p = z + beta p (z = r
+ * without a preconditioner, as the reference).  opt NULL: the reference's
+ * settings.  *nb_iterations counts the iterations that did not converge
+ * (:71); *res_norm is |r| at exit (the true residual's in the mixed form).
+ * Returns TBSLA_HIP_ERR_BREAKDOWN when the iteration broke down. */
 int tbsla_hip_cg(tbsla_hip_matrix_t m, const double *d_rhs, double *d_x,
                  const tbsla_hip_cg_options *opt, int *nb_iterations, double *res_norm,
                  void *stream);
+
+/* Outcome of one system of tbsla_hip_cg_batched. */
+enum {
+  TBSLA_HIP_CG_CONVERGED = 0,
+  TBSLA_HIP_CG_MAX_ITERATIONS = 1,
+  TBSLA_HIP_CG_BREAKDOWN = 2,
+  TBSLA_HIP_CG_INACTIVE = 3
+};
+typedef struct {
+  int iterations;  /* as tbsla_hip_cg's *nb_iterations */
+  int status;      /* TBSLA_HIP_CG_* */
+  double res_norm; /* |r| at exit */
+  double time_us;  /* device wall clock from the first iteration to this system's end
+                      (-1 if it did not finish) */
+} tbsla_hip_cg_result;
+
+/* Many independent CG solves at once (no counterpart in the reference): the
+ * systems are the diagonal blocks of m, system k owning rows [h_row_offsets[k],
+ * h_row_offsets[k+1]) (tbsla_hip_block_diag builds such an m).  One kernel
+ * sequence per iteration serves all systems; each has its own scalars,
+ * threshold and convergence, and stops on its own.  h_active (nullable, nsys
+ * ints): 0 skips a system.  d_rhs / d_x cover all rows. */
+int tbsla_hip_cg_batched(tbsla_hip_matrix_t m, int nsys, const int *h_row_offsets,
+                         const int *h_active, const double *d_rhs, double *d_x,
+                         const tbsla_hip_cg_options *opt, tbsla_hip_cg_result *h_results,
+                         void *stream);
 
 /* ---- SpMM and personalised PageRank ------------------------------------- */
 /* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283; block form

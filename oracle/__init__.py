@@ -120,6 +120,14 @@ def lib() -> C.CDLL:
         _lib.orc_cg_clamp.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int,
                                       C.c_double, dp]
         _lib.orc_csr_spmm.argtypes = [C.c_void_p, dp, dp, C.c_int]
+        _lib.orc_poisson3d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]
+        _lib.orc_uniform_vector.argtypes = [dp, C.c_long, C.c_uint64, C.c_double, C.c_double]
+        _lib.orc_lognormal_field.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_double, C.c_double, C.c_uint64]
+        _lib.orc_sym_cqmat.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.orc_csr_transpose.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.orc_pcg.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                 C.c_int, C.c_void_p, dp]
     return _lib
 
 
@@ -353,6 +361,51 @@ def csr_from_dense(A: np.ndarray) -> Matrix:
     rowptr = np.zeros(A.shape[0] + 1, np.int64)
     np.add.at(rowptr, rows + 1, 1)
     return csr_from_arrays(A.shape[0], A.shape[1], np.cumsum(rowptr), cols, A[rows, cols])
+
+
+def poisson3d(nx, ny, nz, kappa=None) -> Matrix:
+    """7-point -div(k grad u) with Dirichlet boundaries (orc_poisson3d)."""
+    m = Matrix(CSR)
+    k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+    lib().orc_poisson3d(m.ptr, nx, ny, nz, None if k is None else _dp(k))
+    return m
+
+
+def uniform_vector(n, seed, lo=0.0, hi=1.0) -> np.ndarray:
+    out = np.zeros(n)
+    lib().orc_uniform_vector(_dp(out), n, seed, lo, hi)
+    return out
+
+
+def lognormal_field(nx, ny, nz, mu=0.0, sigma=1.0, corr_len=4.0, seed=0) -> np.ndarray:
+    out = np.zeros(nx * ny * nz)
+    lib().orc_lognormal_field(_dp(out), nx, ny, nz, mu, sigma, corr_len, seed)
+    return out
+
+
+def sym_cqmat(n, c, q, seed_mult=1, value_seed=0) -> Matrix:
+    """A = (B + B^T)/2, diagonal = off-diagonal |row| sum + 1 (orc_sym_cqmat)."""
+    B = fill_cqmat(CSR, n, n, c, q, seed_mult, value_seed=value_seed)
+    A = Matrix(CSR)
+    lib().orc_sym_cqmat(A.ptr, B.ptr)
+    return A
+
+
+def transpose(A: Matrix) -> Matrix:
+    T = Matrix(CSR)
+    lib().orc_csr_transpose(T.ptr, A.ptr)
+    return T
+
+
+def pcg(A: Matrix, rhs, max_iterations=10000, tolerance=1e-8, relative=True, near_zero=1e-300,
+        kind=0, G: Matrix | None = None):
+    """Preconditioned CG from x = 0 (kind 0 none, 1 Jacobi, 2 M = G G^T)."""
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    x = np.zeros_like(rhs)
+    rn = C.c_double()
+    it = lib().orc_pcg(A.ptr, _dp(rhs), _dp(x), max_iterations, tolerance, int(relative),
+                       near_zero, kind, None if G is None else G.ptr, C.byref(rn))
+    return x, it, rn.value
 
 
 def coo_from_arrays(n_row, n_col, row, col, values) -> Matrix:

@@ -126,6 +126,18 @@ int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterati
            double tolerance, int relative, double *res_norm_out);
 int orc_cg_clamp(int fmt, const void *m, const double *rhs, double *x, int max_iterations,
                  double tolerance, int relative, double near_zero, double *res_norm_out);
+/* problems.c: SPD generators and preconditioned CG (no reference routine;
+ * the GPU's tbsla_hip_fill_poisson3d / _lognormal_field / _sym_cqmat /
+ * _transpose / _uniform_vector and the PCG of tbsla_hip_cg). */
+int orc_poisson3d(orc_csr *m, int nx, int ny, int nz, const double *kappa);
+void orc_uniform_vector(double *out, long n, uint64_t seed, double lo, double hi);
+void orc_lognormal_field(double *out, int nx, int ny, int nz, double mu, double sigma,
+                         double corr_len, uint64_t seed);
+int orc_sym_cqmat(orc_csr *A, const orc_csr *B);
+int orc_csr_transpose(orc_csr *T, const orc_csr *A);
+/* kind 0: none, 1: Jacobi, 2: M = G G^T; x starts at 0 */
+int orc_pcg(const orc_csr *A, const double *rhs, double *x, int max_iterations, double tolerance,
+            int relative, double near_zero, int kind, const orc_csr *G, double *res_norm_out);
 void orc_csr_spmm(const orc_csr *m, const double *B, double *C, int k);
 
 #ifdef __cplusplus

@@ -30,11 +30,20 @@ _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 class CGOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("tolerance", C.c_double), ("relative", C.c_int),
                 ("precision", C.c_int), ("preconditioner", C.c_int), ("replace_every", C.c_int),
-                ("use_initial_guess", C.c_int), ("near_zero", C.c_double)]
+                ("use_initial_guess", C.c_int), ("near_zero", C.c_double),
+                ("factor", C.c_void_p), ("factor_t", C.c_void_p), ("graph_batch", C.c_int),
+                ("lanes_per_row", C.c_int), ("stall_window", C.c_int)]
+
+
+class CGResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("res_norm", C.c_double),
+                ("time_us", C.c_double)]
 
 
 CG_FP64, CG_MIXED = 0, 1
-PRECOND_NONE, PRECOND_JACOBI = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_FACTORED = 0, 1, 2
+CG_CONVERGED, CG_MAX_ITERATIONS, CG_BREAKDOWN, CG_INACTIVE = 0, 1, 2, 3
+ERR_BREAKDOWN = 6
 
 
 class Geometry(C.Structure):
@@ -63,6 +72,15 @@ SIGNATURES = [
     ("tbsla_hip_fill_cqmat", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint,
                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, _vp]),
     ("tbsla_hip_fill_unit_vector", C.c_int, [_vp, C.c_longlong, C.c_longlong, _vp]),
+    ("tbsla_hip_fill_uniform_vector", C.c_int, [_vp, C.c_longlong, C.c_ulonglong, C.c_double,
+                                                C.c_double, _vp]),
+    ("tbsla_hip_fill_poisson3d", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    ("tbsla_hip_fill_lognormal_field", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                 C.c_double, C.c_double, C.c_ulonglong, _vp]),
+    ("tbsla_hip_fill_sym_cqmat", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint,
+                                           C.c_longlong, _vp]),
+    ("tbsla_hip_transpose", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_block_diag", C.c_int, [_vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_export", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("tbsla_hip_import", C.c_int, [_vp, C.c_int, _GP, _vp, _vp, _vp, _vp]),
     ("tbsla_hip_convert", C.c_int, [_vp, C.c_int, _vp, _vp]),
@@ -80,6 +98,8 @@ SIGNATURES = [
     ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
                                       C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_cg_batched", C.c_int, [_vp, C.c_int, _ip, _ip, _vp, _vp, C.c_void_p, C.c_void_p,
+                                       _vp]),
     ("tbsla_hip_spmm", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_personalized_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _vp,
                                                    _ip, C.POINTER(C.c_double), _vp]),
@@ -92,7 +112,12 @@ _lib = None
 
 
 class TbslaHipError(RuntimeError):
-    pass
+    """A failed library call; .status is the TBSLA_HIP_ERR_* code (0 when the
+    library itself could not be loaded)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
 
 
 def lib() -> C.CDLL:
@@ -115,7 +140,7 @@ def check(status: int) -> None:
     if status != 0:
         L = lib()
         raise TbslaHipError(f"{L.tbsla_hip_status_string(status).decode()}: "
-                            f"{L.tbsla_hip_last_error().decode()}")
+                            f"{L.tbsla_hip_last_error().decode()}", status)
 
 
 def device_count() -> int:
@@ -164,6 +189,19 @@ class DeviceBuffer:
             check(lib().tbsla_hip_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, None))
 
 
+def lognormal_field(nx, ny, nz, mu=0.0, sigma=1.0, corr_len=4.0, seed=0) -> DeviceBuffer:
+    """Smoothed log-normal coefficient field on the device (nx*ny*nz doubles)."""
+    d = DeviceBuffer(nx * ny * nz * 8)
+    check(lib().tbsla_hip_fill_lognormal_field(d.ptr, nx, ny, nz, mu, sigma, corr_len, seed, None))
+    return d
+
+
+def uniform_vector(n, seed, lo=0.0, hi=1.0) -> DeviceBuffer:
+    d = DeviceBuffer(max(n, 1) * 8)
+    check(lib().tbsla_hip_fill_uniform_vector(d.ptr, n, seed, lo, hi, None))
+    return d
+
+
 def _dev(v, dtype=np.float64) -> DeviceBuffer:
     return v if isinstance(v, DeviceBuffer) else DeviceBuffer.from_numpy(np.asarray(v, dtype))
 
@@ -201,6 +239,28 @@ class Matrix:
     def fill_cdiag(self, n_row, n_col, cdiag, pr=0, pc=0, NR=1, NC=1, stream=None):
         check(lib().tbsla_hip_fill_cdiag(self.h, n_row, n_col, cdiag, pr, pc, NR, NC, stream))
         return self
+
+    def fill_poisson3d(self, nx, ny, nz, kappa=None, stream=None):
+        """7-point -div(k grad u) (CSR); kappa: None (k = 1), numpy or DeviceBuffer."""
+        dk = None if kappa is None else _dev(kappa)
+        check(lib().tbsla_hip_fill_poisson3d(self.h, nx, ny, nz, dk.ptr if dk else None, stream))
+        return self
+
+    def fill_sym_cqmat(self, n, c, q, seed_mult=1, value_seed=0, stream=None):
+        check(lib().tbsla_hip_fill_sym_cqmat(self.h, n, c, q, seed_mult, value_seed, stream))
+        return self
+
+    def transpose(self) -> "Matrix":
+        t = Matrix(CSR)
+        check(lib().tbsla_hip_transpose(t.h, self.h, None))
+        return t
+
+    @classmethod
+    def block_diag(cls, parts) -> "Matrix":
+        m = cls(CSR)
+        arr = (C.c_void_p * len(parts))(*[p.h for p in parts])
+        check(lib().tbsla_hip_block_diag(m.h, arr, len(parts), None))
+        return m
 
     # -- geometry -----------------------------------------------------------
     def geometry(self) -> Geometry:
@@ -360,19 +420,56 @@ class Matrix:
         X = None if out is not None else dX.to_numpy(np.float64, n * 64).reshape(n, 64)
         return X, it.value, err.value
 
+    def cg_options(self, max_iterations=10000, tolerance=1e-10, relative=False,
+                   precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0,
+                   use_initial_guess=False, near_zero=0.0, factor=None, factor_t=None,
+                   graph_batch=0, lanes_per_row=0, stall_window=0) -> CGOptions:
+        return CGOptions(max_iterations, tolerance, int(relative), precision, preconditioner,
+                         replace_every, int(use_initial_guess), near_zero,
+                         factor.h if factor is not None else None,
+                         factor_t.h if factor_t is not None else None, graph_batch,
+                         lanes_per_row, stall_window)
+
     def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False,
            precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None,
-           near_zero=0.0):
-        """Corrected tbsla::mpi::Matrix::CG; returns (x, iterations, |r|)."""
+           near_zero=0.0, factor=None, factor_t=None, graph_batch=0, lanes_per_row=0,
+           stall_window=0, stream=None):
+        """Corrected tbsla::mpi::Matrix::CG; returns (x, iterations, |r|).
+        rhs / x0: numpy or DeviceBuffer; M = G G^T with factor=G (a CSR Matrix)."""
         drhs = _dev(rhs)
         n = self.geometry().ln_col
         dx = _dev(np.zeros(n) if x0 is None else x0)
-        o = CGOptions(max_iterations, tolerance, int(relative), precision, preconditioner,
-                      replace_every, int(x0 is not None), near_zero)
+        o = self.cg_options(max_iterations, tolerance, relative, precision, preconditioner,
+                            replace_every, x0 is not None, near_zero, factor, factor_t,
+                            graph_batch, lanes_per_row, stall_window)
         it, rn = C.c_int(), C.c_double()
         check(lib().tbsla_hip_cg(self.h, drhs.ptr, dx.ptr, C.byref(o), C.byref(it),
-                                 C.byref(rn), None))
+                                 C.byref(rn), stream))
         return dx.to_numpy(np.float64, n), it.value, rn.value
+
+    def cg_device(self, drhs: "DeviceBuffer", dx: "DeviceBuffer", options: CGOptions,
+                  stream=None):
+        """CG on device vectors; returns (iterations, |r|)."""
+        it, rn = C.c_int(), C.c_double()
+        check(lib().tbsla_hip_cg(self.h, drhs.ptr, dx.ptr, C.byref(options), C.byref(it),
+                                 C.byref(rn), stream))
+        return it.value, rn.value
+
+    def cg_batched(self, row_offsets, rhs, options: CGOptions, active=None, x0=None,
+                   stream=None):
+        """Independent CG solves on the diagonal blocks [row_offsets[k], row_offsets[k+1]);
+        returns (x, list of CGResult)."""
+        offs = np.ascontiguousarray(row_offsets, np.int32)
+        k = len(offs) - 1
+        n = int(offs[-1])
+        drhs = _dev(rhs)
+        dx = _dev(np.zeros(n) if x0 is None else x0)
+        act = None if active is None else np.ascontiguousarray(active, np.int32)
+        res = (CGResult * k)()
+        check(lib().tbsla_hip_cg_batched(
+            self.h, k, offs.ctypes.data_as(_ip), None if act is None else act.ctypes.data_as(_ip),
+            drhs.ptr, dx.ptr, C.byref(options), res, stream))
+        return dx.to_numpy(np.float64, n), list(res)
 
     # -- conversions --------------------------------------------------------
     def to(self, fmt: int) -> "Matrix":

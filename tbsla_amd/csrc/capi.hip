@@ -48,6 +48,7 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
 }
 
 void release_arrays(tbsla_hip_matrix_s *m) {
+  cg_cache_release(m);  // holds graphs that name these arrays
   if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32) hipDeviceSynchronize();
   hipFree(m->rowptr);
   hipFree(m->colidx);
@@ -93,6 +94,7 @@ const char *tbsla_hip_status_string(int status) {
     case TBSLA_HIP_ERR_ALLOC: return "device allocation failed";
     case TBSLA_HIP_ERR_UNSUPPORTED: return "unsupported for this format";
     case TBSLA_HIP_ERR_RANGE: return "size beyond int32 indexing";
+    case TBSLA_HIP_ERR_BREAKDOWN: return "CG breakdown";
   }
   return "unknown status";
 }

@@ -206,10 +206,13 @@ struct tbsla_hip_matrix_s {
   // scratch owned by the record (reused across calls; grows on demand)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // CG work vectors and captured graphs kept between solves (cg.hip)
+  void *cg_cache = nullptr;
 };
 
 namespace tbsla_hip {
 int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes);
 void release_arrays(tbsla_hip_matrix_s *m);
+void cg_cache_release(tbsla_hip_matrix_s *m);
 int device_alloc(void **p, size_t bytes);
 }  // namespace tbsla_hip

@@ -30,4 +30,11 @@ int launch_axpy_inplace(double *y, const double *x, long long n, hipStream_t s);
 int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int key_bits,
                      hipStream_t s);
 
+// problems.hip: CSR transpose into new arrays (rows = src's local columns)
+int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **colidx_t,
+                         double **values_t, hipStream_t s);
+
+// cg.hip: FP32 copy of the values (m->values32), built once
+int ensure_values32(tbsla_hip_matrix_s *m, hipStream_t s);
+
 }  // namespace tbsla_hip

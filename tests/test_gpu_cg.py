@@ -1,0 +1,193 @@
+"""The fused CG engine on the GPU (tbsla_hip_cg / tbsla_hip_cg_batched).
+
+* Preconditioned iterates (Jacobi, M = G G^T) within a relative 1e-12 of the
+  oracle's PCG (orc_pcg) for the first iterations; the same iteration count
+  at convergence within two.
+* hipGraph batches and direct launches give identical bits; repeated solves
+  give identical bits.
+* A batch of systems gives, for each system, the bits of solving it alone.
+* Mixed precision: the true FP64 residual meets the tolerance.
+* Poisson 16^3 against a sparse direct solve.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as O
+import tbsla_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def to_gpu(o):
+    rp, ci, va = o.arrays()
+    return T.Matrix.from_arrays(T.CSR, o.geom, rp, ci, va)
+
+
+def scipy_of(m):
+    rp, ci, va = m.arrays()
+    n = len(rp) - 1
+    return sp.csr_matrix((va, ci, rp), shape=(n, n))
+
+
+def lower_factor(o):
+    """A lower-triangular G on tril(A)'s pattern with a positive diagonal:
+    G = tril(A) scaled by D^-1 (a crude SPD-preserving factor)."""
+    A = scipy_of(o)
+    d = A.diagonal()
+    L = sp.tril(A).tocsr()
+    G = sp.diags(1.0 / np.sqrt(d)) @ (sp.diags(1.0 / d) @ L) * 0.5 + sp.diags(0.5 / np.sqrt(d))
+    G = sp.csr_matrix(G)
+    G.sort_indices()
+    return O.csr_from_arrays(G.shape[0], G.shape[1], G.indptr, G.indices, G.data)
+
+
+@pytest.fixture(scope="module")
+def sym():
+    o = O.sym_cqmat(6000, 12, 0.1, 0, value_seed=3)
+    return o, to_gpu(o)
+
+
+def test_jacobi_early_iterates(sym):
+    o, g = sym
+    n = o.geom["ln_row"]
+    rhs = 2 * O.unit_vector(21, n) - 1
+    for k in (1, 2, 5, 12):
+        xo, ito, ro = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30, kind=1)
+        xg, itg, rg = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
+                           preconditioner=T.PRECOND_JACOBI)
+        assert itg == ito == k
+        assert rel_max(xg, xo) < 1e-12, (k, rel_max(xg, xo))
+        assert abs(rg - ro) <= 1e-10 * ro
+
+
+def test_factored_early_iterates_and_given_transpose(sym):
+    o, g = sym
+    n = o.geom["ln_row"]
+    Go = lower_factor(o)
+    G = to_gpu(Go)
+    Gt = G.transpose()
+    rhs = 2 * O.unit_vector(5, n) - 1
+    for k in (1, 3, 8):
+        xo, ito, _ = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30, kind=2, G=Go)
+        xg, itg, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
+                          preconditioner=T.PRECOND_FACTORED, factor=G)
+        xt, itt, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
+                          preconditioner=T.PRECOND_FACTORED, factor=G, factor_t=Gt)
+        assert itg == ito == itt == k
+        assert rel_max(xg, xo) < 1e-12, (k, rel_max(xg, xo))
+        assert np.array_equal(xg, xt)
+
+
+@pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI, T.PRECOND_FACTORED])
+def test_converged_iterations_like_oracle(sym, pre):
+    o, g = sym
+    n = o.geom["ln_row"]
+    rhs = 2 * O.unit_vector(8, n) - 1
+    Go = lower_factor(o) if pre == T.PRECOND_FACTORED else None
+    G = to_gpu(Go) if Go is not None else None
+    xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=pre, G=Go)
+    xg, itg, rg = g.cg(rhs, tolerance=1e-10, relative=True, preconditioner=pre, factor=G)
+    assert abs(itg - ito) <= 2, (itg, ito)
+    assert rel_max(xg, xo) < 1e-8
+    A = scipy_of(o)
+    assert np.linalg.norm(rhs - A @ xg) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+
+
+@pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI])
+def test_graph_and_direct_launches_identical(sym, pre):
+    o, g = sym
+    n = o.geom["ln_row"]
+    rhs = 2 * O.unit_vector(9, n) - 1
+    a = g.cg(rhs, tolerance=1e-11, relative=True, preconditioner=pre, graph_batch=0)
+    b = g.cg(rhs, tolerance=1e-11, relative=True, preconditioner=pre, graph_batch=-1)
+    c = g.cg(rhs, tolerance=1e-11, relative=True, preconditioner=pre, graph_batch=6)
+    d = g.cg(rhs, tolerance=1e-11, relative=True, preconditioner=pre, graph_batch=0)
+    assert a[1] == b[1] == c[1] == d[1]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[0], c[0]) and np.array_equal(a[0], d[0])
+
+
+def test_max_iterations_count(sym):
+    o, g = sym
+    n = o.geom["ln_row"]
+    rhs = 2 * O.unit_vector(2, n) - 1
+    for k in (0, 1, 17, 33):
+        _, it, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True)
+        _, ito, _ = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30)
+        assert it == ito == k
+
+
+def make_batch(sizes, seed=0):
+    parts = [T.Matrix(T.CSR).fill_sym_cqmat(n, c, 0.1, 1, value_seed=seed + i)
+             for i, (n, c) in enumerate(sizes)]
+    offs = np.concatenate([[0], np.cumsum([n for n, _ in sizes])]).astype(np.int32)
+    return parts, T.Matrix.block_diag(parts), offs
+
+
+@pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI])
+def test_batched_equals_single(pre):
+    sizes = [(4096, 8), (777, 31), (20000, 16), (1, 3), (9000, 20)]
+    parts, bd, offs = make_batch(sizes)
+    rhs = 2 * O.unit_vector(4, int(offs[-1])) - 1
+    # same lanes per row on both sides (the default picks from the mean row length)
+    opt = bd.cg_options(tolerance=1e-10, relative=True, preconditioner=pre, lanes_per_row=8)
+    x, res = bd.cg_batched(offs, rhs, opt)
+    for k, p in enumerate(parts):
+        xs, its, rs = p.cg(rhs[offs[k]:offs[k + 1]], tolerance=1e-10, relative=True,
+                           preconditioner=pre, lanes_per_row=8)
+        assert res[k].status == T.CG_CONVERGED
+        assert res[k].iterations == its
+        assert np.array_equal(x[offs[k]:offs[k + 1]], xs)
+        assert res[k].res_norm == rs
+        assert res[k].time_us > 0
+    # systems that need more iterations finish later
+    its = np.array([r.iterations for r in res])
+    t = np.array([r.time_us for r in res])
+    assert t[np.argmax(its)] >= t[np.argmin(its)]
+
+
+def test_batched_inactive_and_mixed():
+    sizes = [(3000, 10), (5000, 24), (2500, 8)]
+    parts, bd, offs = make_batch(sizes, seed=7)
+    rhs = 2 * O.unit_vector(6, int(offs[-1])) - 1
+    opt = bd.cg_options(tolerance=1e-8, relative=True, precision=T.CG_MIXED, replace_every=50)
+    x, res = bd.cg_batched(offs, rhs, opt, active=[1, 0, 1])
+    assert res[1].status == T.CG_INACTIVE
+    assert np.all(x[offs[1]:offs[2]] == 0.0)
+    for k in (0, 2):
+        assert res[k].status == T.CG_CONVERGED
+        A = scipy_of(parts[k])
+        b = rhs[offs[k]:offs[k + 1]]
+        assert np.linalg.norm(b - A @ x[offs[k]:offs[k + 1]]) < 1e-8 * np.linalg.norm(b)
+
+
+def test_breakdown_is_reported(sym):
+    # a non-finite right-hand side makes r.r non-finite at the first test
+    o, g = sym
+    rhs = np.ones(o.geom["ln_row"])
+    rhs[17] = np.nan
+    with pytest.raises(T.TbslaHipError) as e:
+        g.cg(rhs, tolerance=1e-12, relative=True, preconditioner=T.PRECOND_JACOBI,
+             max_iterations=50)
+    assert e.value.status == T.ERR_BREAKDOWN
+
+
+def test_poisson_direct_solve():
+    dims = (16, 16, 16)
+    dk = T.lognormal_field(*dims, seed=2)
+    g = T.Matrix(T.CSR).fill_poisson3d(*dims, kappa=dk)
+    A = scipy_of(g).tocsc()
+    n = A.shape[0]
+    rhs = T.uniform_vector(n, 3, -1.0, 1.0).to_numpy(count=n)
+    exact = spla.spsolve(A, rhs)
+    for pre, prec in [(T.PRECOND_NONE, T.CG_FP64), (T.PRECOND_JACOBI, T.CG_FP64),
+                      (T.PRECOND_JACOBI, T.CG_MIXED)]:
+        x, it, rn = g.cg(rhs, tolerance=1e-10, relative=True, preconditioner=pre,
+                         precision=prec, replace_every=50 if prec == T.CG_MIXED else 0)
+        assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+        assert rel_max(x, exact) < 1e-6, (pre, prec, rel_max(x, exact))
