@@ -11,6 +11,16 @@ column), 4 (n+1) for the row pointers, 8 n each for x and y.
 Printed: one JSON line with the roofline of the CSR kernel (HBM-bound,
 against 8 TB/s) and the reference's own OpenMP CPU SpMV (oracle/_ref, the
 reference library compiled from its sources) timed on this host.
+
+The same line carries "cg": time to relative residual 1e-8 (vectors in HBM,
+host clock around each solve, second solve of each kind):
+  * 7-point Poisson 128^3 (k = 1) and the symmetrised cqmat (n = 2^21, c = 16,
+    q = 0.05): plain FP64 CG against mixed precision (FP32 matrix, FP64 dots,
+    true residual every 50), with the error of both against a sparse direct
+    solve of the 32^3 Poisson;
+  * held-out variable-coefficient Poisson (64^3 and 128^3, k log-normal,
+    sigma 1, correlation 4 cells): plain FP64, Jacobi FP64 / mixed, and the
+    learned preconditioner (committed weights) mixed / FP64.
 """
 import argparse
 import json
@@ -73,6 +83,68 @@ def cpu_baseline(n, c, q, stored, iters, reps, threads):
                       f"({t * 1e3:.2f} ms per SpMV)"}
 
 
+def cg_section(args):
+    """CG time-to-1e-8 figures (see the module docstring)."""
+    import numpy as np
+    import tbsla_amd as T
+    from tbsla_amd import learned as LP
+    out = {}
+
+    def timed(A, rhs, **kw):
+        n = A.geometry().ln_row
+        x = T.DeviceBuffer(n * 8)
+        o = A.cg_options(20000, 1e-8, True, **kw)
+        A.cg_device(rhs, x, o)  # work vectors, graphs
+        t0 = time.perf_counter()
+        it, rn = A.cg_device(rhs, x, o)
+        ms = (time.perf_counter() - t0) * 1e3
+        y = T.DeviceBuffer(n * 8)
+        T.check(T.lib().tbsla_hip_spmv(A.h, y.ptr, x.ptr, 0, None))
+        b = rhs.to_numpy(count=n)
+        rel = float(np.linalg.norm(b - y.to_numpy(count=n)) / np.linalg.norm(b))
+        return {"ms": round(ms, 3), "iterations": it, "true_rel_res": rel}, x
+
+    plain = dict(precision=T.CG_FP64)
+    mixed = dict(precision=T.CG_MIXED, replace_every=50)
+    side = 128
+    A = T.Matrix(T.CSR).fill_poisson3d(side, side, side)
+    rhs = T.uniform_vector(side ** 3, 3, -1.0, 1.0)
+    out["poisson128"] = {"n": side ** 3, "nnz": A.geometry().stored,
+                         "plain_fp64": timed(A, rhs, **plain)[0],
+                         "mixed": timed(A, rhs, **mixed)[0]}
+    del A
+    S = T.Matrix(T.CSR).fill_sym_cqmat(1 << 21, 16, 0.05, 0, value_seed=0)
+    rhs = T.uniform_vector(1 << 21, 4, -1.0, 1.0)
+    out["sym_cqmat_2p21"] = {"n": 1 << 21, "nnz": S.geometry().stored,
+                             "plain_fp64": timed(S, rhs, **plain)[0],
+                             "mixed": timed(S, rhs, **mixed)[0]}
+    del S
+    # error against a sparse direct solve (32^3)
+    try:
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        A = T.Matrix(T.CSR).fill_poisson3d(32, 32, 32)
+        rhs = T.uniform_vector(32 ** 3, 5, -1.0, 1.0)
+        rp, ci, va = A.arrays()
+        exact = spla.spsolve(sp.csr_matrix((va, ci, rp)).tocsc(), rhs.to_numpy(count=32 ** 3))
+        errs = {}
+        for name, kw in (("plain_fp64", plain), ("mixed", mixed)):
+            _, x = timed(A, rhs, **kw)
+            xv = x.to_numpy(count=32 ** 3)
+            errs[name] = float(np.max(np.abs(xv - exact)) / np.max(np.abs(exact)))
+        out["poisson32_error_vs_direct"] = errs
+    except ImportError:
+        pass
+    # learned preconditioner on held-out variable-coefficient problems
+    net = T.PrecondNet()
+    have = LP.load_weights(net)
+    for side_v, count in ((64, args.cg_problems), (128, max(1, args.cg_problems // 4))):
+        rows = LP.evaluate(net if have else None, side_v, range(100_000, 100_000 + count))
+        out[f"varcoef{side_v}"] = LP.summarise(rows)
+    out["learned_weights"] = os.path.relpath(LP.WEIGHTS, ROOT) if have else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +152,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 16)))
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG time-to-solution section")
+    ap.add_argument("--cg-problems", type=int, default=8,
+                    help="held-out 64^3 problems in the CG section (128^3: a quarter)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (one block of the 2-D grid per GPU); "
@@ -115,15 +190,18 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
+    sched = list(m.get_csr_schedule())
+    del m, dx, dy
+    cg = None if args.no_cg else cg_section(args)
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
             "config": {"workload": "csr_spmv_cqmat", "format": "CSR", "n": N, "c": C, "q": Q,
                        "seed_mult": 0, "values": "U[0,1) seed 0", "x": "U[0,1) seed 1",
-                       "stored_nnz": g.stored, "nnz_field": g.nnz, "csr_schedule":
-                       list(m.get_csr_schedule()), "fill_s": round(t_fill, 3)},
-            "roofline": roof, "cpu_baseline": cpu}
+                       "stored_nnz": g.stored, "nnz_field": g.nnz, "csr_schedule": sched,
+                       "fill_s": round(t_fill, 3)},
+            "roofline": roof, "cpu_baseline": cpu, "cg": cg}
     print(json.dumps(line))
 
 

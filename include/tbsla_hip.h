@@ -274,6 +274,37 @@ int tbsla_hip_cg_batched(tbsla_hip_matrix_t m, int nsys, const int *h_row_offset
                          const tbsla_hip_cg_options *opt, tbsla_hip_cg_result *h_results,
                          void *stream);
 
+/* ---- learned factored preconditioner (no counterpart in the reference) ---
+ * A 3-layer message-passing network (width 64; BF16 MFMA products with FP32
+ * accumulation) maps an SPD CSR block A (at (0,0), symmetric pattern) to a
+ * lower-triangular G on tril(A)'s pattern with a positive diagonal, for CG
+ * with M = G G^T (TBSLA_HIP_PRECOND_FACTORED).  With s_i = A_ii^-1/2 and
+ * A^ = S A S the network predicts G^ and returns G = S G^; its output heads
+ * start at zero, which is G^ = I (Jacobi).  Training minimises
+ * |Z - G^^T A^ G^ Z|_F^2 / (n k) over k Rademacher probes. */
+typedef struct tbsla_hip_precond_net_s *tbsla_hip_precond_net_t;
+/* number of FP32 parameters */
+int tbsla_hip_precond_net_param_count(void);
+/* parameters drawn from seed; Adam state zero */
+int tbsla_hip_precond_net_create(tbsla_hip_precond_net_t *out, unsigned long long seed);
+int tbsla_hip_precond_net_destroy(tbsla_hip_precond_net_t net);
+int tbsla_hip_precond_net_get_params(tbsla_hip_precond_net_t net, float *h_theta);
+/* also resets the Adam state */
+int tbsla_hip_precond_net_set_params(tbsla_hip_precond_net_t net, const float *h_theta);
+/* G (and G^T if GT is not NULL) for A: CSR, FP64 values, columns sorted */
+int tbsla_hip_precond_net_apply(tbsla_hip_precond_net_t net, tbsla_hip_matrix_t A,
+                                tbsla_hip_matrix_t G, tbsla_hip_matrix_t GT, void *stream);
+/* loss and its gradient (h_grad: param_count floats, nullable) for probes
+ * 1, 2, 4, 8 or 16 drawn from probe_seed */
+int tbsla_hip_precond_net_loss_grad(tbsla_hip_precond_net_t net, tbsla_hip_matrix_t A,
+                                    int probes, unsigned long long probe_seed, double *loss,
+                                    float *h_grad, void *stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the gradient for A, after
+ * clipping its global norm to clip (0: none); *loss before the step */
+int tbsla_hip_precond_net_train_step(tbsla_hip_precond_net_t net, tbsla_hip_matrix_t A,
+                                     int probes, unsigned long long probe_seed, float lr,
+                                     float clip, double *loss, void *stream);
+
 /* ---- SpMM and personalised PageRank ------------------------------------- */
 /* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283; block form
  * src/mpi/matrix/MatrixCSR.cpp:11-30): C = A B, B ln_col x k and C

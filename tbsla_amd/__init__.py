@@ -98,6 +98,17 @@ SIGNATURES = [
     ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
                                       C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_precond_net_param_count", C.c_int, []),
+    ("tbsla_hip_precond_net_create", C.c_int, [C.POINTER(C.c_void_p), C.c_ulonglong]),
+    ("tbsla_hip_precond_net_destroy", C.c_int, [_vp]),
+    ("tbsla_hip_precond_net_get_params", C.c_int, [_vp, C.POINTER(C.c_float)]),
+    ("tbsla_hip_precond_net_set_params", C.c_int, [_vp, C.POINTER(C.c_float)]),
+    ("tbsla_hip_precond_net_apply", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("tbsla_hip_precond_net_loss_grad", C.c_int, [_vp, _vp, C.c_int, C.c_ulonglong,
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_float),
+                                                  _vp]),
+    ("tbsla_hip_precond_net_train_step", C.c_int, [_vp, _vp, C.c_int, C.c_ulonglong, C.c_float,
+                                                   C.c_float, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg_batched", C.c_int, [_vp, C.c_int, _ip, _ip, _vp, _vp, C.c_void_p, C.c_void_p,
                                        _vp]),
     ("tbsla_hip_spmm", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
@@ -502,3 +513,57 @@ class Matrix:
         check(lib().tbsla_hip_import(m.h, fmt, C.byref(g), ptr(a, np.int32), ptr(b, np.int32),
                                      ptr(v, np.float64), None))
         return m
+
+
+class PrecondNet:
+    """The learned factored preconditioner (tbsla_hip_precond_net_*): maps an
+    SPD CSR Matrix A to G (lower, tril(A)'s pattern) for CG with M = G G^T."""
+
+    def __init__(self, seed: int = 0):
+        h = C.c_void_p()
+        check(lib().tbsla_hip_precond_net_create(C.byref(h), seed))
+        self.h = h.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tbsla_hip_precond_net_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+    @staticmethod
+    def param_count() -> int:
+        return lib().tbsla_hip_precond_net_param_count()
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), np.float32)
+        check(lib().tbsla_hip_precond_net_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, theta) -> None:
+        t = np.ascontiguousarray(theta, np.float32)
+        assert t.size == self.param_count()
+        check(lib().tbsla_hip_precond_net_set_params(self.h, t.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def apply(self, A: Matrix, with_transpose: bool = True, stream=None):
+        """(G, G^T or None) as CSR Matrices."""
+        G = Matrix(CSR)
+        GT = Matrix(CSR) if with_transpose else None
+        check(lib().tbsla_hip_precond_net_apply(self.h, A.h, G.h, GT.h if GT else None, stream))
+        return G, GT
+
+    def loss_grad(self, A: Matrix, probes: int, seed: int, stream=None):
+        loss = C.c_double()
+        g = np.zeros(self.param_count(), np.float32)
+        check(lib().tbsla_hip_precond_net_loss_grad(self.h, A.h, probes, seed, C.byref(loss),
+                                                    g.ctypes.data_as(C.POINTER(C.c_float)), stream))
+        return loss.value, g
+
+    def train_step(self, A: Matrix, probes: int, seed: int, lr: float, clip: float = 0.0,
+                   stream=None) -> float:
+        loss = C.c_double()
+        check(lib().tbsla_hip_precond_net_train_step(self.h, A.h, probes, seed, lr, clip,
+                                                     C.byref(loss), stream))
+        return loss.value
+

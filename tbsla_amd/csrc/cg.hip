@@ -557,6 +557,7 @@ struct CgCache {
   int *t_rp = nullptr, *t_ci = nullptr;
   double *t_v64 = nullptr;
   float *g_v32 = nullptr, *t_v32 = nullptr;
+  size_t f32_len = 0;
   // graph
   hipGraphExec_t exec = nullptr;
   GraphKey key{};
@@ -572,6 +573,7 @@ struct CgCache {
     t_rp = t_ci = nullptr;
     t_v64 = nullptr;
     g_v32 = t_v32 = nullptr;
+    f32_len = 0;
     factor = nullptr;
   }
   ~CgCache() {
@@ -777,15 +779,27 @@ int cg_solve(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_ac
     tbsla_hip_matrix_s *GT = o.factor_t;
     if (GT && (GT->format != TBSLA_HIP_CSR || GT->ln_row != n || GT->stored != G->stored))
       return fail(TBSLA_HIP_ERR_INVALID, "factor_t must be the CSR transpose of the factor");
-    // the transpose and the FP32 copies are rebuilt on every call: G may
-    // have been rewritten in place since the last solve
-    c.drop_factor();
+    // the transpose and the FP32 copies are rebuilt on every call (G may
+    // have been rewritten in place since the last solve); their buffers are
+    // kept while large enough
     if (!GT) {
+      hipFree(c.t_rp);
+      hipFree(c.t_ci);
+      hipFree(c.t_v64);
+      c.t_rp = c.t_ci = nullptr;
+      c.t_v64 = nullptr;
       TBSLA_TRY(csr_transpose_arrays(G, &c.t_rp, &c.t_ci, &c.t_v64, s));
     }
     if (mixed) {
-      TBSLA_TRY(device_alloc((void **)&c.g_v32, (size_t)std::max(G->stored, 1LL) * 4));
-      TBSLA_TRY(device_alloc((void **)&c.t_v32, (size_t)std::max(G->stored, 1LL) * 4));
+      if (c.f32_len < (size_t)std::max(G->stored, 1LL)) {
+        hipFree(c.g_v32);
+        hipFree(c.t_v32);
+        c.g_v32 = c.t_v32 = nullptr;
+        c.f32_len = 0;
+        TBSLA_TRY(device_alloc((void **)&c.g_v32, (size_t)std::max(G->stored, 1LL) * 4));
+        TBSLA_TRY(device_alloc((void **)&c.t_v32, (size_t)std::max(G->stored, 1LL) * 4));
+        c.f32_len = (size_t)std::max(G->stored, 1LL);
+      }
       if (G->stored > 0) {
         k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(G->values, G->stored, c.g_v32);
         k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(GT ? GT->values : c.t_v64, G->stored,
