@@ -21,6 +21,10 @@ host clock around each solve, second solve of each kind):
   * held-out variable-coefficient Poisson (64^3 and 128^3, k log-normal,
     sigma 1, correlation 4 cells): plain FP64, Jacobi FP64 / mixed, and the
     learned preconditioner (committed weights) mixed / FP64.
+"batched_cg": 512 symmetrised cqmat systems of uneven size in one batched,
+graph-captured solve (tbsla_amd/batched.py), p50/p99 per-system time and the
+autotuned schedule; "ppr_spmm": personalised PageRank with 64 teleports on
+the n = 2^22 stochastic cqmat, GB/s per fused SpMM iteration.
 """
 import argparse
 import json
@@ -145,6 +149,32 @@ def cg_section(args):
     return out
 
 
+def ppr_section():
+    """Personalised PageRank on the n = 2^22, c = 32 stochastic cqmat with 64
+    one-hot teleports: one fused SpMM + update per iteration.  Algorithmic
+    bytes per iteration: A (12 stored + 4 (n+1)) + X read + X' written
+    (2 * 8 * n * 64)."""
+    import numpy as np
+    import tbsla_amd as T
+    n = 1 << 22
+    m = T.Matrix(T.CSR).fill_cqmat(n, n, 32, 0.1, 0, value_seed=0)
+    s = T.DeviceBuffer(n * 8)
+    T.check(T.lib().tbsla_hip_make_stochastic(m.h, s.ptr, None))
+    g = m.geometry()
+    src = np.random.default_rng(0).choice(n, 64, replace=False).astype(np.int32)
+    X = T.DeviceBuffer(n * 64 * 8)
+    m.personalized_page_rank(src, 0.85, 1e-30, 2, out=X)
+    iters = 20
+    t0 = time.perf_counter()
+    _, it, _ = m.personalized_page_rank(src, 0.85, 1e-30, iters, out=X)
+    ms = (time.perf_counter() - t0) * 1e3
+    nbytes = 12 * g.stored + 4 * (n + 1) + 2 * 8 * n * 64
+    per = ms / it
+    return {"n": n, "k": 64, "iterations": it, "ms_per_iteration": round(per, 4),
+            "bytes_per_iteration": nbytes, "GBps": round(nbytes / (per * 1e-3) / 1e9, 1),
+            "note": "host clock around 20 iterations incl. per-iteration column sums"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +183,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 16)))
     ap.add_argument("--no-cg", action="store_true", help="skip the CG time-to-solution section")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-CG and personalised-PageRank sections")
+    ap.add_argument("--batched-systems", type=int, default=512)
     ap.add_argument("--cg-problems", type=int, default=8,
                     help="held-out 64^3 problems in the CG section (128^3: a quarter)")
     args = ap.parse_args()
@@ -193,6 +226,11 @@ def main():
     sched = list(m.get_csr_schedule())
     del m, dx, dy
     cg = None if args.no_cg else cg_section(args)
+    extra = {}
+    if not args.no_batched:
+        from tbsla_amd import batched
+        extra["batched_cg"] = batched.bench(args.batched_systems)
+        extra["ppr_spmm"] = ppr_section()
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -201,7 +239,7 @@ def main():
                        "seed_mult": 0, "values": "U[0,1) seed 0", "x": "U[0,1) seed 1",
                        "stored_nnz": g.stored, "nnz_field": g.nnz, "csr_schedule": sched,
                        "fill_s": round(t_fill, 3)},
-            "roofline": roof, "cpu_baseline": cpu, "cg": cg}
+            "roofline": roof, "cpu_baseline": cpu, "cg": cg, **extra}
     print(json.dumps(line))
 
 

@@ -327,6 +327,19 @@ int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double beta, double e
  * (1, 2 or 4); 0 restores the choice from the mean row length. */
 int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows);
 int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows);
+/* CSR execution layout (no counterpart in the reference; results within
+ * rounding of the row-major sum).  TBSLA_HIP_CSR_ROWS: the CSR arrays as
+ * they are.  TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in
+ * row order; the others are grouped by column slice (slices a multiple of 8,
+ * at most 64) and multiplied by one XCD per slice, whose L2 then holds that
+ * slice of x; the row kernel adds both parts.  The split copies the entries
+ * once (about 24 bytes per remote entry), is rebuilt by normalize_cols and
+ * dropped when the arrays change. */
+enum { TBSLA_HIP_CSR_ROWS = 0, TBSLA_HIP_CSR_SPLIT = 1 };
+int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int slices,
+                             void *stream);
+int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int *slices,
+                             long long *remote_entries);
 /* Autotuner (no reference counterpart): times every CSR schedule on this
  * block with d_v / d_r (`iters` launches each) and keeps the fastest; other
  * formats have a single schedule.  *best_us receives its time per SpMV. */

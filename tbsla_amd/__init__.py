@@ -42,6 +42,7 @@ class CGResult(C.Structure):
 
 CG_FP64, CG_MIXED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_FACTORED = 0, 1, 2
+CSR_ROWS, CSR_SPLIT = 0, 1
 CG_CONVERGED, CG_MAX_ITERATIONS, CG_BREAKDOWN, CG_INACTIVE = 0, 1, 2, 3
 ERR_BREAKDOWN = 6
 
@@ -109,6 +110,8 @@ SIGNATURES = [
                                                   _vp]),
     ("tbsla_hip_precond_net_train_step", C.c_int, [_vp, _vp, C.c_int, C.c_ulonglong, C.c_float,
                                                    C.c_float, C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_set_csr_layout", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
+    ("tbsla_hip_get_csr_layout", C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_longlong)]),
     ("tbsla_hip_cg_batched", C.c_int, [_vp, C.c_int, _ip, _ip, _vp, _vp, C.c_void_p, C.c_void_p,
                                        _vp]),
     ("tbsla_hip_spmm", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
@@ -430,6 +433,16 @@ class Matrix:
             C.byref(err), None))
         X = None if out is not None else dX.to_numpy(np.float64, n * 64).reshape(n, 64)
         return X, it.value, err.value
+
+    def set_csr_layout(self, layout=CSR_SPLIT, window=64, slices=8, stream=None):
+        check(lib().tbsla_hip_set_csr_layout(self.h, layout, window, slices, stream))
+        return self
+
+    def get_csr_layout(self):
+        lay, w, s, r = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
+        check(lib().tbsla_hip_get_csr_layout(self.h, C.byref(lay), C.byref(w), C.byref(s),
+                                             C.byref(r)))
+        return {"layout": lay.value, "window": w.value, "slices": s.value, "remote": r.value}
 
     def cg_options(self, max_iterations=10000, tolerance=1e-10, relative=False,
                    precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0,

@@ -49,6 +49,7 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
 
 void release_arrays(tbsla_hip_matrix_s *m) {
   cg_cache_release(m);  // holds graphs that name these arrays
+  split_release(m);     // a copy of the entries in another order
   if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32) hipDeviceSynchronize();
   hipFree(m->rowptr);
   hipFree(m->colidx);

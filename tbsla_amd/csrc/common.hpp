@@ -208,11 +208,14 @@ struct tbsla_hip_matrix_s {
   size_t scratch_bytes = 0;
   // CG work vectors and captured graphs kept between solves (cg.hip)
   void *cg_cache = nullptr;
+  // CSR split execution layout (split.hip), when selected
+  void *split = nullptr;
 };
 
 namespace tbsla_hip {
 int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes);
 void release_arrays(tbsla_hip_matrix_s *m);
 void cg_cache_release(tbsla_hip_matrix_s *m);
+void split_release(tbsla_hip_matrix_s *m);
 int device_alloc(void **p, size_t bytes);
 }  // namespace tbsla_hip

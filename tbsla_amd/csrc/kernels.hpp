@@ -34,6 +34,12 @@ int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int ke
 int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **colidx_t,
                          double **values_t, hipStream_t s);
 
+// split.hip: the split CSR layout (near-diagonal + column-sliced remote)
+int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
+                 hipStream_t s);
+int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s);
+int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices);
+
 // cg.hip: FP32 copy of the values (m->values32), built once
 int ensure_values32(tbsla_hip_matrix_s *m, hipStream_t s);
 

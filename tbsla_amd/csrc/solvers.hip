@@ -275,6 +275,8 @@ extern "C" int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s,
                                                        m->colidx, m->f_col, m->values, d_s);
     TBSLA_HIP_LAUNCH_CHECK();
   }
+  int win = 0, sl = 0;  // the split layout copies values: rebuild it
+  if (split_params(m, &win, &sl)) TBSLA_TRY(split_build(m, win, sl, s));
   if (m->values32) {  // keep the FP32 copy in step
     hipFree(m->values32);
     m->values32 = nullptr;

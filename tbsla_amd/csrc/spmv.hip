@@ -325,6 +325,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
         return TBSLA_HIP_SUCCESS;
       }
+      if (m->split) return split_launch(m, d_r, d_v, d_done, s);
       const int L = pick_csr_lanes(m), R = pick_csr_rows(m, L);
       switch (L) {
         case 1: launch_csr<1>(m, R, d_r, d_v, d_done, s); break;

@@ -1,0 +1,79 @@
+"""The split CSR execution layout (tbsla_hip_set_csr_layout) on the GPU.
+
+SpMV through the split layout stays within a relative 1e-12 (max-norm) of
+the oracle for every window / slice count / block position, is bitwise
+repeatable, is rebuilt by normalize_cols, and is dropped when the arrays
+change.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tbsla_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    d = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (d if d > 0 else 1.0))
+
+
+CASES = [
+    (20000, 16, 0.1, (0, 0, 1, 1), 64, 8),
+    (20000, 16, 0.1, (0, 0, 1, 1), 0, 16),
+    (20000, 16, 1.0, (0, 0, 1, 1), 64, 64),
+    (30000, 32, 0.1, (0, 0, 4, 4), 64, 8),
+    (30000, 32, 0.1, (0, 1, 4, 4), 64, 8),
+    (30000, 9, 0.3, (3, 2, 4, 4), 5, 8),
+    (5000, 8, 0.0, (0, 0, 1, 1), 1 << 30, 8),
+    (1, 1, 0.5, (0, 0, 1, 1), 64, 8),
+]
+
+
+@pytest.mark.parametrize("n,c,q,grid,window,slices", CASES)
+def test_split_spmv_matches_oracle(n, c, q, grid, window, slices):
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, *grid, value_seed=3)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, c, q, 0, *grid, value_seed=3)
+    x = O.unit_vector(7, o.geom["ln_col"])
+    rows = g.spmv(x)
+    g.set_csr_layout(T.CSR_SPLIT, window, slices)
+    lay = g.get_csr_layout()
+    assert lay["layout"] == T.CSR_SPLIT and lay["slices"] == slices
+    y1, y2 = g.spmv(x), g.spmv(x)
+    ref = o.spmv(x)
+    assert rel_max(y1, ref) < 1e-12
+    assert np.array_equal(y1, y2)
+    g.set_csr_layout(T.CSR_ROWS)
+    assert np.array_equal(g.spmv(x), rows)
+
+
+def test_split_remote_count_and_normalize():
+    n = 40000
+    o = O.fill_cqmat(O.CSR, n, n, 16, 0.1, 0, value_seed=1)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, 16, 0.1, 0, value_seed=1)
+    g.set_csr_layout(T.CSR_SPLIT, 64, 8)
+    rp, ci, _ = o.arrays()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert g.get_csr_layout()["remote"] == int(np.sum(np.abs(ci - rows) > 64))
+    g.make_stochastic()
+    o.make_stochastic()
+    assert g.get_csr_layout()["layout"] == T.CSR_SPLIT
+    x = O.unit_vector(2, n)
+    assert rel_max(g.spmv(x), o.spmv(x)) < 1e-12
+    b, it, _ = g.page_rank(0.85, 1e-10, 200)
+    bo, ito, _ = o.page_rank(0.85, 1e-10, 200)
+    assert it == ito and rel_max(b, bo) < 1e-12
+    # new arrays drop the layout
+    g.fill_cqmat(n, n, 16, 0.1, 0, value_seed=1)
+    assert g.get_csr_layout()["layout"] == T.CSR_ROWS
+
+
+def test_split_bad_arguments():
+    g = T.Matrix(T.CSR).fill_cqmat(100, 100, 4, 0.1, 0)
+    for w, s in [(-1, 8), (64, 0), (64, 12), (64, 128)]:
+        with pytest.raises(T.TbslaHipError):
+            g.set_csr_layout(T.CSR_SPLIT, w, s)
+    e = T.Matrix(T.ELL).fill_cqmat(100, 100, 4, 0.1, 0)
+    with pytest.raises(T.TbslaHipError):
+        e.set_csr_layout(T.CSR_SPLIT, 64, 8)
