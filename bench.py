@@ -45,18 +45,22 @@ CAL_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "csr_spmv_cqmat_n4194304_c32_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the CSR kernel from the committed rocprofv3
-    --pmc summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
-    calibration copy): its `traffic_bytes_per_launch:` line, or None when it
-    has not been measured."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy): its
+    `traffic_bytes_per_launch:` line, when its `kernel:` line names the
+    kernel measured now; else None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
+    fields = {}
     with open(TRAFFIC_FILE) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
-    return None
+            if ":" in line and not line.startswith(" "):
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
+    if fields.get("kernel") != kernel or "traffic_bytes_per_launch" not in fields:
+        return None
+    return int(float(fields["traffic_bytes_per_launch"]))
 
 
 def algorithmic_bytes_csr(stored, n_rows, n_cols):
@@ -183,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 16)))
     ap.add_argument("--no-cg", action="store_true", help="skip the CG time-to-solution section")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default CSR schedule (row layout) instead of the autotuned one")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-CG and personalised-PageRank sections")
     ap.add_argument("--batched-systems", type=int, default=512)
@@ -203,6 +209,12 @@ def main():
     dx = T.DeviceBuffer(N * 8)
     T.check(T.lib().tbsla_hip_fill_unit_vector(dx.ptr, N, 1, None))
     dy = T.DeviceBuffer(N * 8)
+    # schedule and layout from the autotuner (untimed)
+    t0 = time.time()
+    if not args.no_autotune:
+        m.autotune(dy, dx, 5)
+    t_tune = time.time() - t0
+    layout = m.get_csr_layout()
     if args.warmup:
         m.time_spmv(dy, dx, args.warmup)
     ms = m.time_spmv(dy, dx, args.steps)
@@ -211,7 +223,9 @@ def main():
     gbs = nbytes / (per * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_csr_spmv<%d,%d>" % m.get_csr_schedule(), "bytes_per_launch": nbytes,
+            "kernel": ("k_csr_split<%d,%d> + k_remote_products" if layout["layout"] == T.CSR_SPLIT
+                       else "k_csr_spmv<%d,%d>") % m.get_csr_schedule(),
+            "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
     # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
     # give the scale factor applied to the SpMV kernel's counters
@@ -219,7 +233,7 @@ def main():
     T.check(T.lib().tbsla_hip_copy_kernel(cal[1].ptr, cal[0].ptr, CAL_BYTES, None))
     T.check(T.lib().tbsla_hip_stream_synchronize(None))
     del cal
-    traffic = measured_traffic()
+    traffic = measured_traffic(roof["kernel"])
     if traffic is not None:
         roof["traffic"] = traffic
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
@@ -238,7 +252,8 @@ def main():
             "config": {"workload": "csr_spmv_cqmat", "format": "CSR", "n": N, "c": C, "q": Q,
                        "seed_mult": 0, "values": "U[0,1) seed 0", "x": "U[0,1) seed 1",
                        "stored_nnz": g.stored, "nnz_field": g.nnz, "csr_schedule": sched,
-                       "fill_s": round(t_fill, 3)},
+                       "csr_layout": layout, "fill_s": round(t_fill, 3),
+                       "autotune_s": round(t_tune, 2)},
             "roofline": roof, "cpu_baseline": cpu, "cg": cg, **extra}
     print(json.dumps(line))
 

@@ -340,9 +340,11 @@ int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int s
                              void *stream);
 int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int *slices,
                              long long *remote_entries);
-/* Autotuner (no reference counterpart): times every CSR schedule on this
- * block with d_v / d_r (`iters` launches each) and keeps the fastest; other
- * formats have a single schedule.  *best_us receives its time per SpMV. */
+/* Autotuner (no reference counterpart): times every CSR schedule -- lanes
+ * per row 1..64 x rows per lane group 1, 2, 4, on the row layout and on the
+ * split layout (window 64, 8 and 16 slices) -- on this block with d_v / d_r
+ * (`iters` launches each) and keeps the fastest; other formats have a single
+ * schedule.  *best_us receives its time per SpMV. */
 int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                        float *best_us, void *stream);
 

@@ -461,23 +461,41 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
     if (best_us) *best_us = ms * 1000.0f / iters;
     return TBSLA_HIP_SUCCESS;
   }
-  int bestL = 0, bestR = 0;
-  float best = 0;
-  for (int L = 1; L <= 64; L <<= 1)
-    for (int R : {1, 2, 4}) {
-      m->lanes_per_row = L;
-      m->rows_per_group = R;
-      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
-      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
-      if (bestL == 0 || ms < best) {
-        best = ms;
-        bestL = L;
-        bestR = R;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // candidates: every (lanes, rows) schedule on the row layout, then on the
+  // split layout with 8 and 16 column slices (when the block has entries
+  // far from its diagonal)
+  struct Best {
+    int layout = -1, window = 0, slices = 0, L = 0, R = 0;
+    float ms = 0;
+  } best;
+  auto sweep = [&](int layout, int window, int slices) -> int {
+    for (int L = 1; L <= 64; L <<= 1)
+      for (int R : {1, 2, 4}) {
+        m->lanes_per_row = L;
+        m->rows_per_group = R;
+        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
+        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+        if (best.layout < 0 || ms < best.ms) best = Best{layout, window, slices, L, R, ms};
       }
-    }
-  m->lanes_per_row = bestL;
-  m->rows_per_group = bestR;
-  if (best_us) *best_us = best * 1000.0f / iters;
+    return TBSLA_HIP_SUCCESS;
+  };
+  split_release(m);
+  TBSLA_TRY(sweep(TBSLA_HIP_CSR_ROWS, 0, 0));
+  constexpr int kWindow = 64;
+  for (int slices : {8, 16}) {
+    if (!m->has_arrays || m->ln_row == 0) break;
+    TBSLA_TRY(split_build(m, kWindow, slices, s));
+    long long remote = 0;
+    TBSLA_TRY(tbsla_hip_get_csr_layout(m, nullptr, nullptr, nullptr, &remote));
+    if (remote == 0) break;
+    TBSLA_TRY(sweep(TBSLA_HIP_CSR_SPLIT, kWindow, slices));
+  }
+  split_release(m);
+  if (best.layout == TBSLA_HIP_CSR_SPLIT) TBSLA_TRY(split_build(m, best.window, best.slices, s));
+  m->lanes_per_row = best.L;
+  m->rows_per_group = best.R;
+  if (best_us) *best_us = best.ms * 1000.0f / iters;
   return TBSLA_HIP_SUCCESS;
 }
 

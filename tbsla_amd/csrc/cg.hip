@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   if (check) {
     const double rn = sqrt(rr);
     int fin = kRunning;
-    if (rn < S->stop)
+    if (rn < S->stop || rn == 0.0)
       fin = (a.verify && !S->true_res) ? kPending : kConverged;
     else if (!(rz_new > 0.0) || !isfinite(rz_new) || !isfinite(rr))
       fin = kBreakdown;
@@ -437,6 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int r
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) a.com->t0 = wall_clock64();
   CgSeg *S = a.seg + a.block_seg[b];
   if (b != S->blk0) return;
   if (seg_flag(S, &s_flag) == kInactive) return;
@@ -451,7 +452,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int r
     S->shift = 0;
     S->true_res = 1;
     S->t_done = 0;
-    if (b == 0) a.com->t0 = wall_clock64();
   }
 }
 
@@ -535,7 +535,7 @@ struct Buf {
 // What a captured graph bakes in: every kernel argument and the launch plan.
 struct GraphKey {
   CgArgs args;
-  const void *spmv_rowptr, *spmv_colidx, *spmv_values;
+  const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
   int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
 };
@@ -912,6 +912,7 @@ int cg_solve(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_ac
   key.spmv_rowptr = m->rowptr;
   key.spmv_colidx = m->colidx;
   key.spmv_values = m->values;
+  key.spmv_split = m->split;
   key.L = p.L;
   key.pre = pre;
   key.mixed = mixed;
