@@ -15,8 +15,9 @@
  *     tbsla_hip_malloc); h_* are host pointers.
  *   - `stream` is a hipStream_t passed as void* (NULL: the null stream).
  *     Work is enqueued on it; functions that return scalars to the host
- *     (iteration counts, norms) synchronise that stream.  The library creates
- *     no streams of its own.
+ *     (iteration counts, norms) synchronise that stream.  The only stream
+ *     the library creates is the one a handle's CG uses when it is given
+ *     the null stream (which cannot be captured into a hipGraph).
  *   - Thread-compatible: one handle per thread, or external synchronisation.
  *   - Geometry follows the reference exactly (n_row .. NC, nnz), including
  *     the count-pass nnz of CSR fill_cqmat (MatrixCSR.cpp:423-434) that can
@@ -327,8 +328,8 @@ int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double beta, double e
  * (1, 2 or 4); 0 restores the choice from the mean row length. */
 int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows);
 int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows);
-/* CSR execution layout (no counterpart in the reference; results within
- * rounding of the row-major sum).  TBSLA_HIP_CSR_ROWS: the CSR arrays as
+/* CSR / ELL execution layout (no counterpart in the reference; results
+ * within rounding of the row-major sum; ELL padding slots count as entries).  TBSLA_HIP_CSR_ROWS: the CSR arrays as
  * they are.  TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in
  * row order; the others are grouped by column slice (slices a multiple of 8,
  * at most 64) and multiplied by one XCD per slice, whose L2 then holds that

@@ -456,12 +456,33 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
                        float *best_us, void *stream) {
   if (!m || iters < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad argument");
   float ms = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (m->format == TBSLA_HIP_ELL && m->has_arrays && m->max_col > 0 && m->ln_row > 0) {
+    // ELL: its column-major kernel, or the split layout (8 or 16 slices)
+    split_release(m);
+    TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
+    TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+    float best = ms;
+    int best_slices = 0;
+    for (int slices : {8, 16}) {
+      TBSLA_TRY(split_build(m, 64, slices, s));
+      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
+      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+      if (ms < best) {
+        best = ms;
+        best_slices = slices;
+      }
+    }
+    split_release(m);
+    if (best_slices) TBSLA_TRY(split_build(m, 64, best_slices, s));
+    if (best_us) *best_us = best * 1000.0f / iters;
+    return TBSLA_HIP_SUCCESS;
+  }
   if (m->format != TBSLA_HIP_CSR) {
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
     if (best_us) *best_us = ms * 1000.0f / iters;
     return TBSLA_HIP_SUCCESS;
   }
-  hipStream_t s = static_cast<hipStream_t>(stream);
   // candidates: every (lanes, rows) schedule on the row layout, then on the
   // split layout with 8 and 16 column slices (when the block has entries
   // far from its diagonal)

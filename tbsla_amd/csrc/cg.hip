@@ -552,6 +552,7 @@ struct CgCache {
   Buf<int> block_seg;
   CgCom *h_com = nullptr;  // pinned status copies (2)
   hipEvent_t ev[2] = {nullptr, nullptr};
+  hipStream_t own = nullptr;  // used when the caller passes the null stream
   // factor G: FP32 copies of G and G^T
   tbsla_hip_matrix_s *factor = nullptr;
   int *t_rp = nullptr, *t_ci = nullptr;
@@ -588,6 +589,7 @@ struct CgCache {
     if (h_com) hipHostFree(h_com);
     for (auto &e : ev)
       if (e) hipEventDestroy(e);
+    if (own) hipStreamDestroy(own);
     if (exec) hipGraphExecDestroy(exec);
     drop_factor();
   }
@@ -708,9 +710,27 @@ int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, dou
 }  // namespace
 
 // The solver behind tbsla_hip_cg and tbsla_hip_cg_batched.
+int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_active,
+                const double *d_rhs, double *d_x, const tbsla_hip_cg_options &o,
+                tbsla_hip_cg_result *res, hipStream_t s);
+
+// The null stream cannot be captured into a graph: a solve asked for on it
+// runs on a stream of the handle's own (default priority), ordered after
+// the null stream's earlier work; it is synchronised on return.
 int cg_solve(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_active,
              const double *d_rhs, double *d_x, const tbsla_hip_cg_options &o,
              tbsla_hip_cg_result *res, hipStream_t s) {
+  if (s != nullptr) return cg_solve_on(m, nsys, h_rows, h_active, d_rhs, d_x, o, res, s);
+  if (!m->cg_cache) m->cg_cache = new CgCache();
+  CgCache &c = *static_cast<CgCache *>(m->cg_cache);
+  if (!c.own) TBSLA_HIP_CHECK(hipStreamCreateWithFlags(&c.own, hipStreamNonBlocking));
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());  // earlier null-stream work
+  return cg_solve_on(m, nsys, h_rows, h_active, d_rhs, d_x, o, res, c.own);
+}
+
+int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_active,
+                const double *d_rhs, double *d_x, const tbsla_hip_cg_options &o,
+                tbsla_hip_cg_result *res, hipStream_t s) {
   if (m->format != TBSLA_HIP_CSR) return fail(TBSLA_HIP_ERR_UNSUPPORTED, "CG needs a CSR block");
   if (m->ln_row != m->ln_col || m->f_row != m->f_col)
     return fail(TBSLA_HIP_ERR_INVALID, "CG needs a square diagonal block");
@@ -933,7 +953,10 @@ int cg_solve(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_ac
     int st = TBSLA_HIP_SUCCESS;
     for (int k = 0; k < B && st == TBSLA_HIP_SUCCESS; k++) st = enqueue_iteration(m, a, p, c.ax.p, 1 + k);
     hipError_t e = hipStreamEndCapture(s, &graph);
-    if (st) return st;
+    if (st) {
+      if (graph) hipGraphDestroy(graph);
+      return st;
+    }
     if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);

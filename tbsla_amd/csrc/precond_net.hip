@@ -518,7 +518,8 @@ __global__ __launch_bounds__(kBlock) void k_head_bwd_rows(
   lds[wave][H + 1 + lane] = a_wo;
   lds[wave][2 * H + 2 + lane] = a_ve;
   lds[wave][3 * H + 2 + lane] = a_c;
-  const float bd = wave_sum(a_bd), bo = wave_sum(a_bo);
+  // a_bd and a_bo hold the same per-node / per-edge sums in every lane
+  const float bd = __shfl(a_bd, 0, kWave), bo = __shfl(a_bo, 0, kWave);
   if (lane == 0) {
     lds[wave][H] = bd;
     lds[wave][2 * H + 1] = bo;

@@ -55,16 +55,30 @@ __device__ inline bool is_local(long long c, long long r, int window) {
   return c - r <= window && r - c <= window;
 }
 
+// Entries of row r: CSR rp[r]..rp[r+1], or, for ELL (ell_w > 0, column-major
+// with n rows), the ell_w slots j * n + r (padding slots included: they hold
+// (0, 0.0) and the reference multiplies them too, MatrixELL.cpp:181-205).
+struct RowSrc {
+  const int *rp;
+  int ell_w;
+  long long n;
+  __device__ int count(long long r) const { return ell_w > 0 ? ell_w : rp[r + 1] - rp[r]; }
+  __device__ long long at(long long r, int k) const {
+    return ell_w > 0 ? (long long)k * n + r : (long long)rp[r] + k;
+  }
+};
+
 __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, int window,
-                                                        const int *__restrict__ rp,
-                                                        const int *__restrict__ ci,
+                                                        RowSrc src, const int *__restrict__ ci,
                                                         int *__restrict__ lcnt,
                                                         int *__restrict__ rcnt) {
   for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
        r += (long long)gridDim.x * kBlock) {
     int l = 0, q = 0;
-    for (int j = rp[r]; j < rp[r + 1]; j++) {
-      if (is_local(ci[j] - f_col, r, window)) l++;
+    const int cnt = src.count(r);
+    for (int k = 0; k < cnt; k++) {
+      const long long c = (long long)max(ci[src.at(r, k)] - f_col, 0);
+      if (is_local(c, r, window)) l++;
       else q++;
     }
     lcnt[r] = l;
@@ -75,8 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, 
 // local entries in row order; remote entries in row-major order with their
 // (slice, row) sort key
 __global__ __launch_bounds__(kBlock) void k_split_fill(
-    long long n, long long n_col, int f_col, int window, int slices, int rb,
-    const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+    long long n, long long n_col, int f_col, int window, int slices, int rb, RowSrc src,
+    const int *__restrict__ ci, const double *__restrict__ v,
     const int *__restrict__ lrp, const int *__restrict__ rptr, int *__restrict__ lci,
     double *__restrict__ lv, int *__restrict__ rc, double *__restrict__ rv,
     unsigned long long *__restrict__ keys, int *__restrict__ payload,
@@ -84,8 +98,11 @@ __global__ __launch_bounds__(kBlock) void k_split_fill(
   for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
        r += (long long)gridDim.x * kBlock) {
     int pl = lrp[r], pr = rptr[r];
-    for (int j = rp[r]; j < rp[r + 1]; j++) {
-      const long long c = ci[j] - f_col;
+    const int cnt = src.count(r);
+    for (int k = 0; k < cnt; k++) {
+      const long long j = src.at(r, k);
+      // ELL clamps a negative index to 0 (MatrixELL.cpp:198-199)
+      const long long c = (long long)max(ci[j] - f_col, 0);
       if (is_local(c, r, window)) {
         lci[pl] = (int)c;
         lv[pl] = v[j];
@@ -313,7 +330,8 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
       (st = device_alloc(&ws, scan_workspace_bytes(n + 1) + 64)))
     return failed(st);
   long long *d_total = reinterpret_cast<long long *>(static_cast<char *>(ws) + scan_workspace_bytes(n + 1));
-  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, m->rowptr, m->colidx, lcnt, rcnt);
+  RowSrc src{m->rowptr, m->format == TBSLA_HIP_ELL ? m->max_col : 0, n};
+  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, src, m->colidx, lcnt, rcnt);
   long long tot[2] = {0, 0};
   int *cnts[2] = {lcnt, rcnt}, *ptrs[2] = {sp->lrp, sp->rptr};
   for (int k = 0; k < 2; k++) {
@@ -340,7 +358,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
     return failed(st);
   hipMemsetAsync(scount, 0, (size_t)(slices + 1) * 4, s);
   const int rb = bits_of(std::max(n, 1LL));
-  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, rb, m->rowptr, m->colidx,
+  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, rb, src, m->colidx,
                                            m->values, sp->lrp, sp->rptr, sp->lci, sp->lv, rc, rv, keys,
                                            pay, scount);
   if (sp->nr > 1 && (st = radix_sort_pairs(keys, pay, sp->nr, rb + bits_of(slices), s))) return failed(st);
@@ -367,7 +385,8 @@ using namespace tbsla_hip;
 extern "C" int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int slices,
                                         void *stream) {
   if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null handle");
-  if (m->format != TBSLA_HIP_CSR) return fail(TBSLA_HIP_ERR_UNSUPPORTED, "CSR only");
+  if (m->format != TBSLA_HIP_CSR && m->format != TBSLA_HIP_ELL)
+    return fail(TBSLA_HIP_ERR_UNSUPPORTED, "CSR and ELL blocks only");
   if (layout == TBSLA_HIP_CSR_ROWS) {
     split_release(m);
     return TBSLA_HIP_SUCCESS;
@@ -375,7 +394,8 @@ extern "C" int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int wi
   if (layout != TBSLA_HIP_CSR_SPLIT) return fail(TBSLA_HIP_ERR_INVALID, "unknown layout");
   if (window < 0 || slices < 1 || slices > 64 || slices % 8)
     return fail(TBSLA_HIP_ERR_INVALID, "window >= 0, slices a multiple of 8 up to 64");
-  if (!m->has_arrays || m->ln_row == 0) return fail(TBSLA_HIP_ERR_INVALID, "empty block");
+  if (!m->has_arrays || m->ln_row == 0 || (m->format == TBSLA_HIP_ELL && m->max_col == 0))
+    return fail(TBSLA_HIP_ERR_INVALID, "empty block");
   return split_build(m, window, slices, static_cast<hipStream_t>(stream));
 }
 

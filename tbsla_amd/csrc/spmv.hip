@@ -345,6 +345,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
         return TBSLA_HIP_SUCCESS;
       }
+      if (m->split) return split_launch(m, d_r, d_v, d_done, s);
       k_ell_spmv<<<grid_for(m->ln_row), kBlock, 0, s>>>(m->ln_row, m->max_col, m->colidx,
                                                          m->values, d_v, m->f_col, d_r, d_done);
       TBSLA_HIP_LAUNCH_CHECK();

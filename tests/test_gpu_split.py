@@ -69,11 +69,23 @@ def test_split_remote_count_and_normalize():
     assert g.get_csr_layout()["layout"] == T.CSR_ROWS
 
 
+@pytest.mark.parametrize("grid", [(0, 0, 1, 1), (1, 0, 2, 2)])
+def test_split_ell_matches_oracle(grid):
+    n = 20000
+    o = O.fill_cqmat(O.ELL, n, n, 16, 0.1, 0, *grid, value_seed=5)
+    g = T.Matrix(T.ELL).fill_cqmat(n, n, 16, 0.1, 0, *grid, value_seed=5)
+    x = O.unit_vector(3, o.geom["ln_col"])
+    g.set_csr_layout(T.CSR_SPLIT, 64, 8)
+    y = g.spmv(x)
+    assert rel_max(y, o.spmv(x)) < 1e-12
+    assert np.array_equal(y, g.spmv(x))
+
+
 def test_split_bad_arguments():
     g = T.Matrix(T.CSR).fill_cqmat(100, 100, 4, 0.1, 0)
     for w, s in [(-1, 8), (64, 0), (64, 12), (64, 128)]:
         with pytest.raises(T.TbslaHipError):
             g.set_csr_layout(T.CSR_SPLIT, w, s)
-    e = T.Matrix(T.ELL).fill_cqmat(100, 100, 4, 0.1, 0)
+    e = T.Matrix(T.COO).fill_cqmat(100, 100, 4, 0.1, 0)
     with pytest.raises(T.TbslaHipError):
         e.set_csr_layout(T.CSR_SPLIT, 64, 8)
