@@ -456,7 +456,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int r
 }
 
 // After verification: accept (true |r| below the threshold) or resume at
-// global iteration j from the replaced residual.
+// the next global iteration j from the replaced residual (only j's parity
+// matters).
 __global__ __launch_bounds__(kBlock) void k_cg_vcheck(CgArgs a, int j) {
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
@@ -475,9 +476,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_vcheck(CgArgs a, int j) {
       S->done = kConverged;
       atomicAdd(&a.com->ndone, 1);
     } else {
+      // the recurrence had drifted from the true residual: restart from it
+      // (beta = 0 at the next iteration: an infinite "old r.z")
       S->shift = (j ^ S->stop_par) & 1;
       S->done = kRunning;
     }
+  }
+  __syncthreads();
+  if (!(sqrt(rr) < S->stop)) {
+    double *old_rz = a.prz[S->stop_par];
+    for (int k = S->blk0 + (int)threadIdx.x; k < S->blk1; k += kBlock)
+      old_rz[k] = k == S->blk0 ? INFINITY : 0.0;
+  }
+}
+
+// ax = A x (FP64 matrix) for the rows of the systems the mode selects, in
+// the engine's workgroup layout (workgroups of other systems return at
+// once); L lanes per row.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_cg_ax(CgArgs a, int par_g, int mode,
+                                                  double *__restrict__ ax) {
+  __shared__ int s_flag;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  int rsel;
+  if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
+  const int r0 = S->row0 + (b - S->blk0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const int lane = threadIdx.x & (L - 1);
+  for (int row = r0 + (int)threadIdx.x / L; row < r1; row += kBlock / L) {
+    double acc = 0.0;
+    for (int j = a.rowptr[row] + lane; j < a.rowptr[row + 1]; j += L)
+      acc = fma(a.val64[j], a.x[a.colidx[j] - a.f_col], acc);
+    acc = lane_sum<L>(acc);
+    if (lane == 0) ax[row] = acc;
   }
 }
 
@@ -559,8 +591,8 @@ struct CgCache {
   double *t_v64 = nullptr;
   float *g_v32 = nullptr, *t_v32 = nullptr;
   size_t f32_len = 0;
-  // graph
-  hipGraphExec_t exec = nullptr;
+  // graphs: [0] plain batch, [1] batch ending with a replacement step
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   GraphKey key{};
   std::vector<CgSeg> host_seg;
   std::vector<int> host_blocks;
@@ -590,7 +622,8 @@ struct CgCache {
     for (auto &e : ev)
       if (e) hipEventDestroy(e);
     if (own) hipStreamDestroy(own);
-    if (exec) hipGraphExecDestroy(exec);
+    for (auto &x : exec)
+      if (x) hipGraphExecDestroy(x);
     drop_factor();
   }
 };
@@ -682,9 +715,22 @@ void launch_residual(const CgArgs &a, const Plan &p, const double *ax, int par, 
 }
 
 // r = b - A x with the FP64 matrix, for the systems the mode selects
+void launch_ax(const CgArgs &a, const Plan &p, int par, int mode, double *ax) {
+  switch (p.L) {
+    case 1: k_cg_ax<1><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    case 2: k_cg_ax<2><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    case 4: k_cg_ax<4><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    case 8: k_cg_ax<8><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    case 16: k_cg_ax<16><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    case 32: k_cg_ax<32><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+    default: k_cg_ax<64><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
+  }
+}
+
 int true_residual(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double *ax, int par,
                   int mode) {
-  TBSLA_TRY(spmv_launch(m, ax, a.x, 0, true, p.s));
+  (void)m;
+  launch_ax(a, p, par, mode, ax);
   launch_residual(a, p, ax, par, mode, 0);
   if (p.pre == 2) {
     launch_gtr(a, p, par, mode);
@@ -693,16 +739,31 @@ int true_residual(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double 
   return TBSLA_HIP_SUCCESS;
 }
 
-// iteration i (global): K1 (tested when i > 0), K2, K3, replacement
-int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double *ax, int i) {
+// iteration i (global): K1 (tested when i > 0), K2, K3; then, after every
+// replace_every-th iteration (i a multiple of it), the true FP64 residual
+// replaces the recurrence's.  tail: also settle the systems waiting for
+// their true-residual test (a replacement step does it anyway: the same
+// FP64 product serves both), so they wait at most one batch, without a host
+// round trip.
+int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double *ax, int i,
+                      bool tail) {
   const int par = i & 1;
   launch_k1_any(a, p, par, i > 0, 0);
   launch_k2(a, p, par, i > 0);
-  const bool replace = p.replace > 0 && (i + 1) % p.replace == 0;
+  const bool replace = p.replace > 0 && i > 0 && i % p.replace == 0;
   if (replace) {
     TBSLA_TRY(true_residual(m, a, p, ax, par, kAfterIter));
   } else if (p.pre == 2) {
     launch_gz(a, p, par, kAfterIter);
+  }
+  if (p.verify && (replace || tail)) {
+    launch_ax(a, p, par, kVerify, ax);
+    launch_residual(a, p, ax, par, kVerify, 0);
+    if (p.pre == 2) {
+      launch_gtr(a, p, par, kVerify);
+      launch_gz(a, p, par, kVerify);
+    }
+    k_cg_vcheck<<<p.nblocks, kBlock, 0, p.s>>>(a, (i + 1) & 1);
   }
   return TBSLA_HIP_SUCCESS;
 }
@@ -911,19 +972,23 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   k_cg_start<<<nblocks, kBlock, 0, s>>>(a, o.tolerance, o.relative, near_zero);
   TBSLA_HIP_LAUNCH_CHECK();
 
-  // graph batch: an even number of iterations, a multiple of replace_every
+  // Graph batches of B iterations starting at 1 + tB (B even, so every
+  // node's iteration parity is fixed): variant 0 without a replacement
+  // step, variant 1 whose last iteration (a multiple of replace_every) has
+  // one.  B: 16, or the largest even divisor of replace_every up to 16.
   int B = 0;
   if (o.graph_batch >= 0) {
     B = o.graph_batch > 0 ? o.graph_batch : 16;
     if (p.replace > 0) {
-      B = p.replace % 2 == 0 ? p.replace : 2 * p.replace;
-      if (B > 128) B = 0;
+      B = 0;
+      for (int d = 2; d <= 16; d += 2)
+        if (p.replace % d == 0) B = d;
     }
     if (B % 2) B = 0;
   }
   int it = 0;  // next global iteration to enqueue
   if (max_it > 0) {
-    TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, 0));
+    TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, 0, false));
     it = 1;
   }
   GraphKey key;
@@ -941,28 +1006,36 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.nblocks = nblocks;
   key.spmv_lanes = pick_csr_lanes(m);
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
-  if (B > 0 && !(c.exec && c.key == key)) {
-    if (c.exec) {
-      hipGraphExecDestroy(c.exec);
-      c.exec = nullptr;
-    }
-    // iterations 1 + kB ... : the first is odd; the replacement pattern
-    // repeats with period B
-    hipGraph_t graph = nullptr;
-    TBSLA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = TBSLA_HIP_SUCCESS;
-    for (int k = 0; k < B && st == TBSLA_HIP_SUCCESS; k++) st = enqueue_iteration(m, a, p, c.ax.p, 1 + k);
-    hipError_t e = hipStreamEndCapture(s, &graph);
-    if (st) {
-      if (graph) hipGraphDestroy(graph);
-      return st;
-    }
-    if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-      c.exec = nullptr;
-      return fail(TBSLA_HIP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  if (B > 0 && !(c.exec[0] && c.key == key)) {
+    for (auto &x : c.exec)
+      if (x) {
+        hipGraphExecDestroy(x);
+        x = nullptr;
+      }
+    const int variants = p.replace > 0 ? 2 : 1;
+    for (int v = 0; v < variants; v++) {
+      // variant v: iterations 1..B; with v == 1 the last one replaces (its
+      // index is taken as a multiple of replace_every of the same parity)
+      hipGraph_t graph = nullptr;
+      TBSLA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int st = TBSLA_HIP_SUCCESS;
+      for (int k = 0; k < B && st == TBSLA_HIP_SUCCESS; k++) {
+        const bool last = k == B - 1;
+        const int i = (v == 1 && last) ? B * p.replace : 1 + k;
+        st = enqueue_iteration(m, a, p, c.ax.p, i, last);
+      }
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (st) {
+        if (graph) hipGraphDestroy(graph);
+        return st;
+      }
+      if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c.exec[v], graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        c.exec[v] = nullptr;
+        return fail(TBSLA_HIP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
     }
     c.key = key;
   }
@@ -975,11 +1048,13 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   bool finished = false;
   while (it < max_it && !finished) {
     if (B > 0 && it + B <= max_it) {
-      TBSLA_HIP_CHECK(hipGraphLaunch(c.exec, s));
+      const int last = it + B - 1;  // the batch's last iteration
+      const bool rep = p.replace > 0 && last % p.replace == 0;
+      TBSLA_HIP_CHECK(hipGraphLaunch(c.exec[rep ? 1 : 0], s));
       it += B;
     } else {
       const int stop = std::min(max_it, it + poll_every);
-      for (; it < stop; it++) TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, it));
+      for (; it < stop; it++) TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, it, it == stop - 1));
     }
     TBSLA_HIP_LAUNCH_CHECK();
     const int slot = batches & 1;
