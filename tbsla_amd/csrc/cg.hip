@@ -99,20 +99,36 @@ struct CgArgs {
   int stall_window;  // > 0: breakdown when |r| has not dropped below |r0| by then
 };
 
-// Sum of a system's workgroup partials, identical in every workgroup: thread
-// k adds partials k, k+256, ... in order; waves reduce by xor butterflies
-// (every lane gets the same bits); wave totals are added in wave order.
-__device__ double seg_total(const double *__restrict__ p, int b0, int b1, double *lds) {
-  double v = 0.0;
-  for (int i = b0 + (int)threadIdx.x; i < b1; i += kBlock) v += p[i];
-  v = group_sum<kWave>(v);
-  __syncthreads();
-  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
-  __syncthreads();
-  double t = lds[0];
+// Sums of a system's workgroup partials are taken in one fixed order:
+// thread k adds partials k, k+256, ... in order; waves reduce by xor
+// butterflies (every lane gets the same bits); wave totals are added in wave
+// order.
+// Up to three such sums in one pass (one chain of loads); results in every
+// thread.  Every workgroup of a system takes them from the same partials in
+// the same order, so all hold bitwise-equal totals.
+__device__ void seg_sums(const double *__restrict__ p0, const double *__restrict__ p1,
+                         const double *__restrict__ p2, int np, int b0, int b1, double *out) {
+  __shared__ double lds[3][kBlock / kWave];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int i = b0 + (int)threadIdx.x; i < b1; i += kBlock) {
+    v[0] += p0[i];
+    if (np > 1) v[1] += p1[i];
+    if (np > 2) v[2] += p2[i];
+  }
 #pragma unroll
-  for (int w = 1; w < kBlock / kWave; w++) t += lds[w];
-  return t;
+  for (int k = 0; k < 3; k++) v[k] = group_sum<kWave>(v[k]);
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+    for (int k = 0; k < 3; k++) lds[k][threadIdx.x / kWave] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double t = lds[k][0];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; w++) t += lds[k][w];
+    out[k] = t;
+  }
 }
 
 // workgroup sum, result valid in thread 0
@@ -158,13 +174,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   if (seg_flag(S, &s_flag) != kRunning) return;
   const int par = (par_g ^ S->shift) & 1;
-  const int b0 = S->blk0, b1 = S->blk1;
-  const double rz_new = seg_total(a.prz[par ^ 1], b0, b1, lds);
-  const double rz_old = seg_total(a.prz[par], b0, b1, lds);
-  const double rr = ZBUF ? seg_total(a.prr[par ^ 1], b0, b1, lds) : rz_new;
+  const int b0 = S->blk0;
+  double t[3];
+  seg_sums(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], ZBUF ? 3 : 2, b0, S->blk1, t);
+  const double rz_new = t[0], rz_old = t[1];
+  const double rr = ZBUF ? t[2] : rz_new;
   if (check) {
     const double rn = sqrt(rr);
     int fin = kRunning;
@@ -238,12 +256,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   if (seg_flag(S, &s_flag) != kRunning) return;
   const int par = (par_g ^ S->shift) & 1;
-  const int b0 = S->blk0, b1 = S->blk1;
-  const double rz = seg_total(a.prz[par ^ 1], b0, b1, lds);
-  const double pap = seg_total(a.ppap, b0, b1, lds);
+  const int b0 = S->blk0;
+  double t[3];
+  seg_sums(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, S->blk1, t);
+  const double rz = t[0], pap = t[1];
   const double alpha = rz / fmax(pap, S->near_zero);
   const double *rc = a.r[par];
   double *rn = a.r[par ^ 1];
@@ -325,7 +345,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode)
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   int rsel;
   if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
   const double *rc = a.r[rsel];
@@ -392,7 +413,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(CgArgs a, const double *
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   int rsel;
   if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
   double *rc = a.r[rsel];
@@ -424,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(CgArgs a, const double *
     if (PRE == 0) a.prz[rsel ^ 1][b] = rr;
     if (PRE == 1) a.prz[rsel ^ 1][b] = rz;
     if (mode == kStart) {
-      a.prz[0][b] = b == b0 ? INFINITY : 0.0;
+      a.prz[0][b] = b == b0 ? INFINITY : 0.0;  // the first beta is 0
       a.ppap[b] = bb;
     }
     if (b == b0) S->true_res = 1;
@@ -438,11 +460,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int r
   __shared__ int s_flag;
   const int b = blockIdx.x;
   if (b == 0 && threadIdx.x == 0) a.com->t0 = wall_clock64();
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   if (b != S->blk0) return;
   if (seg_flag(S, &s_flag) == kInactive) return;
-  const double bb = seg_total(a.ppap, S->blk0, S->blk1, lds);
-  const double rr = seg_total(a.prr[1], S->blk0, S->blk1, lds);
+  (void)lds;
+  (void)sid;
+  double t[3];
+  seg_sums(a.ppap, a.prr[1], nullptr, 2, S->blk0, S->blk1, t);
+  const double bb = t[0], rr = t[1];
   if (threadIdx.x == 0) {
     S->stop = relative ? tol * sqrt(bb) : tol;
     S->near_zero = near_zero;
@@ -462,11 +488,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_vcheck(CgArgs a, int j) {
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
+  const int sid = a.block_seg[b];
+  CgSeg *S = a.seg + sid;
   if (b != S->blk0) return;
   if (seg_flag(S, &s_flag) != kPending) return;
-  const int psel = S->stop_par ^ 1;
-  const double rr = seg_total(a.prr[psel], S->blk0, S->blk1, lds);
+  (void)lds;
+  (void)sid;
+  double t[3];
+  seg_sums(a.prr[S->stop_par ^ 1], nullptr, nullptr, 1, S->blk0, S->blk1, t);
+  const double rr = t[0];
   if (threadIdx.x == 0) {
     const double rn = sqrt(rr);
     S->rnorm = rn;
@@ -581,6 +611,7 @@ struct CgCache {
   Buf<double> ppap, prr0, prr1, prz0, prz1;
   Buf<CgSeg> seg;
   Buf<CgCom> com;
+
   Buf<int> block_seg;
   CgCom *h_com = nullptr;  // pinned status copies (2)
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -617,6 +648,7 @@ struct CgCache {
     t32.release();
     seg.release();
     com.release();
+
     block_seg.release();
     if (h_com) hipHostFree(h_com);
     for (auto &e : ev)
@@ -818,15 +850,28 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   if (!m->cg_cache) m->cg_cache = new CgCache();
   CgCache &c = *static_cast<CgCache *>(m->cg_cache);
 
-  // workgroup layout: each system gets ceil(rows / rpb) workgroups
+  // lanes per row of the iteration's SpMV: a power of two near an eighth of
+  // the mean row length (rows are short in the SPD problems CG serves; one
+  // lane per row lets a workgroup cover its rows in one pass)
+  int L = o.lanes_per_row;
+  if (L <= 0) {
+    const double mean = n > 0 ? (double)m->stored / n : 1.0;
+    L = 1;
+    while (L < 64 && L < mean / 8) L <<= 1;
+  }
+  if (L < 1 || L > 64 || (L & (L - 1))) return fail(TBSLA_HIP_ERR_INVALID, "lanes_per_row");
+
+  // workgroup layout: a workgroup covers whole passes of kBlock / L rows (one
+  // pass when the system allows), at most kMaxBlocksPerSystem per system
   std::vector<CgSeg> &hs = c.host_seg;
   std::vector<int> &hb = c.host_blocks;
   hs.assign(nsys, CgSeg{});
   hb.clear();
+  const long long pass_rows = kBlock / L;
   for (int k = 0; k < nsys; k++) {
     const long long rows = h_rows[k + 1] - h_rows[k];
-    long long rpb = std::max<long long>(kBlock, (rows + kMaxBlocksPerSystem - 1) / kMaxBlocksPerSystem);
-    rpb = (rpb + kBlock - 1) / kBlock * kBlock;
+    long long rpb = std::max<long long>(pass_rows, (rows + kMaxBlocksPerSystem - 1) / kMaxBlocksPerSystem);
+    rpb = (rpb + pass_rows - 1) / pass_rows * pass_rows;
     const int nb = (int)((rows + rpb - 1) / rpb);
     CgSeg &S = hs[k];
     S.row0 = h_rows[k];
@@ -848,6 +893,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   for (auto *v : parts) TBSLA_TRY(v->ensure((size_t)nblocks));
   TBSLA_TRY(c.seg.ensure(nsys));
   TBSLA_TRY(c.com.ensure(1));
+
   TBSLA_TRY(c.block_seg.ensure(nblocks));
   if (!c.h_com) TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_com, 2 * sizeof(CgCom), hipHostMallocDefault));
   for (auto &e : c.ev)
@@ -925,12 +971,12 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.seg = c.seg.p;
   a.com = c.com.p;
   a.block_seg = c.block_seg.p;
+
   a.verify = (mixed || o.replace_every > 0) ? 1 : 0;
   a.stall_window = o.stall_window;
 
   Plan p;
-  p.L = o.lanes_per_row > 0 ? o.lanes_per_row : pick_csr_lanes(m);
-  if (p.L < 1 || p.L > 64 || (p.L & (p.L - 1))) return fail(TBSLA_HIP_ERR_INVALID, "lanes_per_row");
+  p.L = L;
   p.pre = pre;
   p.mixed = mixed;
   p.replace = std::max(0, o.replace_every);
@@ -945,6 +991,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.seg.p, hs.data(), sizeof(CgSeg) * nsys, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.block_seg.p, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.com.p, &com0, sizeof com0, hipMemcpyHostToDevice, s));
+
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.b.p, d_rhs, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
   // x of inactive systems stays as given; active ones start at 0 unless the
   // caller's x is the initial guess

@@ -52,6 +52,8 @@ constexpr int obo = owo + H;             // 1
 constexpr int owd = obo + 1;             // H
 constexpr int obd = owd + H;             // 1
 constexpr int kParams = obd + 1;
+static_assert(ob1 % 4 == 0 && ob2 % 4 == 0 && ove % 4 == 0 && oc % 4 == 0 && owo % 4 == 0,
+              "offsets loaded as 16-byte vectors");
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -174,7 +176,10 @@ __global__ __launch_bounds__(kBlock) void k_prep_tmap(long long n, const int *__
 // forward
 // ---------------------------------------------------------------------------
 
-// wave per node, lane = output channel
+// Layer 0: a workgroup takes 64 nodes.  Threads 0..63 form each node's
+// 8 inputs [x_i, sum_j e_ij x_j] (into LDS and z0); then every thread
+// computes 16 of the 64 x 64 outputs, channel = thread % 64, with its W0
+// column in registers (pre = b0 + sum_k z_k W0[k][c], k in order).
 __global__ __launch_bounds__(kBlock) void k_layer0(long long n, const int *__restrict__ erp,
                                                    const int *__restrict__ eci,
                                                    const float *__restrict__ ev,
@@ -182,34 +187,51 @@ __global__ __launch_bounds__(kBlock) void k_layer0(long long n, const int *__res
                                                    const float *__restrict__ th,
                                                    float *__restrict__ z0, float *__restrict__ h,
                                                    bf16_t *__restrict__ hb) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const long long waves = (long long)gridDim.x * (kBlock / kWave);
-  for (long long i = (long long)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < n;
-       i += waves) {
+  __shared__ float zs[64][2 * F];
+  const long long n0 = (long long)blockIdx.x * 64;
+  if (threadIdx.x < 64) {
+    const long long i = n0 + threadIdx.x;
     float z[2 * F];
 #pragma unroll
-    for (int k = 0; k < F; k++) {
-      z[k] = x[i * F + k];
-      z[F + k] = 0.f;
-    }
-    for (int j = erp[i]; j < erp[i + 1]; j++) {
-      const float e = ev[j];
-      const long long c = eci[j];
+    for (int k = 0; k < 2 * F; k++) z[k] = 0.f;
+    if (i < n) {
 #pragma unroll
-      for (int k = 0; k < F; k++) z[F + k] = fmaf(e, x[c * F + k], z[F + k]);
-    }
-    float pre = th[ob0 + lane];
+      for (int k = 0; k < F; k++) z[k] = x[i * F + k];
+      for (int j = erp[i]; j < erp[i + 1]; j++) {
+        const float e = ev[j];
+        const long long c = eci[j];
 #pragma unroll
-    for (int k = 0; k < 2 * F; k++) pre = fmaf(z[k], th[oW0 + k * H + lane], pre);
+        for (int k = 0; k < F; k++) z[F + k] = fmaf(e, x[c * F + k], z[F + k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 2 * F; k++) z0[i * 2 * F + k] = z[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * F; k++) zs[threadIdx.x][k] = z[k];
+  }
+  __syncthreads();
+  const int c = threadIdx.x & (kWave - 1);
+  float w[2 * F];
+#pragma unroll
+  for (int k = 0; k < 2 * F; k++) w[k] = th[oW0 + k * H + c];
+  const float b0 = th[ob0 + c];
+  for (int nd = threadIdx.x / kWave; nd < 64; nd += kBlock / kWave) {
+    const long long i = n0 + nd;
+    if (i >= n) break;
+    float pre = b0;
+#pragma unroll
+    for (int k = 0; k < 2 * F; k++) pre = fmaf(zs[nd][k], w[k], pre);
     const float hv = fmaxf(pre, 0.f);
-    h[i * H + lane] = hv;
-    hb[i * H + lane] = to_bf16(hv);
-    if (lane < 2 * F) z0[i * 2 * F + lane] = z[lane];
+    h[i * H + c] = hv;
+    hb[i * H + c] = to_bf16(hv);
   }
 }
 
+typedef float floatq __attribute__((ext_vector_type(4)));
+
 // h' = relu(U[:, :64] + E U[:, 64:] + b); the last layer also gives the
-// diagonal head g^_ii = exp(wd . h + bd)
+// diagonal head g^_ii = exp(wd . h + bd).  Four nodes per wavefront, 16
+// lanes per node, four channels per lane (16-byte loads).
 __global__ __launch_bounds__(kBlock) void k_aggregate(long long n, const int *__restrict__ erp,
                                                       const int *__restrict__ eci,
                                                       const float *__restrict__ ev,
@@ -217,18 +239,35 @@ __global__ __launch_bounds__(kBlock) void k_aggregate(long long n, const int *__
                                                       const float *__restrict__ th, int ob,
                                                       float *__restrict__ h, bf16_t *__restrict__ hb,
                                                       float *__restrict__ gdiag) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const long long waves = (long long)gridDim.x * (kBlock / kWave);
-  for (long long i = (long long)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < n;
-       i += waves) {
-    float pre = U[i * H2 + lane] + th[ob + lane];
-    for (int j = erp[i]; j < erp[i + 1]; j++) pre = fmaf(ev[j], U[(long long)eci[j] * H2 + H + lane], pre);
-    const float hv = fmaxf(pre, 0.f);
-    h[i * H + lane] = hv;
-    hb[i * H + lane] = to_bf16(hv);
+  const int q = threadIdx.x & 15;  // channels 4q .. 4q+3
+  const long long groups = (long long)gridDim.x * (kBlock / 16);
+  const floatq bias = *reinterpret_cast<const floatq *>(th + ob + 4 * q);
+  // wd starts at an offset that is not a multiple of four floats: scalar loads
+  const floatq wd = {th[owd + 4 * q], th[owd + 4 * q + 1], th[owd + 4 * q + 2], th[owd + 4 * q + 3]};
+  const float bd = th[obd];
+  for (long long i = (long long)blockIdx.x * (kBlock / 16) + threadIdx.x / 16; i < n; i += groups) {
+    floatq pre = *reinterpret_cast<const floatq *>(U + i * H2 + 4 * q) + bias;
+    for (int j = erp[i]; j < erp[i + 1]; j++) {
+      const float e = ev[j];
+      const floatq u = *reinterpret_cast<const floatq *>(U + (long long)eci[j] * H2 + H + 4 * q);
+      pre.x = fmaf(e, u.x, pre.x);
+      pre.y = fmaf(e, u.y, pre.y);
+      pre.z = fmaf(e, u.z, pre.z);
+      pre.w = fmaf(e, u.w, pre.w);
+    }
+    floatq hv;
+    hv.x = fmaxf(pre.x, 0.f);
+    hv.y = fmaxf(pre.y, 0.f);
+    hv.z = fmaxf(pre.z, 0.f);
+    hv.w = fmaxf(pre.w, 0.f);
+    *reinterpret_cast<floatq *>(h + i * H + 4 * q) = hv;
+    const uint32_t lo = (uint32_t)to_bf16(hv.x) | ((uint32_t)to_bf16(hv.y) << 16);
+    const uint32_t hi = (uint32_t)to_bf16(hv.z) | ((uint32_t)to_bf16(hv.w) << 16);
+    *reinterpret_cast<uint2 *>(hb + i * H + 4 * q) = make_uint2(lo, hi);
     if (gdiag) {
-      const float q = wave_sum(th[owd + lane] * hv) + th[obd];
-      if (lane == 0) gdiag[i] = expf(q);
+      float d = wd.x * hv.x + wd.y * hv.y + wd.z * hv.z + wd.w * hv.w;
+      d = group_sumf<16>(d);
+      if (q == 0) gdiag[i] = expf(d + bd);
     }
   }
 }
@@ -333,7 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm_wgrad(long long n, const bf16_t
       out[(wave * 16 + (lane >> 4) * 4 + q) * H2 + t * 16 + (lane & 15)] = acc[t][q];
 }
 
-// edge head: wave per row i, one lower edge at a time
+// edge head: four rows per wavefront, 16 lanes per row, four channels per
+// lane; the row's lower edges one at a time
 __global__ __launch_bounds__(kBlock) void k_edge_head(long long n, const int *__restrict__ erp,
                                                       const int *__restrict__ eci,
                                                       const float *__restrict__ ev,
@@ -345,24 +385,29 @@ __global__ __launch_bounds__(kBlock) void k_edge_head(long long n, const int *__
                                                       const double *__restrict__ s,
                                                       float *__restrict__ ghat,
                                                       double *__restrict__ gval) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const long long waves = (long long)gridDim.x * (kBlock / kWave);
-  const float ve = th[ove + lane], cc = th[oc + lane], wo = th[owo + lane], bo = th[obo];
-  for (long long i = (long long)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < n;
-       i += waves) {
-    const float pi = P[i * H2 + lane];
+  const int q = threadIdx.x & 15;
+  const long long groups = (long long)gridDim.x * (kBlock / 16);
+  const floatq ve = *reinterpret_cast<const floatq *>(th + ove + 4 * q);
+  const floatq cc = *reinterpret_cast<const floatq *>(th + oc + 4 * q);
+  const floatq wo = *reinterpret_cast<const floatq *>(th + owo + 4 * q);
+  const float bo = th[obo];
+  for (long long i = (long long)blockIdx.x * (kBlock / 16) + threadIdx.x / 16; i < n; i += groups) {
+    const floatq pi = *reinterpret_cast<const floatq *>(P + i * H2 + 4 * q);
     const int lo = nlow[i], g0 = grp[i];
     for (int k = 0; k < lo; k++) {
       const int j = eci[erp[i] + k];
       const float e = ev[erp[i] + k];
-      const float sv = pi + P[(long long)j * H2 + H + lane] + e * ve + cc;
-      const float g = wave_sum(wo * fmaxf(sv, 0.f)) + bo;
-      if (lane == 0) {
+      const floatq pj = *reinterpret_cast<const floatq *>(P + (long long)j * H2 + H + 4 * q);
+      const floatq sv = pi + pj + e * ve + cc;
+      float g = wo.x * fmaxf(sv.x, 0.f) + wo.y * fmaxf(sv.y, 0.f) + wo.z * fmaxf(sv.z, 0.f) +
+                wo.w * fmaxf(sv.w, 0.f);
+      g = group_sumf<16>(g) + bo;
+      if (q == 0) {
         ghat[g0 + k] = g;
         if (gval) gval[g0 + k] = s[i] * (double)g;
       }
     }
-    if (lane == 0) {
+    if (q == 0) {
       ghat[g0 + lo] = gdiag[i];
       if (gval) gval[g0 + lo] = s[i] * (double)gdiag[i];
     }
@@ -826,20 +871,21 @@ int forward(tbsla_hip_precond_net_s *net, double *gval, hipStream_t s) {
   TBSLA_TRY(net->P.ensure(n * H2));
   TBSLA_TRY(net->gdiag.ensure(n));
   TBSLA_TRY(net->ghat.ensure(net->ng));
-  const unsigned rg = row_grid(n);
   const unsigned gg = (unsigned)((n + 63) / 64);
+  const unsigned qg = (unsigned)std::min<long long>(4096, (n + 15) / 16);  // 16 nodes per workgroup
   const float *th = net->th.p;
-  k_layer0<<<rg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->x.p, th, net->z0.p,
-                                 net->h1.p, net->hb1.p);
+  k_layer0<<<(unsigned)((n + 63) / 64), kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p,
+                                                         net->x.p, th, net->z0.p, net->h1.p,
+                                                         net->hb1.p);
   k_gemm_rows<H, H2><<<gg, kBlock, 0, s>>>(n, net->hb1.p, net->wt.p, net->U.p, 0, nullptr);
-  k_aggregate<<<rg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->U.p, th, ob1,
+  k_aggregate<<<qg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->U.p, th, ob1,
                                     net->h2.p, net->hb2.p, nullptr);
   k_gemm_rows<H, H2><<<gg, kBlock, 0, s>>>(n, net->hb2.p, net->wt.p + H * H2, net->U.p, 0, nullptr);
-  k_aggregate<<<rg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->U.p, th, ob2,
+  k_aggregate<<<qg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->U.p, th, ob2,
                                     net->h3.p, net->hb3.p, net->gdiag.p);
   k_gemm_rows<H, H2><<<gg, kBlock, 0, s>>>(n, net->hb3.p, net->wt.p + 2 * H * H2, net->P.p, 0,
                                            nullptr);
-  k_edge_head<<<rg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->nlow.p, net->grp.p,
+  k_edge_head<<<qg, kBlock, 0, s>>>(n, net->erp.p, net->eci.p, net->ev.p, net->nlow.p, net->grp.p,
                                     net->P.p, th, net->gdiag.p, net->s.p, net->ghat.p, gval);
   TBSLA_HIP_LAUNCH_CHECK();
   return TBSLA_HIP_SUCCESS;
