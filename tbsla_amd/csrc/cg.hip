@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -97,6 +98,8 @@ struct CgArgs {
   const int *block_seg;
   int verify;        // recurrence convergence waits for the true residual
   int stall_window;  // > 0: breakdown when |r| has not dropped below |r0| by then
+  int a_nnz;         // stored entries of A
+  int g_nnz;         // stored entries of G (and of G^T)
 };
 
 // Sums of a system's workgroup partials are taken in one fixed order:
@@ -164,6 +167,83 @@ __device__ inline double lane_sum(double v) {
   return v;
 }
 
+// Operands of the first row a thread handles, loaded before the reductions
+// they do not depend on.  Entries j = beg + lane + k L (k < P) of the row, in
+// the order the row loop visits them; the arithmetic on them is the loop's,
+// so results are bitwise those of the plain loop.
+template <int P, typename V>
+struct RowPrefetch {
+  int beg, end;
+  int col[P];
+  V v[P];
+  double s[P], t[P];
+};
+
+// A system's workgroup partials as loaded by seg_issue (raw, unmasked) and
+// summed by seg_finish in seg_sums' order.  Loads complete in issue order, so
+// K1 and K2 issue them between their row loads and the dependent gathers:
+// the reductions then wait for the partials only, and the gathers stay in
+// flight through the cross-lane and LDS work.
+constexpr int kPartLoads = kMaxBlocksPerSystem / kBlock;
+static_assert(kMaxBlocksPerSystem % kBlock == 0, "partials per thread");
+
+struct SegLoad {
+  double v[3][kPartLoads];
+};
+
+struct SegShared {
+  double part[3][kBlock / kWave];
+  int flag;
+};
+
+__device__ inline void seg_issue(const double *__restrict__ p0, const double *__restrict__ p1,
+                                 const double *__restrict__ p2, int np, int b0, int b1,
+                                 SegLoad &l) {
+#pragma unroll
+  for (int k = 0; k < kPartLoads; k++) {
+    const int i = min(b0 + (int)threadIdx.x + k * kBlock, b1 - 1);
+    l.v[0][k] = p0[i];
+    if (np > 1) l.v[1][k] = p1[i];
+    if (np > 2) l.v[2][k] = p2[i];
+  }
+}
+
+// The sums (results in every thread) and the system's done flag as one value
+// for the workgroup; the sums are valid only when the flag is kRunning.
+__device__ inline int seg_finish(int flag, const SegLoad &l, int np, int b0, int b1,
+                                 SegShared &sh, double *out) {
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kPartLoads; k++) {
+    if (b0 + (int)threadIdx.x + k * kBlock < b1) {
+      v[0] += l.v[0][k];
+      if (np > 1) v[1] += l.v[1][k];
+      if (np > 2) v[2] += l.v[2][k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    if (k < np) v[k] = group_sum<kWave>(v[k]);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (k < np) sh.part[k][threadIdx.x / kWave] = v[k];
+  if (threadIdx.x == 0) sh.flag = flag;
+  __syncthreads();
+  const int f = sh.flag;
+  if (f != kRunning) return f;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (k < np) {
+      double t = sh.part[k][0];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; w++) t += sh.part[k][w];
+      out[k] = t;
+    }
+  }
+  return f;
+}
+
 // K1.  check: apply the convergence test of the previous iteration first
 // (the count it++ of a passed test is made by K2, which no K1 workgroup
 // races with).  check_only: the final test after the last iteration.
@@ -171,16 +251,50 @@ template <int L, bool MIX, bool ZBUF>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int check,
                                                       int check_only) {
   TBSLA_NO_FMA
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int P = L == 1 ? 8 : 4;
+  constexpr int NP = ZBUF ? 3 : 2;
   __shared__ double lds[kBlock / kWave];
-  __shared__ int s_flag;
+  __shared__ SegShared sh;
   const int b = blockIdx.x;
-  const int sid = a.block_seg[b];
-  CgSeg *S = a.seg + sid;
-  if (seg_flag(S, &s_flag) != kRunning) return;
+  CgSeg *S = a.seg + a.block_seg[b];
   const int par = (par_g ^ S->shift) & 1;
-  const int b0 = S->blk0;
+  const int b0 = S->blk0, b1 = S->blk1;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const int lane = threadIdx.x & (L - 1);
+  const double *zv = ZBUF ? a.z : a.r[par];
+  const double *po = a.p[par];
+  const V *val = MIX ? (const V *)a.val32 : (const V *)a.val64;
+  const int row_first = r0 + (int)threadIdx.x / L;
+  const int rowc = min(row_first, r1 - 1);
+  // row extent and own entries, then the row's first entries (indices
+  // clamped into the arrays, which hold at least one entry; entries past the
+  // row end are never used), then the partials, then the gathers.  No
+  // branches between them: a join would make the partials' wait cover the
+  // gathers.
+  RowPrefetch<P, V> f;
+  f.beg = a.rowptr[rowc];
+  f.end = a.rowptr[rowc + 1];
+  const double own_z = zv[rowc], own_p = po[rowc];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const int j = min(f.beg + lane + k * L, a.a_nnz - 1);
+    f.col[k] = a.colidx[j];
+    f.v[k] = val[j];
+  }
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], NP, b0, b1, sl);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    f.s[k] = zv[f.col[k] - a.f_col];
+    f.t[k] = po[f.col[k] - a.f_col];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads above issued in this order
   double t[3];
-  seg_sums(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], ZBUF ? 3 : 2, b0, S->blk1, t);
+  if (seg_finish(flag, sl, NP, b0, b1, sh, t) != kRunning) return;
   const double rz_new = t[0], rz_old = t[1];
   const double rr = ZBUF ? t[2] : rz_new;
   if (check) {
@@ -206,43 +320,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
     if (fin != kRunning || check_only) return;
   }
   const double beta = rz_new / fmax(rz_old, S->near_zero);
-  const double *zv = ZBUF ? a.z : a.r[par];
-  const double *po = a.p[par];
   double *pn = a.p[par ^ 1];
-  const int r0 = S->row0 + (b - b0) * S->rpb;
-  const int r1 = min(S->row1, r0 + S->rpb);
-  const int lane = threadIdx.x & (L - 1);
   double pap = 0.0;
-  for (int row = r0 + (int)threadIdx.x / L; row < r1; row += kBlock / L) {
-    const int beg = a.rowptr[row], end = a.rowptr[row + 1];
-    if (MIX) {
-      float acc = 0.0f;
-      for (int j = beg + lane; j < end; j += L) {
-        const int c = a.colidx[j] - a.f_col;
-        const double v = zv[c] + beta * po[c];
-        acc = fmaf(a.val32[j], (float)v, acc);
+  for (int row = row_first; row < r1; row += kBlock / L) {
+    int j, end;
+    V acc = 0;
+    double own;
+    if (row == row_first) {
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        if (f.beg + lane + k * L < f.end) {
+          const double v = f.s[k] + beta * f.t[k];
+          if (MIX) acc = fmaf((float)f.v[k], (float)v, (float)acc);
+          else acc = fma((double)f.v[k], v, (double)acc);
+        }
       }
-      acc = lane_sum<L>(acc);
-      if (lane == 0) {
-        const double own = zv[row] + beta * po[row];
-        pn[row] = own;
-        a.ap[row] = (double)acc;
-        pap += own * (double)acc;
-      }
+      j = f.beg + lane + P * L;
+      end = f.end;
+      own = own_z + beta * own_p;
     } else {
-      double acc = 0.0;
-      for (int j = beg + lane; j < end; j += L) {
-        const int c = a.colidx[j] - a.f_col;
-        const double v = zv[c] + beta * po[c];
-        acc = fma(a.val64[j], v, acc);
-      }
-      acc = lane_sum<L>(acc);
-      if (lane == 0) {
-        const double own = zv[row] + beta * po[row];
-        pn[row] = own;
-        a.ap[row] = acc;
-        pap += own * acc;
-      }
+      j = a.rowptr[row] + lane;
+      end = a.rowptr[row + 1];
+      own = zv[row] + beta * po[row];
+    }
+    for (; j < end; j += L) {
+      const int c = a.colidx[j] - a.f_col;
+      const double v = zv[c] + beta * po[c];
+      if (MIX) acc = fmaf((float)val[j], (float)v, (float)acc);
+      else acc = fma((double)val[j], v, (double)acc);
+    }
+    acc = lane_sum<L>(acc);
+    if (lane == 0) {
+      pn[row] = own;
+      a.ap[row] = (double)acc;
+      pap += own * (double)acc;
     }
   }
   pap = wg_sum(pap, lds);
@@ -253,53 +364,97 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
 template <int PRE, bool MIX>
 __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int count) {
   TBSLA_NO_FMA
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int P = 4;
   __shared__ double lds[kBlock / kWave];
-  __shared__ int s_flag;
+  __shared__ SegShared sh;
   const int b = blockIdx.x;
-  const int sid = a.block_seg[b];
-  CgSeg *S = a.seg + sid;
-  if (seg_flag(S, &s_flag) != kRunning) return;
+  CgSeg *S = a.seg + a.block_seg[b];
   const int par = (par_g ^ S->shift) & 1;
-  const int b0 = S->blk0;
-  double t[3];
-  seg_sums(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, S->blk1, t);
-  const double rz = t[0], pap = t[1];
-  const double alpha = rz / fmax(pap, S->near_zero);
+  const int b0 = S->blk0, b1 = S->blk1;
   const double *rc = a.r[par];
   double *rn = a.r[par ^ 1];
   const double *pn = a.p[par ^ 1];
+  const V *tv = MIX ? (const V *)a.t_v32 : (const V *)a.t_v64;
   const int r0 = S->row0 + (b - b0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
+  const int row_first = r0 + (int)threadIdx.x;
+  const int rowc = min(row_first, r1 - 1);
+  // the first row's operands (see RowPrefetch), then the partials, then the
+  // gathers of t = G^T r'
+  const double fx = a.x[rowc], fp = pn[rowc], fr = rc[rowc], fap = a.ap[rowc];
+  const double fd = PRE == 1 ? a.dinv[rowc] : 0.0;
+  RowPrefetch<P, V> f;
+  f.beg = f.end = 0;
+  if (PRE == 2) {
+    f.beg = a.t_rp[rowc];
+    f.end = a.t_rp[rowc + 1];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      const int j = min(f.beg + k, a.g_nnz - 1);
+      f.col[k] = a.t_ci[j];
+      f.v[k] = tv[j];
+    }
+  }
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, b1, sl);
+  __builtin_amdgcn_sched_barrier(0);
+  if (PRE == 2) {
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      f.s[k] = rc[f.col[k] - a.f_col];
+      f.t[k] = a.ap[f.col[k] - a.f_col];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads above issued in this order
+  double t[3];
+  if (seg_finish(flag, sl, 2, b0, b1, sh, t) != kRunning) return;
+  const double rz = t[0], pap = t[1];
+  const double alpha = rz / fmax(pap, S->near_zero);
   double rr = 0.0, rzp = 0.0;
-  for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
-    a.x[row] = a.x[row] + alpha * pn[row];
-    const double ri = rc[row] + (-alpha) * a.ap[row];
+  for (int row = row_first; row < r1; row += kBlock) {
+    const bool first = row == row_first;
+    const double xv = first ? fx : a.x[row];
+    const double pv = first ? fp : pn[row];
+    const double rv = first ? fr : rc[row];
+    const double apv = first ? fap : a.ap[row];
+    a.x[row] = xv + alpha * pv;
+    const double ri = rv + (-alpha) * apv;
     rn[row] = ri;
     rr += ri * ri;
     if (PRE == 1) {
-      const double zi = ri * a.dinv[row];
+      const double zi = ri * (first ? fd : a.dinv[row]);
       a.z[row] = zi;
       rzp += ri * zi;
     }
     if (PRE == 2) {
       // t = G^T r' with r'_c = r_c - alpha Ap_c computed as above
-      if (MIX) {
-        float acc = 0.0f;
-        for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++) {
-          const int c = a.t_ci[j] - a.f_col;
-          const double rcv = rc[c] + (-alpha) * a.ap[c];
-          acc = fmaf(a.t_v32[j], (float)rcv, acc);
+      V acc = 0;
+      int j, end;
+      if (first) {
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          if (f.beg + k < f.end) {
+            const double rcv = f.s[k] + (-alpha) * f.t[k];
+            if (MIX) acc = fmaf((float)f.v[k], (float)rcv, (float)acc);
+            else acc = fma((double)f.v[k], rcv, (double)acc);
+          }
         }
-        a.t32[row] = acc;
+        j = f.beg + P;
+        end = f.end;
       } else {
-        double acc = 0.0;
-        for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++) {
-          const int c = a.t_ci[j] - a.f_col;
-          const double rcv = rc[c] + (-alpha) * a.ap[c];
-          acc = fma(a.t_v64[j], rcv, acc);
-        }
-        a.t64[row] = acc;
+        j = a.t_rp[row];
+        end = a.t_rp[row + 1];
       }
+      for (; j < end; j++) {
+        const int c = a.t_ci[j] - a.f_col;
+        const double rcv = rc[c] + (-alpha) * a.ap[c];
+        if (MIX) acc = fmaf((float)tv[j], (float)rcv, (float)acc);
+        else acc = fma((double)tv[j], rcv, (double)acc);
+      }
+      if (MIX) a.t32[row] = (float)acc;
+      else a.t64[row] = (double)acc;
     }
   }
   rr = wg_sum(rr, lds);
@@ -338,37 +493,76 @@ __device__ bool mode_select(const CgSeg *S, int mode, int par_g, int *s_flag, in
   return true;
 }
 
-// K3: z = G t; r.z partials.  Also the tail of a residual replacement.
+// K3: z = G t; r.z partials.  Also the tail of a residual replacement.  The
+// first row's entries of G and the values of t they pick are loaded before
+// the done flag is shared (as in K1 and K2).
 template <bool MIX>
 __global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode) {
   TBSLA_NO_FMA
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int P = 4;
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
-  const int sid = a.block_seg[b];
-  CgSeg *S = a.seg + sid;
-  int rsel;
-  if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
+  CgSeg *S = a.seg + a.block_seg[b];
+  const int rsel = mode == kAfterIter ? ((par_g ^ S->shift) & 1) ^ 1
+                   : mode == kVerify  ? S->stop_par
+                                      : 0;
   const double *rc = a.r[rsel];
+  const V *gv = MIX ? (const V *)a.g_v32 : (const V *)a.g_v64;
+  const V *tv = MIX ? (const V *)a.t32 : (const V *)a.t64;
   const int b0 = S->blk0;
   const int r0 = S->row0 + (b - b0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
+  const int row_first = r0 + (int)threadIdx.x;
+  const int rowc = min(row_first, r1 - 1);
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  RowPrefetch<P, V> f;
+  f.beg = a.g_rp[rowc];
+  f.end = a.g_rp[rowc + 1];
+  const double fr = rc[rowc];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const int j = min(f.beg + k, a.g_nnz - 1);
+    f.col[k] = a.g_ci[j];
+    f.v[k] = gv[j];
+  }
+  V tg[P];
+#pragma unroll
+  for (int k = 0; k < P; k++) tg[k] = tv[f.col[k] - a.f_col];
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads above issued in this order
+  if (threadIdx.x == 0) s_flag = flag;
+  __syncthreads();
+  const int fl = s_flag;
+  const bool part = mode == kAfterIter ? fl == kRunning
+                    : mode == kVerify  ? fl == kPending
+                                       : fl != kInactive;
+  if (!part) return;
   double rz = 0.0;
-  for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
-    double zi;
-    if (MIX) {
-      float acc = 0.0f;
-      for (int j = a.g_rp[row]; j < a.g_rp[row + 1]; j++)
-        acc = fmaf(a.g_v32[j], a.t32[a.g_ci[j] - a.f_col], acc);
-      zi = (double)acc;
+  for (int row = row_first; row < r1; row += kBlock) {
+    const bool first = row == row_first;
+    V acc = 0;
+    int j, end;
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < P; k++)
+        if (f.beg + k < f.end) {
+          if (MIX) acc = fmaf((float)f.v[k], (float)tg[k], (float)acc);
+          else acc = fma((double)f.v[k], (double)tg[k], (double)acc);
+        }
+      j = f.beg + P;
+      end = f.end;
     } else {
-      double acc = 0.0;
-      for (int j = a.g_rp[row]; j < a.g_rp[row + 1]; j++)
-        acc = fma(a.g_v64[j], a.t64[a.g_ci[j] - a.f_col], acc);
-      zi = acc;
+      j = a.g_rp[row];
+      end = a.g_rp[row + 1];
     }
+    for (; j < end; j++) {
+      if (MIX) acc = fmaf((float)gv[j], (float)tv[a.g_ci[j] - a.f_col], (float)acc);
+      else acc = fma((double)gv[j], (double)tv[a.g_ci[j] - a.f_col], (double)acc);
+    }
+    const double zi = (double)acc;
     a.z[row] = zi;
-    rz += rc[row] * zi;
+    rz += (first ? fr : rc[row]) * zi;
   }
   rz = wg_sum(rz, lds);
   if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
@@ -613,6 +807,10 @@ struct CgCache {
   Buf<CgCom> com;
 
   Buf<int> block_seg;
+  // stand-ins for A's or G^T's entry arrays when they hold no entry: one
+  // column index (the block's first column) and one zero value
+  Buf<int> dummy_i;
+  Buf<double> dummy_d;
   CgCom *h_com = nullptr;  // pinned status copies (2)
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipStream_t own = nullptr;  // used when the caller passes the null stream
@@ -650,6 +848,8 @@ struct CgCache {
     com.release();
 
     block_seg.release();
+    dummy_i.release();
+    dummy_d.release();
     if (h_com) hipHostFree(h_com);
     for (auto &e : ev)
       if (e) hipEventDestroy(e);
@@ -974,6 +1174,33 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
 
   a.verify = (mixed || o.replace_every > 0) ? 1 : 0;
   a.stall_window = o.stall_window;
+  // the prefetches of K1 and K2 read entry 0 of A's and G^T's arrays even
+  // when they hold none: a one-entry stand-in then (never used otherwise)
+  if (!c.dummy_i.p) {
+    TBSLA_TRY(c.dummy_i.ensure(1));
+    TBSLA_TRY(c.dummy_d.ensure(1));
+    const double zero = 0.0;
+    TBSLA_HIP_CHECK(hipMemcpy(c.dummy_d.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+  }
+  const int fcol = m->f_col;
+  TBSLA_HIP_CHECK(hipMemcpyAsync(c.dummy_i.p, &fcol, sizeof fcol, hipMemcpyHostToDevice, s));
+  a.a_nnz = (int)std::max(m->stored, 1LL);
+  if (m->stored == 0) {
+    a.colidx = c.dummy_i.p;
+    a.val64 = c.dummy_d.p;
+    a.val32 = (const float *)c.dummy_d.p;
+  }
+  if (pre == TBSLA_HIP_PRECOND_FACTORED) {
+    a.g_nnz = (int)std::max(G->stored, 1LL);
+    if (G->stored == 0) {
+      a.g_ci = c.dummy_i.p;
+      a.g_v64 = c.dummy_d.p;
+      a.g_v32 = (const float *)c.dummy_d.p;
+      a.t_ci = c.dummy_i.p;
+      a.t_v64 = c.dummy_d.p;
+      a.t_v32 = (const float *)c.dummy_d.p;
+    }
+  }
 
   Plan p;
   p.L = L;
