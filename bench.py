@@ -63,6 +63,14 @@ def measured_traffic(kernel: str):
     return int(float(fields["traffic_bytes_per_launch"]))
 
 
+def csr_kernel_name(k: dict, split: bool) -> str:
+    """The row kernel's name as rocprofv3 prints its template arguments
+    (k_csr_rows<vec, lanes, rows, nontemporal, split>)."""
+    name = "k_csr_rows<%d, %d, %d, %d, %s>" % (k["vec"], k["lanes"], k["rows"], k["nontemporal"],
+                                               "true" if split else "false")
+    return name + " + k_remote_products" if split else name
+
+
 def algorithmic_bytes_csr(stored, n_rows, n_cols):
     return 12 * stored + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
 
@@ -223,8 +231,7 @@ def main():
     gbs = nbytes / (per * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": ("k_csr_split<%d,%d> + k_remote_products" if layout["layout"] == T.CSR_SPLIT
-                       else "k_csr_spmv<%d,%d>") % m.get_csr_schedule(),
+            "kernel": csr_kernel_name(m.get_csr_kernel(), layout["layout"] == T.CSR_SPLIT),
             "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
     # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
@@ -237,7 +244,7 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
-    sched = list(m.get_csr_schedule())
+    sched = m.get_csr_kernel()
     del m, dx, dy
     cg = None if args.no_cg else cg_section(args)
     extra = {}

@@ -328,6 +328,17 @@ int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double beta, double e
  * (1, 2 or 4); 0 restores the choice from the mean row length. */
 int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows);
 int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows);
+/* The full CSR row-kernel choice (no reference counterpart): the schedule
+ * above plus `vec` entries per lane load (1, 2 or 4; loads of vec column
+ * indices and vec values, aligned) and `nontemporal` (1: entry loads do not
+ * keep their lines in L2).  Compiled kernels: vec 1 with lanes 1..64 and
+ * rows 1, 2, 4; vec 2 with lanes 4..32 and vec 4 with lanes 4..16, both with
+ * rows 1, 2.  tbsla_hip_set_csr_schedule resets vec to 1 and nontemporal to
+ * 1. */
+int tbsla_hip_set_csr_kernel(tbsla_hip_matrix_t m, int lanes, int rows, int vec,
+                             int nontemporal);
+int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *vec,
+                             int *nontemporal);
 /* CSR / ELL execution layout (no counterpart in the reference; results
  * within rounding of the row-major sum; ELL padding slots count as entries).  TBSLA_HIP_CSR_ROWS: the CSR arrays as
  * they are.  TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in
@@ -341,11 +352,12 @@ int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int s
                              void *stream);
 int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int *slices,
                              long long *remote_entries);
-/* Autotuner (no reference counterpart): times every CSR schedule -- lanes
- * per row 1..64 x rows per lane group 1, 2, 4, on the row layout and on the
- * split layout (window 64, 8 and 16 slices) -- on this block with d_v / d_r
- * (`iters` launches each) and keeps the fastest; other formats have a single
- * schedule.  *best_us receives its time per SpMV. */
+/* Autotuner (no reference counterpart): times every compiled CSR row
+ * kernel (tbsla_hip_set_csr_kernel: lanes x rows x vec x nontemporal) on the
+ * row layout and on the split layout (window 64, 8 and 16 slices) on this
+ * block with d_v / d_r (`iters` launches each) and keeps the fastest; ELL
+ * compares its column-major kernel with the split layout; other formats
+ * have a single schedule.  *best_us receives its time per SpMV. */
 int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                        float *best_us, void *stream);
 

@@ -119,6 +119,8 @@ SIGNATURES = [
                                                    _ip, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_set_csr_schedule", C.c_int, [_vp, C.c_int, C.c_int]),
     ("tbsla_hip_get_csr_schedule", C.c_int, [_vp, _ip, _ip]),
+    ("tbsla_hip_set_csr_kernel", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("tbsla_hip_get_csr_kernel", C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     ("tbsla_hip_autotune", C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_float), _vp]),
 ]
 
@@ -379,6 +381,14 @@ class Matrix:
         a, b = C.c_int(), C.c_int()
         check(lib().tbsla_hip_get_csr_schedule(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_csr_kernel(self, lanes: int, rows: int, vec: int = 1, nontemporal: int = 1) -> None:
+        check(lib().tbsla_hip_set_csr_kernel(self.h, lanes, rows, vec, nontemporal))
+
+    def get_csr_kernel(self) -> dict:
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().tbsla_hip_get_csr_kernel(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"lanes": a.value, "rows": b.value, "vec": c.value, "nontemporal": d.value}
 
     def autotune(self, dr: DeviceBuffer, dv: DeviceBuffer, iters: int = 10,
                  stream=None) -> float:

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "csr_rows.hpp"
 #include "kernels.hpp"
 
 namespace tbsla_hip {
@@ -441,6 +442,8 @@ int tbsla_hip_set_csr_schedule(tbsla_hip_matrix_t m, int lanes, int rows) {
     return fail(TBSLA_HIP_ERR_INVALID, "rows must be 0, 1, 2 or 4");
   m->lanes_per_row = lanes;
   m->rows_per_group = rows;
+  m->csr_vec = 0;
+  m->csr_nt = -1;
   return TBSLA_HIP_SUCCESS;
 }
 
@@ -449,6 +452,31 @@ int tbsla_hip_get_csr_schedule(tbsla_hip_matrix_t m, int *lanes, int *rows) {
   int L = pick_csr_lanes(m);
   if (lanes) *lanes = L;
   if (rows) *rows = pick_csr_rows(m, L);
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_set_csr_kernel(tbsla_hip_matrix_t m, int lanes, int rows, int vec,
+                             int nontemporal) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  if (!csr_rows_valid(vec, lanes, rows, nontemporal))
+    return fail(TBSLA_HIP_ERR_INVALID,
+                "no CSR kernel for this (lanes, rows, vec, nontemporal): vec 1 takes lanes 1..64 "
+                "and rows 1, 2, 4; vec 2 lanes 4..32, vec 4 lanes 4..16, both rows 1, 2");
+  m->lanes_per_row = lanes;
+  m->rows_per_group = rows;
+  m->csr_vec = vec;
+  m->csr_nt = nontemporal;
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *vec,
+                             int *nontemporal) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  const int L = pick_csr_lanes(m);
+  if (lanes) *lanes = L;
+  if (rows) *rows = pick_csr_rows(m, L);
+  if (vec) *vec = pick_csr_vec(m);
+  if (nontemporal) *nontemporal = pick_csr_nt(m);
   return TBSLA_HIP_SUCCESS;
 }
 
@@ -487,18 +515,23 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
   // split layout with 8 and 16 column slices (when the block has entries
   // far from its diagonal)
   struct Best {
-    int layout = -1, window = 0, slices = 0, L = 0, R = 0;
+    int layout = -1, window = 0, slices = 0, L = 0, R = 0, V = 1, NT = 1;
     float ms = 0;
   } best;
   auto sweep = [&](int layout, int window, int slices) -> int {
-    for (int L = 1; L <= 64; L <<= 1)
-      for (int R : {1, 2, 4}) {
-        m->lanes_per_row = L;
-        m->rows_per_group = R;
-        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
-        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
-        if (best.layout < 0 || ms < best.ms) best = Best{layout, window, slices, L, R, ms};
-      }
+    for (int V : {1, 2, 4})
+      for (int L = 1; L <= 64; L <<= 1)
+        for (int R : {1, 2, 4})
+          for (int NT : {1, 0}) {
+            if (!csr_rows_valid(V, L, R, NT)) continue;
+            m->lanes_per_row = L;
+            m->rows_per_group = R;
+            m->csr_vec = V;
+            m->csr_nt = NT;
+            TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
+            TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+            if (best.layout < 0 || ms < best.ms) best = Best{layout, window, slices, L, R, V, NT, ms};
+          }
     return TBSLA_HIP_SUCCESS;
   };
   split_release(m);
@@ -516,6 +549,8 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
   if (best.layout == TBSLA_HIP_CSR_SPLIT) TBSLA_TRY(split_build(m, best.window, best.slices, s));
   m->lanes_per_row = best.L;
   m->rows_per_group = best.R;
+  m->csr_vec = best.V;
+  m->csr_nt = best.NT;
   if (best_us) *best_us = best.ms * 1000.0f / iters;
   return TBSLA_HIP_SUCCESS;
 }

@@ -792,7 +792,7 @@ struct Buf {
 struct GraphKey {
   CgArgs args;
   const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
-  int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows;
+  int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
 };
 
@@ -1280,6 +1280,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.nblocks = nblocks;
   key.spmv_lanes = pick_csr_lanes(m);
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
+  key.spmv_vec = pick_csr_vec(m);
+  key.spmv_nt = pick_csr_nt(m);
   if (B > 0 && !(c.exec[0] && c.key == key)) {
     for (auto &x : c.exec)
       if (x) {

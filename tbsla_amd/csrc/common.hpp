@@ -203,6 +203,8 @@ struct tbsla_hip_matrix_s {
   // SpMV schedule
   int lanes_per_row = 0;     // CSR: 0 = pick from the mean row length
   int rows_per_group = 0;    // CSR: rows each lane group keeps in flight (0 = pick)
+  int csr_vec = 0;           // CSR: entries per lane load (0 = 1)
+  int csr_nt = -1;           // CSR: non-temporal loads of the entries (-1 = on)
   // scratch owned by the record (reused across calls; grows on demand)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -19,6 +19,8 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
                 bool zero_first, hipStream_t s, const int *d_done = nullptr);
 int pick_csr_lanes(const tbsla_hip_matrix_s *m);
 int pick_csr_rows(const tbsla_hip_matrix_s *m, int lanes);
+int pick_csr_vec(const tbsla_hip_matrix_s *m);
+int pick_csr_nt(const tbsla_hip_matrix_s *m);
 
 // vec.hip
 int launch_fill(double *d, long long n, double v, hipStream_t s);

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "csr_rows.hpp"
 #include "kernels.hpp"
 
 namespace tbsla_hip {
@@ -153,71 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_remote_products(const long long *__r
   for (long long k = b + threadIdx.x; k < e; k += kBlock) P[k] = ldg_stream(sv + k) * x[ldg_stream(sc + k)];
 }
 
-// the row kernel: near-diagonal entries as k_csr_spmv, then the row's
-// remote products (lanes of the group take them in turn; the group sum is
-// a fixed tree)
-template <int L, int R>
-__global__ __launch_bounds__(kBlock) void k_csr_split(long long n_rows, const int *__restrict__ lrp,
-                                                      const int *__restrict__ lci,
-                                                      const double *__restrict__ lv,
-                                                      const int *__restrict__ rptr,
-                                                      const int *__restrict__ rpos,
-                                                      const double *__restrict__ P,
-                                                      const double *__restrict__ x,
-                                                      double *__restrict__ y,
-                                                      const int *__restrict__ done) {
-  if (done && *done) return;
-  constexpr int G = kWave / L;
-  const int lane = threadIdx.x & (L - 1);
-  const int g = (threadIdx.x & (kWave - 1)) / L;
-  const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
-  const long long base = wave * (G * R);
-  if (base >= n_rows) return;
-  int beg[R], end[R], rb[R], re[R];
-  int len = 0;
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const long long row = base + r * G + g;
-    if (row < n_rows) {
-      beg[r] = lrp[row];
-      end[r] = lrp[row + 1];
-      rb[r] = rptr[row];
-      re[r] = rptr[row + 1];
-    } else {
-      beg[r] = end[r] = rb[r] = re[r] = 0;
-    }
-    len = max(len, end[r] - beg[r]);
-  }
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, kWave));
-  const int trips = (len + L - 1) / L;
-  double sum[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) sum[r] = 0.0;
-  for (int t = 0; t < trips; t++) {
-    int c[R];
-    double v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int j = beg[r] + t * L + lane;
-      const bool in = j < end[r];
-      c[r] = in ? ldg_stream(lci + j) : 0;
-      v[r] = in ? ldg_stream(lv + j) : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) sum[r] = fma(v[r], x[c[r]], sum[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < R; r++)
-    for (int j = rb[r] + lane; j < re[r]; j += L) sum[r] += P[rpos[j]];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const double s = group_sum<L>(sum[r]);
-    const long long row = base + r * G + g;
-    if (lane == 0 && row < n_rows) y[row] = s;
-  }
-}
-
 unsigned sgrid(long long n) {
   long long g = (n + kBlock - 1) / kBlock;
   return (unsigned)std::max<long long>(1, std::min<long long>(g, 65536));
@@ -227,25 +163,6 @@ int bits_of(long long v) {
   int b = 1;
   while ((1LL << b) <= v) b++;
   return b;
-}
-
-template <int L, int R>
-void launch_split_lr(const tbsla_hip_matrix_s *m, const SplitCsr *sp, double *y, const double *x,
-                     const int *done, hipStream_t s) {
-  const long long rows_per_wave = (long long)(kWave / L) * R;
-  const long long waves = (m->ln_row + rows_per_wave - 1) / rows_per_wave;
-  k_csr_split<L, R><<<grid_for(waves * kWave), kBlock, 0, s>>>(
-      m->ln_row, sp->lrp, sp->lci, sp->lv, sp->rptr, sp->rpos, sp->P, x, y, done);
-}
-
-template <int L>
-void launch_split_l(const tbsla_hip_matrix_s *m, const SplitCsr *sp, int R, double *y,
-                    const double *x, const int *done, hipStream_t s) {
-  switch (R) {
-    case 1: launch_split_lr<L, 1>(m, sp, y, x, done, s); break;
-    case 2: launch_split_lr<L, 2>(m, sp, y, x, done, s); break;
-    default: launch_split_lr<L, 4>(m, sp, y, x, done, s); break;
-  }
 }
 
 }  // namespace
@@ -280,16 +197,11 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
                                                                   sp->sv, x, sp->P, done);
     }
   }
-  const int L = pick_csr_lanes(m), R = pick_csr_rows(m, L);
-  switch (L) {
-    case 1: launch_split_l<1>(m, sp, R, y, x, done, s); break;
-    case 2: launch_split_l<2>(m, sp, R, y, x, done, s); break;
-    case 4: launch_split_l<4>(m, sp, R, y, x, done, s); break;
-    case 8: launch_split_l<8>(m, sp, R, y, x, done, s); break;
-    case 16: launch_split_l<16>(m, sp, R, y, x, done, s); break;
-    case 32: launch_split_l<32>(m, sp, R, y, x, done, s); break;
-    default: launch_split_l<64>(m, sp, R, y, x, done, s); break;
-  }
+  // the near-diagonal entries with the block's CSR schedule, then each
+  // row's remote products (lanes of the group take them in turn)
+  CsrRowsArgs a{m->ln_row, sp->lrp, sp->lci, sp->lv, x, 0, y, done, sp->rptr, sp->rpos, sp->P};
+  const int L = pick_csr_lanes(m);
+  csr_rows_launch<true>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
   TBSLA_HIP_LAUNCH_CHECK();
   return TBSLA_HIP_SUCCESS;
 }

@@ -2,8 +2,8 @@
 //
 // All kernels are HBM-bound streams over the matrix arrays with gathers of x
 // (which stays in L2 / the 256 MB MALL):
-//   CSR    L lanes per row (L = 1..64), R rows per lane group in flight;
-//          loads of values/colidx coalesced and non-temporal; shuffle
+//   CSR    L lanes per row, R rows per lane group in flight, V entries per
+//          lane load, optional non-temporal loads (csr_rows.hpp); shuffle
 //          reduction.
 //   ELL    one lane per row over column-major slots: every slot load is a
 //          contiguous 64-row stripe.
@@ -18,69 +18,12 @@
 #include <climits>
 
 #include "common.hpp"
+#include "csr_rows.hpp"
 #include "kernels.hpp"
 
 namespace tbsla_hip {
 
 namespace {
-
-// L lanes per row, R rows per lane group: a wavefront owns 64/L * R
-// consecutive rows; at step r its groups take the next 64/L rows, so every
-// step's loads cover one contiguous stretch of colidx/values, and the R steps'
-// loads are all in flight before the first x gather is consumed.
-template <int L, int R>
-__global__ __launch_bounds__(kBlock) void k_csr_spmv(long long n_rows, const int *__restrict__ rowptr,
-                                                     const int *__restrict__ colidx,
-                                                     const double *__restrict__ vals,
-                                                     const double *__restrict__ x, int f_col,
-                                                     double *__restrict__ y,
-                                                     const int *__restrict__ done) {
-  if (done && *done) return;
-  constexpr int G = kWave / L;
-  const int lane = threadIdx.x & (L - 1);
-  const int g = (threadIdx.x & (kWave - 1)) / L;
-  const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
-  const long long base = wave * (G * R);
-  if (base >= n_rows) return;
-  int beg[R], end[R];
-  int len = 0;
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const long long row = base + r * G + g;
-    if (row < n_rows) {
-      beg[r] = rowptr[row];
-      end[r] = rowptr[row + 1];
-    } else {
-      beg[r] = end[r] = 0;
-    }
-    len = max(len, end[r] - beg[r]);
-  }
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, kWave));
-  const int trips = (len + L - 1) / L;
-  double sum[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) sum[r] = 0.0;
-  for (int t = 0; t < trips; t++) {
-    int c[R];
-    double v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int j = beg[r] + t * L + lane;
-      const bool in = j < end[r];
-      c[r] = in ? ldg_stream(colidx + j) - f_col : 0;
-      v[r] = in ? ldg_stream(vals + j) : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) sum[r] = fma(v[r], x[c[r]], sum[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const double s = group_sum<L>(sum[r]);
-    const long long row = base + r * G + g;
-    if (lane == 0 && row < n_rows) y[row] = s;
-  }
-}
 
 // MatrixELL::Ax (MatrixELL.cpp:181-205): a negative local column (padding
 // of a block with f_col > 0) reads x[0], multiplied by the zero padding.
@@ -266,25 +209,6 @@ unsigned stride_grid(long long n) {
   return (unsigned)g;
 }
 
-template <int L, int R>
-void launch_csr_lr(const tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
-                   hipStream_t s) {
-  const long long rows_per_wave = (long long)(kWave / L) * R;
-  const long long waves = (m->ln_row + rows_per_wave - 1) / rows_per_wave;
-  k_csr_spmv<L, R><<<grid_for(waves * kWave), kBlock, 0, s>>>(m->ln_row, m->rowptr, m->colidx,
-                                                               m->values, x, m->f_col, y, done);
-}
-
-template <int L>
-void launch_csr(const tbsla_hip_matrix_s *m, int R, double *y, const double *x, const int *done,
-                hipStream_t s) {
-  switch (R) {
-    case 1: launch_csr_lr<L, 1>(m, y, x, done, s); break;
-    case 2: launch_csr_lr<L, 2>(m, y, x, done, s); break;
-    default: launch_csr_lr<L, 4>(m, y, x, done, s); break;
-  }
-}
-
 }  // namespace
 
 int launch_fill(double *d, long long n, double v, hipStream_t s) {
@@ -316,6 +240,11 @@ int pick_csr_rows(const tbsla_hip_matrix_s *m, int L) {
   return L >= 64 ? 1 : (L >= 32 ? 2 : 4);
 }
 
+// entries per lane load (1 unless set) and the non-temporal load hint (on
+// unless set)
+int pick_csr_vec(const tbsla_hip_matrix_s *m) { return m->csr_vec > 0 ? m->csr_vec : 1; }
+int pick_csr_nt(const tbsla_hip_matrix_s *m) { return m->csr_nt >= 0 ? m->csr_nt : 1; }
+
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
                 bool zero_first, hipStream_t s, const int *d_done) {
   switch (m->format) {
@@ -326,16 +255,10 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         return TBSLA_HIP_SUCCESS;
       }
       if (m->split) return split_launch(m, d_r, d_v, d_done, s);
-      const int L = pick_csr_lanes(m), R = pick_csr_rows(m, L);
-      switch (L) {
-        case 1: launch_csr<1>(m, R, d_r, d_v, d_done, s); break;
-        case 2: launch_csr<2>(m, R, d_r, d_v, d_done, s); break;
-        case 4: launch_csr<4>(m, R, d_r, d_v, d_done, s); break;
-        case 8: launch_csr<8>(m, R, d_r, d_v, d_done, s); break;
-        case 16: launch_csr<16>(m, R, d_r, d_v, d_done, s); break;
-        case 32: launch_csr<32>(m, R, d_r, d_v, d_done, s); break;
-        default: launch_csr<64>(m, R, d_r, d_v, d_done, s); break;
-      }
+      CsrRowsArgs a{m->ln_row, m->rowptr, m->colidx, m->values, d_v, m->f_col, d_r, d_done,
+                    nullptr, nullptr, nullptr};
+      const int L = pick_csr_lanes(m);
+      csr_rows_launch<false>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
       TBSLA_HIP_LAUNCH_CHECK();
       return TBSLA_HIP_SUCCESS;
     }

@@ -55,6 +55,16 @@ def test_status_strings_and_bad_arguments():
     assert L.tbsla_hip_set_csr_schedule(h, 3, 0) == 1
     assert L.tbsla_hip_set_csr_schedule(h, 16, 3) == 1
     assert L.tbsla_hip_set_csr_schedule(h, 16, 4) == 0
+    # vectorised CSR kernels: only the compiled (lanes, rows, vec, nt) sets
+    for bad in [(32, 1, 4, 0), (16, 4, 2, 0), (8, 2, 3, 0), (8, 2, 2, 2), (2, 1, 2, 1), (64, 1, 2, 0)]:
+        assert L.tbsla_hip_set_csr_kernel(h, *bad) == 1, bad
+    assert L.tbsla_hip_set_csr_kernel(h, 8, 2, 4, 0) == 0
+    got = [ctypes.c_int() for _ in range(4)]
+    assert L.tbsla_hip_get_csr_kernel(h, *[ctypes.byref(v) for v in got]) == 0
+    assert [v.value for v in got] == [8, 2, 4, 0]
+    assert L.tbsla_hip_set_csr_schedule(h, 16, 4) == 0
+    assert L.tbsla_hip_get_csr_kernel(h, *[ctypes.byref(v) for v in got]) == 0
+    assert [v.value for v in got] == [16, 4, 1, 1]
     assert L.tbsla_hip_fill_cqmat(h, 10, 10, -1, 0.1, 1, 0, 0, 1, 1, -1, None) == 1
     assert L.tbsla_hip_fill_cqmat(h, 10, 10, 3, 0.1, 1, 2, 0, 1, 1, -1, None) == 1
     assert L.tbsla_hip_destroy(h) == 0

@@ -182,6 +182,45 @@ def test_csr_schedules_agree(lanes):
         close(irr.spmv(x[:3000]), oirr.spmv(x[:3000]))
 
 
+VEC_KERNELS = ([(1, L, R, nt) for L in (1, 4, 16, 64) for R in (1, 4) for nt in (0, 1)]
+               + [(2, L, R, nt) for L in (4, 8, 16, 32) for R in (1, 2) for nt in (0, 1)]
+               + [(4, L, R, nt) for L in (4, 8, 16) for R in (1, 2) for nt in (0, 1)])
+
+
+@pytest.mark.parametrize("vec,lanes,rows,nt", VEC_KERNELS)
+def test_csr_vector_kernels_agree(vec, lanes, rows, nt):
+    """Every compiled row kernel (entries per lane load, lanes, rows,
+    non-temporal hint) stays within 1e-12 of the oracle: regular cqmat rows,
+    rows of very different lengths (starts not aligned to the vector width),
+    a block with f_col > 0, and the split layout."""
+    n = 6000
+    o = O.fill_cqmat(O.CSR, n, n, 40, 0.2, 0, value_seed=2)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, 40, 0.2, 0, value_seed=2)
+    x = O.unit_vector(1, n)
+    g.set_csr_kernel(lanes, rows, vec, nt)
+    assert g.get_csr_kernel() == {"lanes": lanes, "rows": rows, "vec": vec, "nontemporal": nt}
+    y = g.spmv(x)
+    close(y, o.spmv(x))
+    assert np.array_equal(y, g.spmv(x))
+    g.set_csr_layout(T.CSR_SPLIT, 64, 8)
+    close(g.spmv(x), o.spmv(x))
+    rng = np.random.default_rng(vec * 100 + lanes)
+    lens = rng.integers(0, 97, 1500)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = rng.integers(0, 1500, rp[-1]).astype(np.int32)
+    vals = rng.random(rp[-1])
+    geom = dict(n_row=1500, n_col=1500, f_row=0, f_col=0, ln_row=1500, ln_col=1500, pr=0, pc=0,
+                NR=1, NC=1, nnz=int(rp[-1]))
+    irr = T.Matrix.from_arrays(T.CSR, geom, rp, cols, vals)
+    irr.set_csr_kernel(lanes, rows, vec, nt)
+    close(irr.spmv(x[:1500]), O.csr_from_arrays(1500, 1500, rp, cols, vals).spmv(x[:1500]))
+    ob = O.fill_cqmat(O.CSR, n, n, 12, 0.3, 0, 1, 1, 2, 2, value_seed=4)
+    gb = T.Matrix(T.CSR).fill_cqmat(n, n, 12, 0.3, 0, 1, 1, 2, 2, value_seed=4)
+    gb.set_csr_kernel(lanes, rows, vec, nt)
+    xb = O.unit_vector(5, ob.geom["ln_col"])
+    close(gb.spmv(xb), ob.spmv(xb))
+
+
 def test_autotune_keeps_results():
     n = 50000
     o = O.fill_cqmat(O.CSR, n, n, 24, 0.1, 0, value_seed=2)
@@ -191,6 +230,8 @@ def test_autotune_keeps_results():
     us = g.autotune(dy, dx, 3)
     L, R = g.get_csr_schedule()
     assert us > 0 and L in (1, 2, 4, 8, 16, 32, 64) and R in (1, 2, 4)
+    k = g.get_csr_kernel()
+    assert k["vec"] in (1, 2, 4) and k["nontemporal"] in (0, 1)
     close(g.spmv(x), o.spmv(x))
 
 
