@@ -8,6 +8,7 @@
 // order with separate multiply and add (no FMA), so C is bit-identical to
 // the reference's loop (src/mpi/matrix/MatrixCSR.cpp:11-30).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -60,10 +61,61 @@ struct PprState {
   double max_err;
 };
 
+// Row slots: a workgroup holds kPprSlots rows at a time (kPprLanes lanes
+// per row, each lane owning 4 of the 64 columns).  Workgroup b runs on XCD
+// b mod 8 (workgroups are dispatched round-robin over the eight XCDs): XCD x
+// takes the x-th contiguous eighth of the rows and its slots take those rows
+// in turn (slot v of the XCD's W slots: rows v, v + W, ...), so at any
+// moment an XCD works on a window of about W consecutive rows, whose
+// near-diagonal X rows stay in its L2.  Other grid sizes: every slot of the
+// grid in turn.
+constexpr int kPprLanes = 16;                  // lanes per row
+constexpr int kPprSlots = kBlock / kPprLanes;  // rows in flight per workgroup
+constexpr int kPprDepth = 8;                   // entries whose X rows load together
+constexpr int kPprGrid = 2048;                 // workgroups of a step
+
+struct RowWalk {
+  long long first, stride, end;
+};
+__device__ inline RowWalk xcd_walk(long long n_rows, int nb, int slot) {
+  const int b = blockIdx.x;
+  if (nb % 8 != 0) return RowWalk{(long long)b * kPprSlots + slot, (long long)nb * kPprSlots, n_rows};
+  const long long per = (n_rows + 7) / 8;
+  const long long x0 = min(n_rows, (long long)(b % 8) * per);
+  return RowWalk{x0 + (long long)(b / 8) * kPprSlots + slot, (long long)(nb / 8) * kPprSlots,
+                 min(n_rows, x0 + per)};
+}
+
+constexpr int kPprCols = kWave / kPprLanes;  // columns per lane
+
+// kPprCols consecutive doubles (aligned to their size) in one load / store
+template <int N>
+struct DVec {
+  typedef double type __attribute__((ext_vector_type(N)));
+};
+template <>
+struct DVec<1> {
+  typedef double type;
+};
+typedef DVec<kPprCols>::type pprvec_t;
+template <typename VT>
+__device__ inline double vget(const VT &v, int q) {
+  if constexpr (std::is_same<VT, double>::value) return v;
+  else return v[q];
+}
+template <typename VT>
+__device__ inline void vset(VT &v, int q, double x) {
+  if constexpr (std::is_same<VT, double>::value) v = x;
+  else v[q] = x;
+}
+
 // One personalised-PageRank step for k = 64 columns fused with the SpMM:
 // Y[i,:] = A[i,:] X; X'[i,k] = (beta Y[i,k] + S_k (1-beta) [i == src_k]) / S_k;
 // per-block partials of |X' - X| and of X' per column.  The page_rank update
 // (src/mpi/matrix/Matrix.cpp:123-141) with a one-hot teleport per column.
+// Each row's entries are read kPprDepth at a time and their X rows loaded
+// before any is added; products are added in storage order with separate
+// multiply and add (the reference's arithmetic).
 __global__ __launch_bounds__(kBlock) void k_ppr_step(long long n_rows, int f_col, int ln_col,
                                                      const int *__restrict__ rowptr,
                                                      const int *__restrict__ colidx,
@@ -77,64 +129,109 @@ __global__ __launch_bounds__(kBlock) void k_ppr_step(long long n_rows, int f_col
                                                      double *__restrict__ p_sum) {
   TBSLA_NO_FMA
   if (st->done) return;
-  __shared__ double lds_e[kBlock / kWave][kWave];
-  __shared__ double lds_s[kBlock / kWave][kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const double Sk = S[lane];
-  const int sk = src[lane];
-  const double tel = Sk * (1 - beta);
-  double e = 0.0, sm = 0.0;
-  const long long waves = (long long)gridDim.x * (kBlock / kWave);
-  for (long long row = (long long)blockIdx.x * (kBlock / kWave) + wave; row < n_rows;
-       row += waves) {
-    const int r = __builtin_amdgcn_readfirstlane((int)row);
-    double acc = 0.0;
+  __shared__ double lds_e[kPprSlots][kWave];
+  __shared__ double lds_s[kPprSlots][kWave];
+  const int lane = threadIdx.x & (kPprLanes - 1);
+  const int slot = threadIdx.x / kPprLanes;
+  const int c0 = kPprCols * lane;  // this lane's columns c0..c0+kPprCols-1
+  double e[kPprCols], sm[kPprCols];
+#pragma unroll
+  for (int q = 0; q < kPprCols; q++) e[q] = sm[q] = 0.0;
+  const RowWalk wk = xcd_walk(n_rows, gridDim.x, slot);
+  for (long long row = wk.first; row < wk.end; row += wk.stride) {
+    const int r = (int)row;
+    double acc[kPprCols];
+#pragma unroll
+    for (int q = 0; q < kPprCols; q++) acc[q] = 0.0;
     const int beg = rowptr[r], end = rowptr[r + 1];
-    for (int j = beg; j < end; j++) {
-      const int c = colidx[j] - f_col;
-      if (c < 0 || c >= ln_col) continue;
-      acc = acc + vals[j] * X[(size_t)c * kWave + lane];
+    for (int j = beg; j < end; j += kPprDepth) {
+      int c[kPprDepth];
+      double v[kPprDepth];
+      pprvec_t xb[kPprDepth];
+#pragma unroll
+      for (int u = 0; u < kPprDepth; u++) {
+        const int jj = min(j + u, end - 1);
+        c[u] = colidx[jj] - f_col;
+        v[u] = vals[jj];
+      }
+#pragma unroll
+      for (int u = 0; u < kPprDepth; u++) {
+        const bool ok = c[u] >= 0 && c[u] < ln_col;  // block filter (MatrixCSR.cpp:19)
+        xb[u] = *reinterpret_cast<const pprvec_t *>(X + (size_t)(ok ? c[u] : 0) * kWave + c0);
+      }
+#pragma unroll
+      for (int u = 0; u < kPprDepth; u++)
+        if (j + u < end && c[u] >= 0 && c[u] < ln_col)
+#pragma unroll
+          for (int q = 0; q < kPprCols; q++) acc[q] = acc[q] + v[u] * vget(xb[u], q);
     }
-    double v = beta * acc + ((r + f_col == sk) ? tel : 0.0);
-    v = v / Sk;
-    const size_t o = (size_t)r * kWave + lane;
-    e += fabs(v - X[o]);
-    sm += v;
-    Xn[o] = v;
+    const size_t o = (size_t)r * kWave + c0;
+    const pprvec_t old = *reinterpret_cast<const pprvec_t *>(X + o);
+    pprvec_t nv;
+#pragma unroll
+    for (int q = 0; q < kPprCols; q++) {
+      const double Sk = S[c0 + q];
+      double t = beta * acc[q] + ((r + f_col == src[c0 + q]) ? Sk * (1 - beta) : 0.0);
+      t = t / Sk;
+      e[q] += fabs(t - vget(old, q));
+      sm[q] += t;
+      vset(nv, q, t);
+    }
+    *reinterpret_cast<pprvec_t *>(Xn + o) = nv;
   }
-  lds_e[wave][lane] = e;
-  lds_s[wave][lane] = sm;
+#pragma unroll
+  for (int q = 0; q < kPprCols; q++) {
+    lds_e[slot][c0 + q] = e[q];
+    lds_s[slot][c0 + q] = sm[q];
+  }
   __syncthreads();
-  if (wave == 0) {
+  if (threadIdx.x < kWave) {
+    const int col = threadIdx.x;
     double te = 0.0, ts = 0.0;
-    for (int w = 0; w < kBlock / kWave; w++) {
-      te += lds_e[w][lane];
-      ts += lds_s[w][lane];
+    for (int w = 0; w < kPprSlots; w++) {
+      te += lds_e[w][col];
+      ts += lds_s[w][col];
     }
-    p_err[(size_t)blockIdx.x * kWave + lane] = te;
-    p_sum[(size_t)blockIdx.x * kWave + lane] = ts;
+    p_err[(size_t)blockIdx.x * kWave + col] = te;
+    p_sum[(size_t)blockIdx.x * kWave + col] = ts;
   }
 }
 
-// Column totals in block order; converged when every column's change is
+// Sum over nb per-block rows of 64 partials, in block order within each of
+// kBlock / kWave quarters, the quarters then in order (a fixed order);
+// result for column threadIdx.x (< 64) in thread threadIdx.x.
+__device__ inline double block_order_sum(const double *__restrict__ part, int nb, double *lds) {
+  const int col = threadIdx.x & (kWave - 1), qt = threadIdx.x / kWave;
+  constexpr int kQ = kBlock / kWave;
+  const int per = (nb + kQ - 1) / kQ;
+  const int b0 = min(nb, qt * per), b1 = min(nb, b0 + per);
+  double t = 0.0;
+#pragma unroll 8
+  for (int b = b0; b < b1; b++) t += part[(size_t)b * kWave + col];
+  lds[qt * kWave + col] = t;
+  __syncthreads();
+  double tot = 0.0;
+  if (qt == 0)
+    for (int k = 0; k < kQ; k++) tot += lds[k * kWave + col];
+  return tot;
+}
+
+// Column totals in a fixed order; converged when every column's change is
 // below epsilon.
-__global__ __launch_bounds__(kWave) void k_ppr_finalize(PprState *__restrict__ st,
-                                                        const double *__restrict__ p_err,
-                                                        const double *__restrict__ p_sum, int nb,
-                                                        double *__restrict__ S, double epsilon) {
+__global__ __launch_bounds__(kBlock) void k_ppr_finalize(PprState *__restrict__ st,
+                                                         const double *__restrict__ p_err,
+                                                         const double *__restrict__ p_sum, int nb,
+                                                         double *__restrict__ S, double epsilon) {
   if (st->done) return;
-  const int lane = threadIdx.x;
-  double e = 0.0, sm = 0.0;
-  for (int b = 0; b < nb; b++) {
-    e += p_err[(size_t)b * kWave + lane];
-    sm += p_sum[(size_t)b * kWave + lane];
-  }
-  S[lane] = sm;
+  __shared__ double lds[2][kBlock];
+  const double e = block_order_sum(p_err, nb, lds[0]);
+  const double sm = block_order_sum(p_sum, nb, lds[1]);
+  if (threadIdx.x >= kWave) return;
+  S[threadIdx.x] = sm;
   double m = e;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     st->max_err = m;
     st->iters += 1;
     if (m < epsilon) st->done = 1;
@@ -151,33 +248,40 @@ __global__ __launch_bounds__(kBlock) void k_fill_d(double *__restrict__ a, long 
 // (pagerank_norma_end, Matrix.cpp:149-171)
 __global__ __launch_bounds__(kBlock) void k_col_partial(const double *__restrict__ X, long long n_rows,
                                                         double *__restrict__ part) {
-  __shared__ double lds[kBlock / kWave][kWave];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  double sm = 0.0;
-  for (long long r = (long long)blockIdx.x * (kBlock / kWave) + wave; r < n_rows;
-       r += (long long)gridDim.x * (kBlock / kWave))
-    sm += X[(size_t)r * kWave + lane];
-  lds[wave][lane] = sm;
+  __shared__ double lds[kPprSlots][kWave];
+  const int lane = threadIdx.x & (kPprLanes - 1), slot = threadIdx.x / kPprLanes;
+  const int c0 = kPprCols * lane;
+  double sm[kPprCols];
+#pragma unroll
+  for (int q = 0; q < kPprCols; q++) sm[q] = 0.0;
+  const RowWalk wk = xcd_walk(n_rows, gridDim.x, slot);
+  for (long long r = wk.first; r < wk.end; r += wk.stride) {
+    const pprvec_t v = *reinterpret_cast<const pprvec_t *>(X + (size_t)r * kWave + c0);
+#pragma unroll
+    for (int q = 0; q < kPprCols; q++) sm[q] += vget(v, q);
+  }
+#pragma unroll
+  for (int q = 0; q < kPprCols; q++) lds[slot][c0 + q] = sm[q];
   __syncthreads();
-  if (wave == 0) {
+  if (threadIdx.x < kWave) {
     double t = 0.0;
-    for (int w = 0; w < kBlock / kWave; w++) t += lds[w][lane];
-    part[(size_t)blockIdx.x * kWave + lane] = t;
+    for (int w = 0; w < kPprSlots; w++) t += lds[w][threadIdx.x];
+    part[(size_t)blockIdx.x * kWave + threadIdx.x] = t;
   }
 }
 
-__global__ __launch_bounds__(kWave) void k_col_total(const double *__restrict__ part, int nb,
-                                                     double *__restrict__ tot) {
-  double t = 0.0;
-  for (int b = 0; b < nb; b++) t += part[(size_t)b * kWave + threadIdx.x];
-  tot[threadIdx.x] = t;
+__global__ __launch_bounds__(kBlock) void k_col_total(const double *__restrict__ part, int nb,
+                                                      double *__restrict__ tot) {
+  __shared__ double lds[kBlock];
+  const double t = block_order_sum(part, nb, lds);
+  if (threadIdx.x < kWave) tot[threadIdx.x] = t;
 }
 
 __global__ __launch_bounds__(kBlock) void k_col_divide(double *__restrict__ X, long long n,
                                                        const double *__restrict__ tot) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
-    X[i] /= tot[i % kWave];
+    X[i] /= tot[i & (kWave - 1)];
 }
 
 unsigned stride_grid(long long n) {
@@ -222,7 +326,11 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     return fail(TBSLA_HIP_ERR_INVALID, "personalised PageRank needs a square diagonal block");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long n = m->ln_row;
-  const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>(2048, (n + 3) / 4));
+  // kPprGrid workgroups (a multiple of 8: one contiguous eighth of the rows
+  // per XCD) unless the block has fewer rows than slots
+  const unsigned nb = n >= (long long)kPprSlots * kPprGrid
+                          ? (unsigned)kPprGrid
+                          : (unsigned)std::max<long long>(1, (n + kPprSlots - 1) / kPprSlots);
   double *buf = nullptr, *part = nullptr, *S = nullptr;
   int *src = nullptr;
   PprState *st = nullptr;
@@ -256,7 +364,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
       k_ppr_step<<<nb, kBlock, 0, s>>>(n, m->f_col, m->ln_col, m->rowptr, m->colidx, m->values,
                                        Xb[launched % 2], Xb[(launched + 1) % 2], src, S, beta, st,
                                        part, part + (size_t)nb * kWave);
-      k_ppr_finalize<<<1, kWave, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
+      k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -269,7 +377,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   }
   double *X = Xb[hst.iters % 2];
   k_col_partial<<<nb, kBlock, 0, s>>>(X, n, part);
-  k_col_total<<<1, kWave, 0, s>>>(part, (int)nb, S);
+  k_col_total<<<1, kBlock, 0, s>>>(part, (int)nb, S);
   k_col_divide<<<stride_grid(n * kWave), kBlock, 0, s>>>(X, n * kWave, S);
   if (X != d_X) hipMemcpyAsync(d_X, X, (size_t)n * kWave * 8, hipMemcpyDeviceToDevice, s);
   hipError_t e = hipGetLastError();

@@ -40,8 +40,10 @@ def stochastic_pair(n, c, q, seed):
     return o, g
 
 
-def test_ppr_columns_match_single_vector_oracle():
-    n = 20000
+@pytest.mark.parametrize("n", [20000, 70000])
+def test_ppr_columns_match_single_vector_oracle(n):
+    """n = 70000 takes the XCD-grouped row walk (a full grid of workgroups);
+    n = 20000 the one-slot-per-row walk of a smaller grid."""
     o, g = stochastic_pair(n, 16, 0.1, 2)
     rng = np.random.default_rng(1)
     src = rng.choice(n, 64, replace=False).astype(np.int32)
