@@ -47,20 +47,22 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "csr_spmv_cqmat_n4194304_c32_traff
 
 def measured_traffic(kernel: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this
-    workload (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy): its
-    `traffic_bytes_per_launch:` line, when its `kernel:` line names the
-    kernel measured now; else None."""
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy): the
+    `traffic_bytes_per_launch:` line that follows the `kernel:` line naming
+    the kernel measured now; else None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    fields = {}
+    current = None
     with open(TRAFFIC_FILE) as f:
         for line in f:
-            if ":" in line and not line.startswith(" "):
-                k, v = line.split(":", 1)
-                fields[k.strip()] = v.strip()
-    if fields.get("kernel") != kernel or "traffic_bytes_per_launch" not in fields:
-        return None
-    return int(float(fields["traffic_bytes_per_launch"]))
+            if ":" not in line or line.startswith(" "):
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            if k == "kernel":
+                current = v
+            elif k == "traffic_bytes_per_launch" and current == kernel:
+                return int(float(v))
+    return None
 
 
 def csr_kernel_name(k: dict, split: bool) -> str:
@@ -197,6 +199,8 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG time-to-solution section")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default CSR schedule (row layout) instead of the autotuned one")
+    ap.add_argument("--csr-kernel", default=None, metavar="VEC,LANES,ROWS,NT",
+                    help="use this CSR row kernel (row layout) instead of the autotuned one")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-CG and personalised-PageRank sections")
     ap.add_argument("--batched-systems", type=int, default=512)
@@ -219,7 +223,10 @@ def main():
     dy = T.DeviceBuffer(N * 8)
     # schedule and layout from the autotuner (untimed)
     t0 = time.time()
-    if not args.no_autotune:
+    if args.csr_kernel:
+        vec, lanes, rows, nt = (int(t) for t in args.csr_kernel.split(","))
+        m.set_csr_kernel(lanes, rows, vec, nt)
+    elif not args.no_autotune:
         m.autotune(dy, dx, 5)
     t_tune = time.time() - t0
     layout = m.get_csr_layout()

@@ -235,7 +235,13 @@ typedef struct {
   int lanes_per_row;     /* lanes per matrix row in the iteration's SpMV (power of two
                             <= 64); 0 picks from the mean row length */
   int stall_window;      /* > 0: breakdown when |r| >= |r0| after that many iterations */
+  int factor_form;       /* _FACTORED: TBSLA_HIP_FACTOR_AUTO picks (the separate form); _SEPARATE applies G^T
+                            in the update kernel and G in a kernel of its own; _FUSED
+                            computes z = G (G^T r) in the update kernel (one kernel
+                            fewer per iteration, G^T rows gathered once per G entry);
+                            the iterates are bitwise the same either way */
 } tbsla_hip_cg_options;
+enum { TBSLA_HIP_FACTOR_AUTO = 0, TBSLA_HIP_FACTOR_SEPARATE = 1, TBSLA_HIP_FACTOR_FUSED = 2 };
 
 /* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call
  * corrected (:61 passes output and input swapped, so the reference never

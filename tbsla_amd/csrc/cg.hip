@@ -360,7 +360,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   if (threadIdx.x == 0) a.ppap[b] = pap;
 }
 
-// K2.  PRE: 0 none, 1 Jacobi, 2 M = G G^T.
+// K2.  PRE: 0 none, 1 Jacobi, 2 M = G G^T (t = G^T r' here, z = G t in K3),
+// 3 M = G G^T with z = G (G^T r') here: each G entry (row, k) gathers G^T's
+// row k and r', Ap at its columns; t_k is summed exactly as PRE 2 sums it
+// and z as K3 does, so the iterates are bitwise those of PRE 2 + K3.
 template <int PRE, bool MIX>
 __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int count) {
   TBSLA_NO_FMA
@@ -376,6 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   double *rn = a.r[par ^ 1];
   const double *pn = a.p[par ^ 1];
   const V *tv = MIX ? (const V *)a.t_v32 : (const V *)a.t_v64;
+  const V *gv = MIX ? (const V *)a.g_v32 : (const V *)a.g_v64;
   const int r0 = S->row0 + (b - b0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
   const int row_first = r0 + (int)threadIdx.x;
@@ -394,6 +398,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       const int j = min(f.beg + k, a.g_nnz - 1);
       f.col[k] = a.t_ci[j];
       f.v[k] = tv[j];
+    }
+  }
+  // PRE 3: the first row's G entries and the extents of the G^T rows they
+  // name
+  RowPrefetch<P, V> fg;
+  int tb[P], te[P];
+  fg.beg = fg.end = 0;
+  if (PRE == 3) {
+    fg.beg = a.g_rp[rowc];
+    fg.end = a.g_rp[rowc + 1];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      const int j = min(fg.beg + k, a.g_nnz - 1);
+      fg.col[k] = a.g_ci[j];
+      fg.v[k] = gv[j];
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      tb[k] = a.t_rp[fg.col[k] - a.f_col];
+      te[k] = a.t_rp[fg.col[k] - a.f_col + 1];
     }
   }
   const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
@@ -456,13 +480,54 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       if (MIX) a.t32[row] = (float)acc;
       else a.t64[row] = (double)acc;
     }
+    if (PRE == 3) {
+      // z = G t' with t'_k = sum over G^T row k of G^T_kc r'_c, r'_c = r_c - alpha Ap_c
+      V zacc = 0;
+      int jg, gend;
+      if (first) {
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          if (fg.beg + k < fg.end) {
+            V tk = 0;
+            for (int j = tb[k]; j < te[k]; j++) {
+              const int c = a.t_ci[j] - a.f_col;
+              const double rcv = rc[c] + (-alpha) * a.ap[c];
+              if (MIX) tk = fmaf((float)tv[j], (float)rcv, (float)tk);
+              else tk = fma((double)tv[j], rcv, (double)tk);
+            }
+            if (MIX) zacc = fmaf((float)fg.v[k], (float)tk, (float)zacc);
+            else zacc = fma((double)fg.v[k], (double)tk, (double)zacc);
+          }
+        }
+        jg = fg.beg + P;
+        gend = fg.end;
+      } else {
+        jg = a.g_rp[row];
+        gend = a.g_rp[row + 1];
+      }
+      for (; jg < gend; jg++) {
+        const int k = a.g_ci[jg] - a.f_col;
+        V tk = 0;
+        for (int j = a.t_rp[k]; j < a.t_rp[k + 1]; j++) {
+          const int c = a.t_ci[j] - a.f_col;
+          const double rcv = rc[c] + (-alpha) * a.ap[c];
+          if (MIX) tk = fmaf((float)tv[j], (float)rcv, (float)tk);
+          else tk = fma((double)tv[j], rcv, (double)tk);
+        }
+        if (MIX) zacc = fmaf((float)gv[jg], (float)tk, (float)zacc);
+        else zacc = fma((double)gv[jg], (double)tk, (double)zacc);
+      }
+      const double zi = (double)zacc;
+      a.z[row] = zi;
+      rzp += ri * zi;
+    }
   }
   rr = wg_sum(rr, lds);
-  if (PRE == 1) rzp = wg_sum(rzp, lds);
+  if (PRE == 1 || PRE == 3) rzp = wg_sum(rzp, lds);
   if (threadIdx.x == 0) {
     a.prr[par][b] = rr;
     if (PRE == 0) a.prz[par][b] = rr;
-    if (PRE == 1) a.prz[par][b] = rzp;
+    if (PRE == 1 || PRE == 3) a.prz[par][b] = rzp;
     if (b == b0) {
       S->true_res = 0;
       if (count) S->iters += 1;  // the test K1 made on this iteration passed
@@ -792,7 +857,7 @@ struct Buf {
 struct GraphKey {
   CgArgs args;
   const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
-  int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt;
+  int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
 };
 
@@ -881,6 +946,7 @@ namespace {
 
 struct Plan {
   int L, pre, mixed, replace, verify, stall;
+  bool fused;  // M = G G^T: z = G (G^T r) in K2
   long long n;
   int nblocks;
   hipStream_t s;
@@ -915,7 +981,7 @@ void launch_k1_any(const CgArgs &a, const Plan &p, int par, int check, int check
   }
 }
 
-void launch_k2(const CgArgs &a, const Plan &p, int par, int count) {
+void launch_k2(const CgArgs &a, const Plan &p, int par, int count, bool fused) {
   const int g = p.nblocks;
   if (p.pre == 0) {
     if (p.mixed) k_cg_update<0, true><<<g, kBlock, 0, p.s>>>(a, par, count);
@@ -923,6 +989,9 @@ void launch_k2(const CgArgs &a, const Plan &p, int par, int count) {
   } else if (p.pre == 1) {
     if (p.mixed) k_cg_update<1, true><<<g, kBlock, 0, p.s>>>(a, par, count);
     else k_cg_update<1, false><<<g, kBlock, 0, p.s>>>(a, par, count);
+  } else if (fused) {
+    if (p.mixed) k_cg_update<3, true><<<g, kBlock, 0, p.s>>>(a, par, count);
+    else k_cg_update<3, false><<<g, kBlock, 0, p.s>>>(a, par, count);
   } else {
     if (p.mixed) k_cg_update<2, true><<<g, kBlock, 0, p.s>>>(a, par, count);
     else k_cg_update<2, false><<<g, kBlock, 0, p.s>>>(a, par, count);
@@ -971,7 +1040,7 @@ int true_residual(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double 
   return TBSLA_HIP_SUCCESS;
 }
 
-// iteration i (global): K1 (tested when i > 0), K2, K3; then, after every
+// iteration i (global): K1 (tested when i > 0), K2, K3 (unless K2 made z); then, after every
 // replace_every-th iteration (i a multiple of it), the true FP64 residual
 // replaces the recurrence's.  tail: also settle the systems waiting for
 // their true-residual test (a replacement step does it anyway: the same
@@ -981,11 +1050,13 @@ int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, dou
                       bool tail) {
   const int par = i & 1;
   launch_k1_any(a, p, par, i > 0, 0);
-  launch_k2(a, p, par, i > 0);
   const bool replace = p.replace > 0 && i > 0 && i % p.replace == 0;
+  // a replacement step recomputes z from the true residual: K2 then needs
+  // no z of its own
+  launch_k2(a, p, par, i > 0, p.fused && !replace);
   if (replace) {
     TBSLA_TRY(true_residual(m, a, p, ax, par, kAfterIter));
-  } else if (p.pre == 2) {
+  } else if (p.pre == 2 && !p.fused) {
     launch_gz(a, p, par, kAfterIter);
   }
   if (p.verify && (replace || tail)) {
@@ -1036,6 +1107,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown preconditioner");
   if (o.precision != TBSLA_HIP_CG_FP64 && o.precision != TBSLA_HIP_CG_MIXED)
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown precision");
+  if (o.factor_form < TBSLA_HIP_FACTOR_AUTO || o.factor_form > TBSLA_HIP_FACTOR_FUSED)
+    return fail(TBSLA_HIP_ERR_INVALID, "unknown factor_form");
   tbsla_hip_matrix_s *G = o.factor;
   const int pre = o.preconditioner;
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
@@ -1209,6 +1282,11 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   p.replace = std::max(0, o.replace_every);
   p.verify = a.verify;
   p.stall = o.stall_window;
+  // the fused form saves a kernel per iteration at the price of gathering
+  // each G^T row once per G entry; on MI355X that costs more than the kernel
+  // it saves (7-point Poisson, 64^3: 38 vs 37 us per iteration; 128^3: 286
+  // vs 216 us), so the automatic choice is the separate form
+  p.fused = pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_FUSED;
   p.n = n;
   p.nblocks = nblocks;
   p.s = s;
@@ -1278,6 +1356,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.replace = p.replace;
   key.batch = B;
   key.nblocks = nblocks;
+  key.fused = p.fused ? 1 : 0;
   key.spmv_lanes = pick_csr_lanes(m);
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
   key.spmv_vec = pick_csr_vec(m);

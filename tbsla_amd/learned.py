@@ -92,6 +92,7 @@ class Solver:
         self.n = A.geometry().ln_row
         self.x = T.DeviceBuffer(self.n * 8)
         self.G, self.GT = T.Matrix(T.CSR), T.Matrix(T.CSR)
+        self.factor_form = T.FACTOR_AUTO
 
     def run(self, method: str, tol: float = 1e-8, max_iterations: int = 20000) -> dict:
         A = self.A
@@ -103,7 +104,8 @@ class Solver:
         if method.startswith("learned"):
             T.check(T.lib().tbsla_hip_precond_net_apply(self.net.h, A.h, self.G.h, self.GT.h, None))
             o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_FACTORED, rep,
-                             factor=self.G, factor_t=self.GT, stall_window=100)
+                             factor=self.G, factor_t=self.GT, stall_window=100,
+                             factor_form=self.factor_form)
         elif method.startswith("jacobi"):
             o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_JACOBI, rep)
         else:

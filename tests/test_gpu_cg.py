@@ -84,6 +84,30 @@ def test_factored_early_iterates_and_given_transpose(sym):
         assert np.array_equal(xg, xt)
 
 
+@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
+def test_factored_fused_and_separate_identical(sym, precision):
+    """z = G (G^T r) in the update kernel gives bitwise the iterates of the
+    separate G^T and G kernels (with residual replacement every 7 in the mixed
+    form, whose steps take the separate path), and both match the oracle."""
+    o, g = sym
+    n = o.geom["ln_row"]
+    Go = lower_factor(o)
+    G = to_gpu(Go)
+    rhs = 2 * O.unit_vector(15, n) - 1
+    rep = 7 if precision == T.CG_MIXED else 0
+    runs = [g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+                 preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=form)
+            for form in (T.FACTOR_SEPARATE, T.FACTOR_FUSED, T.FACTOR_AUTO)]
+    for x, it, rn in runs[1:]:
+        assert it == runs[0][1] and rn == runs[0][2]
+        assert np.array_equal(x, runs[0][0])
+    xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=2, G=Go)
+    assert abs(runs[0][1] - ito) <= 2
+    assert rel_max(runs[0][0], xo) < 1e-7
+    with pytest.raises(T.TbslaHipError):
+        g.cg(rhs, preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=3)
+
+
 @pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI, T.PRECOND_FACTORED])
 def test_converged_iterations_like_oracle(sym, pre):
     o, g = sym
