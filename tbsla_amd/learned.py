@@ -178,6 +178,8 @@ def main(argv=None):
     ev.add_argument("--first", type=int, default=100_000)
     ev.add_argument("--count", type=int, default=20)
     ev.add_argument("--weights", default=WEIGHTS)
+    ev.add_argument("--methods", default=",".join(METHODS),
+                    help="comma-separated subset of " + ",".join(METHODS))
     args = ap.parse_args(argv)
 
     def log(s):
@@ -193,7 +195,12 @@ def main(argv=None):
         net = T.PrecondNet()
         if not load_weights(net, args.weights):
             sys.exit(f"no weights at {args.weights}")
-        rows = evaluate(net, args.side, range(args.first, args.first + args.count), log=log)
+        methods = tuple(m for m in args.methods.split(",") if m)
+        unknown = [m for m in methods if m not in METHODS]
+        if unknown:
+            sys.exit(f"unknown methods {unknown}")
+        rows = evaluate(net, args.side, range(args.first, args.first + args.count), methods,
+                        log=log)
         log(json.dumps(summarise(rows)))
 
 
