@@ -239,9 +239,19 @@ typedef struct {
                             in the update kernel and G in a kernel of its own; _FUSED
                             computes z = G (G^T r) in the update kernel (one kernel
                             fewer per iteration, G^T rows gathered once per G entry);
-                            the iterates are bitwise the same either way */
+                            the iterates are bitwise the same either way; _PRODUCT
+                            builds A G once per call (rows of A with at most 64
+                            products, else the separate form) and computes
+                            Ap' = (A G) t + beta Ap with r.z = t.t, so no kernel
+                            applies G on its own (a different rounding; Ap is
+                            recomputed in FP64 at every residual replacement) */
 } tbsla_hip_cg_options;
-enum { TBSLA_HIP_FACTOR_AUTO = 0, TBSLA_HIP_FACTOR_SEPARATE = 1, TBSLA_HIP_FACTOR_FUSED = 2 };
+enum {
+  TBSLA_HIP_FACTOR_AUTO = 0,
+  TBSLA_HIP_FACTOR_SEPARATE = 1,
+  TBSLA_HIP_FACTOR_FUSED = 2,
+  TBSLA_HIP_FACTOR_PRODUCT = 3
+};
 
 /* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call
  * corrected (:61 passes output and input swapped, so the reference never

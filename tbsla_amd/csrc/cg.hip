@@ -82,6 +82,12 @@ struct CgArgs {
   const double *t_v64;
   const float *t_v32;
   const double *dinv;
+  // product form of M = G G^T: rows of A G (local columns)
+  const int *ag_rp;
+  const int *ag_ci;
+  const double *ag_v64;
+  const float *ag_v32;
+  int ag_nnz;
   const double *b;
   double *x;
   double *r[2];
@@ -244,6 +250,33 @@ __device__ inline int seg_finish(int flag, const SegLoad &l, int np, int b0, int
   return f;
 }
 
+// The convergence test of the previous iteration (CG.cpp:66-72) on
+// |r| = sqrt(rr), made alike by every workgroup of the system from the same
+// sums; the system's first workgroup records the outcome.  Returns it.
+__device__ inline int k1_test(const CgArgs &a, CgSeg *S, int b, int b0, int par, double rz_new,
+                              double rr, int check_only) {
+  const double rn = sqrt(rr);
+  int fin = kRunning;
+  if (rn < S->stop || rn == 0.0)
+    fin = (a.verify && !S->true_res) ? kPending : kConverged;
+  else if (!(rz_new > 0.0) || !isfinite(rz_new) || !isfinite(rr))
+    fin = kBreakdown;
+  else if (!check_only && a.stall_window > 0 && S->iters >= a.stall_window && rn >= S->rnorm0)
+    fin = kBreakdown;
+  if (b == b0 && threadIdx.x == 0) {
+    S->rnorm = rn;
+    if (fin != kRunning) {
+      S->stop_par = par;
+      S->t_done = wall_clock64();
+      __atomic_store_n(&S->done, fin, __ATOMIC_RELAXED);
+      atomicAdd(fin == kPending ? &a.com->npending : &a.com->ndone, 1);
+    } else if (check_only) {
+      S->iters += 1;  // the count K2 would have made
+    }
+  }
+  return fin;
+}
+
 // K1.  check: apply the convergence test of the previous iteration first
 // (the count it++ of a passed test is made by K2, which no K1 workgroup
 // races with).  check_only: the final test after the last iteration.
@@ -298,25 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   const double rz_new = t[0], rz_old = t[1];
   const double rr = ZBUF ? t[2] : rz_new;
   if (check) {
-    const double rn = sqrt(rr);
-    int fin = kRunning;
-    if (rn < S->stop || rn == 0.0)
-      fin = (a.verify && !S->true_res) ? kPending : kConverged;
-    else if (!(rz_new > 0.0) || !isfinite(rz_new) || !isfinite(rr))
-      fin = kBreakdown;
-    else if (!check_only && a.stall_window > 0 && S->iters >= a.stall_window && rn >= S->rnorm0)
-      fin = kBreakdown;
-    if (b == b0 && threadIdx.x == 0) {
-      S->rnorm = rn;
-      if (fin != kRunning) {
-        S->stop_par = par;
-        S->t_done = wall_clock64();
-        __atomic_store_n(&S->done, fin, __ATOMIC_RELAXED);
-        atomicAdd(fin == kPending ? &a.com->npending : &a.com->ndone, 1);
-      } else if (check_only) {
-        S->iters += 1;  // the count K2 would have made
-      }
-    }
+    const int fin = k1_test(a, S, b, b0, par, rz_new, rr, check_only);
     if (fin != kRunning || check_only) return;
   }
   const double beta = rz_new / fmax(rz_old, S->near_zero);
@@ -360,7 +375,103 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   if (threadIdx.x == 0) a.ppap[b] = pap;
 }
 
+// K1 of the product form of M = G G^T: z = G t is never stored.  For its
+// rows the kernel forms z_i = (G t)_i (in K3's order), p'_i = z_i + beta p_i
+// and, by the recurrence A p' = A G t + beta A p, Ap'_i = (A G t)_i +
+// beta Ap_i from the rows of the product matrix A G -- one kernel fewer per
+// iteration, no z gathers.  The recurrence is restarted from A p in FP64 at
+// every residual replacement, and beta = 0 (start, restart) drops it.
+template <int L, bool MIX>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_ag(CgArgs a, int par_g, int check,
+                                                       int check_only) {
+  TBSLA_NO_FMA
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int P = L == 1 ? 8 : 4;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ SegShared sh;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  const int par = (par_g ^ S->shift) & 1;
+  const int b0 = S->blk0, b1 = S->blk1;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const int lane = threadIdx.x & (L - 1);
+  const V *tv = MIX ? (const V *)a.t32 : (const V *)a.t64;
+  const V *agv = MIX ? (const V *)a.ag_v32 : (const V *)a.ag_v64;
+  const V *gv = MIX ? (const V *)a.g_v32 : (const V *)a.g_v64;
+  const double *po = a.p[par];
+  const int row_first = r0 + (int)threadIdx.x / L;
+  const int rowc = min(row_first, r1 - 1);
+  // as K1: the first row's entries, the partials, then the gathers
+  RowPrefetch<P, V> f;
+  f.beg = a.ag_rp[rowc];
+  f.end = a.ag_rp[rowc + 1];
+  const double own_p = po[rowc], own_ap = a.ap[rowc];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    const int j = min(f.beg + lane + k * L, a.ag_nnz - 1);
+    f.col[k] = a.ag_ci[j];
+    f.v[k] = agv[j];
+  }
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], 3, b0, b1, sl);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < P; k++) f.s[k] = (double)tv[f.col[k]];
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads above issued in this order
+  double t[3];
+  if (seg_finish(flag, sl, 3, b0, b1, sh, t) != kRunning) return;
+  const double rz_new = t[0], rz_old = t[1], rr = t[2];
+  if (check) {
+    const int fin = k1_test(a, S, b, b0, par, rz_new, rr, check_only);
+    if (fin != kRunning || check_only) return;
+  }
+  const double beta = rz_new / fmax(rz_old, S->near_zero);
+  double *pn = a.p[par ^ 1];
+  double pap = 0.0;
+  for (int row = row_first; row < r1; row += kBlock / L) {
+    const bool first = row == row_first;
+    int j, end;
+    V acc = 0;
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < P; k++)
+        if (f.beg + lane + k * L < f.end) {
+          if (MIX) acc = fmaf((float)f.v[k], (float)f.s[k], (float)acc);
+          else acc = fma((double)f.v[k], f.s[k], (double)acc);
+        }
+      j = f.beg + lane + P * L;
+      end = f.end;
+    } else {
+      j = a.ag_rp[row] + lane;
+      end = a.ag_rp[row + 1];
+    }
+    for (; j < end; j += L) {
+      if (MIX) acc = fmaf((float)agv[j], (float)tv[a.ag_ci[j]], (float)acc);
+      else acc = fma((double)agv[j], (double)tv[a.ag_ci[j]], (double)acc);
+    }
+    acc = lane_sum<L>(acc);
+    if (lane == 0) {
+      V zacc = 0;
+      for (int jg = a.g_rp[row]; jg < a.g_rp[row + 1]; jg++) {
+        if (MIX) zacc = fmaf((float)gv[jg], (float)tv[a.g_ci[jg] - a.f_col], (float)zacc);
+        else zacc = fma((double)gv[jg], (double)tv[a.g_ci[jg] - a.f_col], (double)zacc);
+      }
+      const double own = (double)zacc + beta * (first ? own_p : po[row]);
+      const double apo = first ? own_ap : a.ap[row];
+      const double apn = (double)acc + (beta != 0.0 ? beta * apo : 0.0);
+      pn[row] = own;
+      a.ap[row] = apn;
+      pap += own * apn;
+    }
+  }
+  pap = wg_sum(pap, lds);
+  if (threadIdx.x == 0) a.ppap[b] = pap;
+}
+
 // K2.  PRE: 0 none, 1 Jacobi, 2 M = G G^T (t = G^T r' here, z = G t in K3),
+// 4 M = G G^T in the product form (t = G^T r' and t.t partials: r.z = t.t),
 // 3 M = G G^T with z = G (G^T r') here: each G entry (row, k) gathers G^T's
 // row k and r', Ap at its columns; t_k is summed exactly as PRE 2 sums it
 // and z as K3 does, so the iterates are bitwise those of PRE 2 + K3.
@@ -390,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   const double fd = PRE == 1 ? a.dinv[rowc] : 0.0;
   RowPrefetch<P, V> f;
   f.beg = f.end = 0;
-  if (PRE == 2) {
+  if (PRE == 2 || PRE == 4) {
     f.beg = a.t_rp[rowc];
     f.end = a.t_rp[rowc + 1];
 #pragma unroll
@@ -424,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   SegLoad sl;
   seg_issue(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, b1, sl);
   __builtin_amdgcn_sched_barrier(0);
-  if (PRE == 2) {
+  if (PRE == 2 || PRE == 4) {
 #pragma unroll
     for (int k = 0; k < P; k++) {
       f.s[k] = rc[f.col[k] - a.f_col];
@@ -452,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       a.z[row] = zi;
       rzp += ri * zi;
     }
-    if (PRE == 2) {
+    if (PRE == 2 || PRE == 4) {
       // t = G^T r' with r'_c = r_c - alpha Ap_c computed as above
       V acc = 0;
       int j, end;
@@ -479,6 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       }
       if (MIX) a.t32[row] = (float)acc;
       else a.t64[row] = (double)acc;
+      if (PRE == 4) rzp += (double)acc * (double)acc;
     }
     if (PRE == 3) {
       // z = G t' with t'_k = sum over G^T row k of G^T_kc r'_c, r'_c = r_c - alpha Ap_c
@@ -523,11 +635,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
     }
   }
   rr = wg_sum(rr, lds);
-  if (PRE == 1 || PRE == 3) rzp = wg_sum(rzp, lds);
+  if (PRE == 1 || PRE == 3 || PRE == 4) rzp = wg_sum(rzp, lds);
   if (threadIdx.x == 0) {
     a.prr[par][b] = rr;
     if (PRE == 0) a.prz[par][b] = rr;
-    if (PRE == 1 || PRE == 3) a.prz[par][b] = rzp;
+    if (PRE == 1 || PRE == 3 || PRE == 4) a.prz[par][b] = rzp;
     if (b == b0) {
       S->true_res = 0;
       if (count) S->iters += 1;  // the test K1 made on this iteration passed
@@ -633,10 +745,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode)
   if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
 }
 
-// t = G^T r for the current residual (replacement, verification, start).
-template <bool MIX>
+// t = G^T r for the current residual (replacement, verification, start);
+// TT (the product form): also the t.t partials, r.z of M = G G^T.
+template <bool MIX, bool TT>
 __global__ __launch_bounds__(kBlock) void k_cg_gtr(CgArgs a, int par_g, int mode) {
   TBSLA_NO_FMA
+  __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
   CgSeg *S = a.seg + a.block_seg[b];
@@ -645,18 +759,48 @@ __global__ __launch_bounds__(kBlock) void k_cg_gtr(CgArgs a, int par_g, int mode
   const double *rc = a.r[rsel];
   const int r0 = S->row0 + (b - S->blk0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
+  double tt = 0.0;
   for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
     if (MIX) {
       float acc = 0.0f;
       for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++)
         acc = fmaf(a.t_v32[j], (float)rc[a.t_ci[j] - a.f_col], acc);
       a.t32[row] = acc;
+      if (TT) tt += (double)acc * (double)acc;
     } else {
       double acc = 0.0;
       for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++)
         acc = fma(a.t_v64[j], rc[a.t_ci[j] - a.f_col], acc);
       a.t64[row] = acc;
+      if (TT) tt += acc * acc;
     }
+  }
+  if (TT) {
+    tt = wg_sum(tt, lds);
+    if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = tt;
+  }
+}
+
+// ap = A p (FP64 matrix) for the current p of the running systems (the
+// product form's Ap recurrence restarts from it at each residual
+// replacement).
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_cg_ap(CgArgs a, int par_g) {
+  __shared__ int s_flag;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  int rsel;
+  if (!mode_select(S, kAfterIter, par_g, &s_flag, &rsel)) return;
+  const double *pc = a.p[rsel];
+  const int r0 = S->row0 + (b - S->blk0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const int lane = threadIdx.x & (L - 1);
+  for (int row = r0 + (int)threadIdx.x / L; row < r1; row += kBlock / L) {
+    double acc = 0.0;
+    for (int j = a.rowptr[row] + lane; j < a.rowptr[row + 1]; j += L)
+      acc = fma(a.val64[j], pc[a.colidx[j] - a.f_col], acc);
+    acc = lane_sum<L>(acc);
+    if (lane == 0) a.ap[row] = acc;
   }
 }
 
@@ -857,7 +1001,7 @@ struct Buf {
 struct GraphKey {
   CgArgs args;
   const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
-  int L, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused;
+  int L, Lk1, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused, product;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
 };
 
@@ -885,6 +1029,23 @@ struct CgCache {
   double *t_v64 = nullptr;
   float *g_v32 = nullptr, *t_v32 = nullptr;
   size_t f32_len = 0;
+  // product form: A G (rebuilt per call into buffers kept while large
+  // enough, so a graph captured for one problem serves the next of the same
+  // size), FP32 copy for the mixed form
+  int *ag_rp = nullptr, *ag_ci = nullptr;
+  double *ag_v64 = nullptr;
+  float *ag_v32 = nullptr;
+  long long ag_nnz = 0, ag_cap = 0, ag_rows = 0;
+  void drop_product() {
+    hipFree(ag_rp);
+    hipFree(ag_ci);
+    hipFree(ag_v64);
+    hipFree(ag_v32);
+    ag_rp = ag_ci = nullptr;
+    ag_v64 = nullptr;
+    ag_v32 = nullptr;
+    ag_nnz = ag_cap = ag_rows = 0;
+  }
   // graphs: [0] plain batch, [1] batch ending with a replacement step
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   GraphKey key{};
@@ -902,6 +1063,7 @@ struct CgCache {
     g_v32 = t_v32 = nullptr;
     f32_len = 0;
     factor = nullptr;
+    drop_product();
   }
   ~CgCache() {
     hipDeviceSynchronize();
@@ -946,7 +1108,9 @@ namespace {
 
 struct Plan {
   int L, pre, mixed, replace, verify, stall;
-  bool fused;  // M = G G^T: z = G (G^T r) in K2
+  int Lk1;  // lanes per row of K1 (the product form reads A G's longer rows)
+  bool fused;    // M = G G^T: z = G (G^T r) in K2
+  bool product;  // M = G G^T in the product form (K1 with A G rows, no K3)
   long long n;
   int nblocks;
   hipStream_t s;
@@ -959,6 +1123,11 @@ void launch_k1(const CgArgs &a, const Plan &p, int par, int check, int check_onl
 
 template <int L>
 void launch_k1_l(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
+  if (p.product) {
+    if (p.mixed) k_cg_spmv_ag<L, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+    else k_cg_spmv_ag<L, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+    return;
+  }
   const bool zbuf = p.pre != 0;
   if (p.mixed) {
     if (zbuf) launch_k1<L, true, true>(a, p, par, check, check_only);
@@ -970,7 +1139,7 @@ void launch_k1_l(const CgArgs &a, const Plan &p, int par, int check, int check_o
 }
 
 void launch_k1_any(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
-  switch (p.L) {
+  switch (p.Lk1) {
     case 1: launch_k1_l<1>(a, p, par, check, check_only); break;
     case 2: launch_k1_l<2>(a, p, par, check, check_only); break;
     case 4: launch_k1_l<4>(a, p, par, check, check_only); break;
@@ -992,6 +1161,9 @@ void launch_k2(const CgArgs &a, const Plan &p, int par, int count, bool fused) {
   } else if (fused) {
     if (p.mixed) k_cg_update<3, true><<<g, kBlock, 0, p.s>>>(a, par, count);
     else k_cg_update<3, false><<<g, kBlock, 0, p.s>>>(a, par, count);
+  } else if (p.product) {
+    if (p.mixed) k_cg_update<4, true><<<g, kBlock, 0, p.s>>>(a, par, count);
+    else k_cg_update<4, false><<<g, kBlock, 0, p.s>>>(a, par, count);
   } else {
     if (p.mixed) k_cg_update<2, true><<<g, kBlock, 0, p.s>>>(a, par, count);
     else k_cg_update<2, false><<<g, kBlock, 0, p.s>>>(a, par, count);
@@ -1004,8 +1176,26 @@ void launch_gz(const CgArgs &a, const Plan &p, int par, int mode) {
 }
 
 void launch_gtr(const CgArgs &a, const Plan &p, int par, int mode) {
-  if (p.mixed) k_cg_gtr<true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
-  else k_cg_gtr<false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+  if (p.product) {
+    if (p.mixed) k_cg_gtr<true, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+    else k_cg_gtr<false, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+  } else {
+    if (p.mixed) k_cg_gtr<true, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+    else k_cg_gtr<false, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+  }
+}
+
+// the product form's A p (FP64) for the running systems
+void launch_ap(const CgArgs &a, const Plan &p, int par) {
+  switch (p.L) {
+    case 1: k_cg_ap<1><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 2: k_cg_ap<2><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 4: k_cg_ap<4><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 8: k_cg_ap<8><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 16: k_cg_ap<16><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 32: k_cg_ap<32><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    default: k_cg_ap<64><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+  }
 }
 
 void launch_residual(const CgArgs &a, const Plan &p, const double *ax, int par, int mode,
@@ -1035,8 +1225,10 @@ int true_residual(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double 
   launch_residual(a, p, ax, par, mode, 0);
   if (p.pre == 2) {
     launch_gtr(a, p, par, mode);
-    launch_gz(a, p, par, mode);
+    if (!p.product) launch_gz(a, p, par, mode);
   }
+  // the product form's Ap recurrence restarts from the true A p
+  if (p.product && mode == kAfterIter) launch_ap(a, p, par);
   return TBSLA_HIP_SUCCESS;
 }
 
@@ -1053,7 +1245,7 @@ int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, dou
   const bool replace = p.replace > 0 && i > 0 && i % p.replace == 0;
   // a replacement step recomputes z from the true residual: K2 then needs
   // no z of its own
-  launch_k2(a, p, par, i > 0, p.fused && !replace);
+  launch_k2(a, p, par, i > 0, p.fused && !replace);  // product form: PRE 4 (t.t) or, replacing, 2
   if (replace) {
     TBSLA_TRY(true_residual(m, a, p, ax, par, kAfterIter));
   } else if (p.pre == 2 && !p.fused) {
@@ -1064,7 +1256,7 @@ int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, dou
     launch_residual(a, p, ax, par, kVerify, 0);
     if (p.pre == 2) {
       launch_gtr(a, p, par, kVerify);
-      launch_gz(a, p, par, kVerify);
+      if (!p.product) launch_gz(a, p, par, kVerify);
     }
     k_cg_vcheck<<<p.nblocks, kBlock, 0, p.s>>>(a, (i + 1) & 1);
   }
@@ -1107,7 +1299,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown preconditioner");
   if (o.precision != TBSLA_HIP_CG_FP64 && o.precision != TBSLA_HIP_CG_MIXED)
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown precision");
-  if (o.factor_form < TBSLA_HIP_FACTOR_AUTO || o.factor_form > TBSLA_HIP_FACTOR_FUSED)
+  if (o.factor_form < TBSLA_HIP_FACTOR_AUTO || o.factor_form > TBSLA_HIP_FACTOR_PRODUCT)
     return fail(TBSLA_HIP_ERR_INVALID, "unknown factor_form");
   tbsla_hip_matrix_s *G = o.factor;
   const int pre = o.preconditioner;
@@ -1208,6 +1400,40 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     }
     c.factor = G;
   }
+  // the product form: A G, when every row of A has at most 64 products
+  // (else the separate form)
+  bool product = false;
+  if (pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_PRODUCT) {
+    int *rp = nullptr, *ci = nullptr;
+    double *v = nullptr;
+    long long nnz = 0;
+    bool fits = false;
+    TBSLA_TRY(csr_product_arrays(m, G, &rp, &ci, &v, &nnz, &fits, s));
+    if (fits && nnz > 0) {
+      product = true;
+      if (c.ag_rp && c.ag_rows >= n && c.ag_cap >= nnz) {
+        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_rp, rp, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, s));
+        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_ci, ci, (size_t)nnz * 4, hipMemcpyDeviceToDevice, s));
+        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_v64, v, (size_t)nnz * 8, hipMemcpyDeviceToDevice, s));
+        TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+        hipFree(rp);
+        hipFree(ci);
+        hipFree(v);
+      } else {
+        c.drop_product();
+        c.ag_rp = rp;
+        c.ag_ci = ci;
+        c.ag_v64 = v;
+        c.ag_rows = n;
+        c.ag_cap = nnz;
+      }
+      c.ag_nnz = nnz;
+      if (mixed) {
+        if (!c.ag_v32) TBSLA_TRY(device_alloc((void **)&c.ag_v32, (size_t)c.ag_cap * 4));
+        k_to_f32<<<sgrid(nnz), kBlock, 0, s>>>(c.ag_v64, nnz, c.ag_v32);
+      }
+    }
+  }
 
   CgArgs a{};
   a.rowptr = m->rowptr;
@@ -1245,6 +1471,13 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.com = c.com.p;
   a.block_seg = c.block_seg.p;
 
+  if (product) {
+    a.ag_rp = c.ag_rp;
+    a.ag_ci = c.ag_ci;
+    a.ag_v64 = c.ag_v64;
+    a.ag_v32 = c.ag_v32;
+    a.ag_nnz = (int)c.ag_nnz;
+  }
   a.verify = (mixed || o.replace_every > 0) ? 1 : 0;
   a.stall_window = o.stall_window;
   // the prefetches of K1 and K2 read entry 0 of A's and G^T's arrays even
@@ -1283,10 +1516,19 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   p.verify = a.verify;
   p.stall = o.stall_window;
   // the fused form saves a kernel per iteration at the price of gathering
-  // each G^T row once per G entry; on MI355X that costs more than the kernel
-  // it saves (7-point Poisson, 64^3: 38 vs 37 us per iteration; 128^3: 286
-  // vs 216 us), so the automatic choice is the separate form
+  // each G^T row once per G entry, the product form reads A G's longer rows;
+  // on MI355X both cost more than the kernel they save (7-point Poisson,
+  // 64^3: 38 and 52 vs 31-37 us per iteration; 128^3: 286 and 410 vs 200-216
+  // us), so the automatic choice is the separate form
   p.fused = pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_FUSED;
+  p.product = product;
+  p.Lk1 = L;
+  if (product && o.lanes_per_row <= 0) {
+    // the same rule on A G's mean row length (at least L: the workgroup
+    // layout's passes of kBlock / L rows then hold whole passes of K1)
+    const double mean = (double)c.ag_nnz / std::max<long long>(n, 1);
+    while (p.Lk1 < 64 && p.Lk1 < mean / 8) p.Lk1 <<= 1;
+  }
   p.n = n;
   p.nblocks = nblocks;
   p.s = s;
@@ -1312,7 +1554,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   launch_residual(a, p, ax0, 0, kStart, o.use_initial_guess ? 0 : 1);
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
     launch_gtr(a, p, 0, kStart);
-    launch_gz(a, p, 0, kStart);
+    if (!p.product) launch_gz(a, p, 0, kStart);
   }
   // The reference clamps both divisors at an absolute 1e-10 (CG.cpp:35,63,69),
   // which stalls the iteration once r.r drops below it; it is kept for the
@@ -1351,12 +1593,14 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.spmv_values = m->values;
   key.spmv_split = m->split;
   key.L = p.L;
+  key.Lk1 = p.Lk1;
   key.pre = pre;
   key.mixed = mixed;
   key.replace = p.replace;
   key.batch = B;
   key.nblocks = nblocks;
   key.fused = p.fused ? 1 : 0;
+  key.product = p.product ? 1 : 0;
   key.spmv_lanes = pick_csr_lanes(m);
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
   key.spmv_vec = pick_csr_vec(m);

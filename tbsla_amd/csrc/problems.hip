@@ -324,6 +324,8 @@ void set_square(tbsla_hip_matrix_s *m, long long n) {
   m->row_sorted = 1;
 }
 
+}  // namespace
+
 // exclusive scan of n counts into rowptr[0..n], total into *total (host)
 int counts_to_rowptr(const int *cnt, int *rowptr, long long n, long long *total, hipStream_t s) {
   void *ws = nullptr;
@@ -346,8 +348,6 @@ int counts_to_rowptr(const int *cnt, int *rowptr, long long n, long long *total,
   *total = t;
   return TBSLA_HIP_SUCCESS;
 }
-
-}  // namespace
 
 // CSR transpose into freshly allocated arrays (rows of the result = local
 // columns of src; entries of a result row in increasing source row, ties in

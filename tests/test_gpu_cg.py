@@ -105,7 +105,56 @@ def test_factored_fused_and_separate_identical(sym, precision):
     assert abs(runs[0][1] - ito) <= 2
     assert rel_max(runs[0][0], xo) < 1e-7
     with pytest.raises(T.TbslaHipError):
-        g.cg(rhs, preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=3)
+        g.cg(rhs, preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=4)
+
+
+@pytest.fixture(scope="module")
+def poisson():
+    k = O.lognormal_field(16, 16, 16, 0.0, 1.0, 4.0, 7)
+    o = O.poisson3d(16, 16, 16, kappa=k)
+    return o, to_gpu(o)
+
+
+@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
+def test_factored_product_form(poisson, sym, precision):
+    """The product form (A G built per call; Ap' = (A G) t + beta Ap, r.z =
+    t.t) follows the oracle's G G^T PCG: FP64 early iterates within 1e-10,
+    convergence within 3 iterations of it, true residual below the tolerance
+    (mixed: with replacements every 7, which restart Ap from A p).  A matrix
+    whose rows have more than 64 products falls back to the separate form,
+    bitwise."""
+    o, g = poisson
+    n = o.geom["ln_row"]
+    Go = lower_factor(o)
+    G = to_gpu(Go)
+    rhs = 2 * O.unit_vector(19, n) - 1
+    if precision == T.CG_FP64:
+        for k in (1, 3, 8):
+            xo, ito, _ = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30, kind=2, G=Go)
+            xg, itg, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
+                              preconditioner=T.PRECOND_FACTORED, factor=G,
+                              factor_form=T.FACTOR_PRODUCT)
+            assert itg == ito == k
+            assert rel_max(xg, xo) < 1e-10, (k, rel_max(xg, xo))
+    rep = 7 if precision == T.CG_MIXED else 0
+    x, it, rn = g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+                     preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=T.FACTOR_PRODUCT)
+    xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=2, G=Go)
+    assert abs(it - ito) <= 3, (it, ito)
+    A = scipy_of(o)
+    assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+    x2, it2, _ = g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+                      preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=T.FACTOR_PRODUCT)
+    assert it2 == it and np.array_equal(x2, x)
+    # rows of the symmetrised cqmat have ~25 x 13 products: the separate form runs
+    so, sg = sym
+    Gs = to_gpu(lower_factor(so))
+    rs = 2 * O.unit_vector(20, so.geom["ln_row"]) - 1
+    a = sg.cg(rs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_PRODUCT)
+    b = sg.cg(rs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_SEPARATE)
+    assert a[1] == b[1] and np.array_equal(a[0], b[0])
 
 
 @pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI, T.PRECOND_FACTORED])
