@@ -189,6 +189,84 @@ def ppr_section():
             "note": "host clock around 20 iterations incl. per-iteration column sums"}
 
 
+def formats_section(iters: int = 20):
+    """The other cfg2 measurements: SpMV of the ELL, COO, SCOO blocks and of
+    the 16384^2 DENSE slice (block (0,0) of a 256 x 256 grid), the CSR blocks
+    (0,0) and (0,1) of a 4 x 4 grid, PageRank on the stochastic cfg2 CSR
+    (beta 0.85, epsilon 1e-10), and a 1 GiB device copy.  GB/s over each
+    format's algorithmic bytes (stored entries, x and y once); HIP events
+    around `iters` back-to-back SpMVs."""
+    import tbsla_amd as T
+    out = {}
+    dx = T.DeviceBuffer(N * 8)
+    T.check(T.lib().tbsla_hip_fill_unit_vector(dx.ptr, N, 1, None))
+    dy = T.DeviceBuffer(N * 8)
+
+    def timed(m, nbytes):
+        m.time_spmv(dy, dx, 2)
+        us = m.time_spmv(dy, dx, iters) / iters * 1e3
+        return {"us": round(us, 1), "GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+                "frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    for name, fmt in (("ELL", T.ELL), ("COO", T.COO), ("SCOO", T.SCOO)):
+        m = T.Matrix(fmt).fill_cqmat(N, N, C, Q, 0, value_seed=0)
+        g = m.geometry()
+        if fmt == T.ELL:
+            nbytes = 12 * g.ln_row * g.max_col + 8 * g.ln_col + 8 * g.ln_row
+        elif fmt == T.COO:
+            nbytes = 16 * g.stored + 8 * g.n_col + 8 * g.n_row
+        else:
+            nbytes = 16 * g.stored + 8 * g.ln_col + 8 * g.ln_row
+        out[name] = {"bytes": nbytes, **timed(m, nbytes)}
+        del m
+    m = T.Matrix(T.DENSE).fill_cqmat(N, N, C, Q, 0, 0, 0, 256, 256, value_seed=0)
+    g = m.geometry()
+    nbytes = 8 * g.ln_row * g.ln_col + 8 * g.ln_col + 8 * g.ln_row
+    out["DENSE_16384_slice"] = {"bytes": nbytes, **timed(m, nbytes)}
+    del m
+    for pc in (0, 1):
+        m = T.Matrix(T.CSR).fill_cqmat(N, N, C, Q, 0, 0, pc, 4, 4, value_seed=0)
+        g = m.geometry()
+        nbytes = algorithmic_bytes_csr(g.stored, g.ln_row, g.ln_col)
+        m.autotune(dy, dx, 5)
+        out[f"CSR_grid4x4_block0{pc}"] = {"stored": g.stored, "bytes": nbytes,
+                                          "kernel": m.get_csr_kernel(), **timed(m, nbytes)}
+        del m
+    # PageRank on the column-stochastic cfg2 CSR
+    m = T.Matrix(T.CSR).fill_cqmat(N, N, C, Q, 0, value_seed=0)
+    ds = T.DeviceBuffer(N * 8)
+    T.check(T.lib().tbsla_hip_make_stochastic(m.h, ds.ptr, None))
+    m.autotune(dy, dx, 5)
+    import ctypes
+    it, err = ctypes.c_int(), ctypes.c_double()
+    T.check(T.lib().tbsla_hip_page_rank(m.h, 0.85, 1e-10, 3, dy.ptr, ctypes.byref(it),
+                                        ctypes.byref(err), None))
+    t0 = time.perf_counter()
+    T.check(T.lib().tbsla_hip_page_rank(m.h, 0.85, 1e-10, 10000, dy.ptr, ctypes.byref(it),
+                                        ctypes.byref(err), None))
+    ms = (time.perf_counter() - t0) * 1e3
+    g = m.geometry()
+    spmv_bytes = algorithmic_bytes_csr(g.stored, g.ln_row, g.ln_col)
+    out["page_rank"] = {"iterations": it.value, "max_error": err.value, "ms": round(ms, 2),
+                        "ms_per_iteration": round(ms / max(it.value, 1), 4),
+                        "spmv_GBps_equivalent": round(spmv_bytes * it.value / (ms * 1e-3) / 1e9, 1),
+                        "note": "host clock around the whole solve (iterations, vector passes, "
+                                "final normalisation), vectors resident"}
+    del m, ds
+    # a plain device copy for reference (read + write)
+    B = 1 << 30
+    a, b = T.DeviceBuffer(B), T.DeviceBuffer(B)
+    T.check(T.lib().tbsla_hip_copy_kernel(b.ptr, a.ptr, B, None))
+    T.check(T.lib().tbsla_hip_stream_synchronize(None))
+    t0 = time.perf_counter()
+    for _ in range(10):
+        T.check(T.lib().tbsla_hip_copy_kernel(b.ptr, a.ptr, B, None))
+    T.check(T.lib().tbsla_hip_stream_synchronize(None))
+    us = (time.perf_counter() - t0) / 10 * 1e6
+    out["copy_1GiB"] = {"us": round(us, 1), "GBps_moved": round(2 * B / (us * 1e-6) / 1e9, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +279,8 @@ def main():
                     help="keep the default CSR schedule (row layout) instead of the autotuned one")
     ap.add_argument("--csr-kernel", default=None, metavar="VEC,LANES,ROWS,NT",
                     help="use this CSR row kernel (row layout) instead of the autotuned one")
+    ap.add_argument("--no-formats", action="store_true",
+                    help="skip the other formats, grid blocks, PageRank and copy figures")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-CG and personalised-PageRank sections")
     ap.add_argument("--batched-systems", type=int, default=512)
@@ -255,6 +335,8 @@ def main():
     del m, dx, dy
     cg = None if args.no_cg else cg_section(args)
     extra = {}
+    if not args.no_formats:
+        extra["cfg2_other"] = formats_section()
     if not args.no_batched:
         from tbsla_amd import batched
         extra["batched_cg"] = batched.bench(args.batched_systems)
