@@ -29,6 +29,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -1009,6 +1011,7 @@ struct GraphKey {
 
 // Work vectors, partials and graphs kept by a matrix handle between solves.
 struct CgCache {
+  int device = 0;
   Buf<double> b, x, r0, r1, p0, p1, z, ap, t64, dinv, ax;
   Buf<float> t32;
   Buf<double> ppap, prr0, prr1, prz0, prz1;
@@ -1046,9 +1049,22 @@ struct CgCache {
     ag_v32 = nullptr;
     ag_nnz = ag_cap = ag_rows = 0;
   }
-  // graphs: [0] plain batch, [1] batch ending with a replacement step
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  GraphKey key{};
+  // instantiated graphs, one set per plan shape (solver, precision,
+  // preconditioner, batch, layout), most recently used first: exec[0] a
+  // plain batch, exec[1] a batch ending with a replacement step
+  struct GraphSet {
+    GraphKey key;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    void destroy() {
+      for (auto &x : exec)
+        if (x) {
+          hipGraphExecDestroy(x);
+          x = nullptr;
+        }
+    }
+  };
+  static constexpr size_t kGraphSets = 8;
+  std::vector<GraphSet> graphs;
   std::vector<CgSeg> host_seg;
   std::vector<int> host_blocks;
 
@@ -1081,17 +1097,60 @@ struct CgCache {
     for (auto &e : ev)
       if (e) hipEventDestroy(e);
     if (own) hipStreamDestroy(own);
-    for (auto &x : exec)
-      if (x) hipGraphExecDestroy(x);
+    for (auto &g : graphs) g.destroy();
     drop_factor();
   }
 };
 
-void cg_cache_release(tbsla_hip_matrix_s *m) {
-  if (m->cg_cache) {
-    delete static_cast<CgCache *>(m->cg_cache);
-    m->cg_cache = nullptr;
+// Caches of destroyed handles, kept for the next handles' solves: their
+// stream, events, pinned status words, work vectors and instantiated graphs
+// are reused (creating them costs 1-2 ms per new matrix, a third of a 64^3
+// solve).  At most kSpareCaches, per device, under a mutex; never freed at
+// process exit (the runtime may be gone by then).
+constexpr size_t kSpareCaches = 2;
+
+std::mutex &spare_mutex() {
+  static std::mutex *mu = new std::mutex;
+  return *mu;
+}
+std::vector<CgCache *> &spare_caches() {
+  static auto *v = new std::vector<CgCache *>;
+  return *v;
+}
+
+CgCache *take_cache() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  {
+    std::lock_guard<std::mutex> g(spare_mutex());
+    auto &v = spare_caches();
+    for (size_t k = v.size(); k-- > 0;) {
+      if (v[k]->device == dev) {
+        CgCache *c = v[k];
+        v.erase(v.begin() + (long)k);
+        return c;
+      }
+    }
   }
+  CgCache *c = new CgCache();
+  c->device = dev;
+  return c;
+}
+
+void cg_cache_release(tbsla_hip_matrix_s *m) {
+  if (!m->cg_cache) return;
+  CgCache *c = static_cast<CgCache *>(m->cg_cache);
+  m->cg_cache = nullptr;
+  if (c->own) hipStreamSynchronize(c->own);
+  c->drop_factor();  // arrays derived from the old factor
+  {
+    std::lock_guard<std::mutex> g(spare_mutex());
+    if (spare_caches().size() < kSpareCaches) {
+      spare_caches().push_back(c);
+      return;
+    }
+  }
+  delete c;
 }
 
 int ensure_values32(tbsla_hip_matrix_s *m, hipStream_t s) {
@@ -1277,7 +1336,7 @@ int cg_solve(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h_ac
              const double *d_rhs, double *d_x, const tbsla_hip_cg_options &o,
              tbsla_hip_cg_result *res, hipStream_t s) {
   if (s != nullptr) return cg_solve_on(m, nsys, h_rows, h_active, d_rhs, d_x, o, res, s);
-  if (!m->cg_cache) m->cg_cache = new CgCache();
+  if (!m->cg_cache) m->cg_cache = take_cache();
   CgCache &c = *static_cast<CgCache *>(m->cg_cache);
   if (!c.own) TBSLA_HIP_CHECK(hipStreamCreateWithFlags(&c.own, hipStreamNonBlocking));
   TBSLA_HIP_CHECK(hipDeviceSynchronize());  // earlier null-stream work
@@ -1312,7 +1371,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   const int max_it = o.max_iterations;
   if (max_it < 0) return fail(TBSLA_HIP_ERR_INVALID, "max_iterations < 0");
 
-  if (!m->cg_cache) m->cg_cache = new CgCache();
+  if (!m->cg_cache) m->cg_cache = take_cache();
   CgCache &c = *static_cast<CgCache *>(m->cg_cache);
 
   // lanes per row of the iteration's SpMV: a power of two near an eighth of
@@ -1605,12 +1664,34 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
   key.spmv_vec = pick_csr_vec(m);
   key.spmv_nt = pick_csr_nt(m);
-  if (B > 0 && !(c.exec[0] && c.key == key)) {
-    for (auto &x : c.exec)
-      if (x) {
-        hipGraphExecDestroy(x);
-        x = nullptr;
+  // the graph set of this plan: an exact match is replayed as it is; one of
+  // the same shape (other pointers) is updated in place, which costs a
+  // fraction of an instantiation; else a new set is instantiated
+  CgCache::GraphSet *gs = nullptr;
+  if (B > 0) {
+    auto shape_of = [](const GraphKey &k) {
+      return std::make_tuple(k.L, k.Lk1, k.pre, k.mixed, k.replace, k.batch, k.nblocks, k.fused,
+                             k.product, k.args.verify);
+    };
+    size_t pos = c.graphs.size();
+    for (size_t q = 0; q < c.graphs.size() && pos == c.graphs.size(); q++)
+      if (c.graphs[q].key == key) pos = q;
+    for (size_t q = 0; q < c.graphs.size() && pos == c.graphs.size(); q++)
+      if (shape_of(c.graphs[q].key) == shape_of(key)) pos = q;
+    if (pos == c.graphs.size()) {
+      if (c.graphs.size() == CgCache::kGraphSets) {
+        c.graphs.back().destroy();
+        c.graphs.pop_back();
       }
+      c.graphs.insert(c.graphs.begin(), CgCache::GraphSet{});
+    } else if (pos != 0) {
+      CgCache::GraphSet moved = c.graphs[pos];
+      c.graphs.erase(c.graphs.begin() + (long)pos);
+      c.graphs.insert(c.graphs.begin(), moved);
+    }
+    gs = &c.graphs.front();
+  }
+  if (B > 0 && !(gs->exec[0] && gs->key == key)) {
     const int variants = p.replace > 0 ? 2 : 1;
     for (int v = 0; v < variants; v++) {
       // variant v: iterations 1..B; with v == 1 the last one replaces (its
@@ -1629,14 +1710,26 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
         return st;
       }
       if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&c.exec[v], graph, nullptr, nullptr, 0);
+      if (gs->exec[v]) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult ur;
+        if (hipGraphExecUpdate(gs->exec[v], graph, &err_node, &ur) == hipSuccess &&
+            ur == hipGraphExecUpdateSuccess) {
+          hipGraphDestroy(graph);
+          continue;
+        }
+        (void)hipGetLastError();  // the failed update's status
+        hipGraphExecDestroy(gs->exec[v]);
+        gs->exec[v] = nullptr;
+      }
+      e = hipGraphInstantiate(&gs->exec[v], graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e != hipSuccess) {
-        c.exec[v] = nullptr;
+        gs->exec[v] = nullptr;
         return fail(TBSLA_HIP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
       }
     }
-    c.key = key;
+    gs->key = key;
   }
   const int poll_every = B > 0 ? B : 16;
 
@@ -1649,7 +1742,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     if (B > 0 && it + B <= max_it) {
       const int last = it + B - 1;  // the batch's last iteration
       const bool rep = p.replace > 0 && last % p.replace == 0;
-      TBSLA_HIP_CHECK(hipGraphLaunch(c.exec[rep ? 1 : 0], s));
+      TBSLA_HIP_CHECK(hipGraphLaunch(gs->exec[rep ? 1 : 0], s));
       it += B;
     } else {
       const int stop = std::min(max_it, it + poll_every);

@@ -87,11 +87,14 @@ class Solver:
     """CG solves on one matrix with device vectors; times with the host
     clock around the library calls (which end synchronised)."""
 
-    def __init__(self, A: T.Matrix, rhs: T.DeviceBuffer, net: T.PrecondNet | None = None):
+    def __init__(self, A: T.Matrix, rhs: T.DeviceBuffer, net: T.PrecondNet | None = None,
+                 factors: "tuple[T.Matrix, T.Matrix] | None" = None):
+        """factors: the (G, G^T) handles the network writes into; passing the
+        same pair for every problem of one size reuses their arrays."""
         self.A, self.rhs, self.net = A, rhs, net
         self.n = A.geometry().ln_row
         self.x = T.DeviceBuffer(self.n * 8)
-        self.G, self.GT = T.Matrix(T.CSR), T.Matrix(T.CSR)
+        self.G, self.GT = factors if factors is not None else (T.Matrix(T.CSR), T.Matrix(T.CSR))
         self.factor_form = T.FACTOR_AUTO
 
     def run(self, method: str, tol: float = 1e-8, max_iterations: int = 20000) -> dict:
@@ -132,9 +135,10 @@ class Solver:
 def evaluate(net, side: int, seeds, methods=METHODS, tol: float = 1e-8, warm: bool = True,
              log=None) -> list[dict]:
     rows = []
+    factors = (T.Matrix(T.CSR), T.Matrix(T.CSR))  # one pair for all problems of this size
     for k, sd in enumerate(seeds):
         A, rhs = poisson_problem(side, sd)
-        s = Solver(A, rhs, net)
+        s = Solver(A, rhs, net, factors)
         for m in methods:
             if net is None and m.startswith("learned"):
                 continue

@@ -157,6 +157,35 @@ def test_factored_product_form(poisson, sym, precision):
     assert a[1] == b[1] and np.array_equal(a[0], b[0])
 
 
+@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
+def test_cache_of_destroyed_handle_reused(precision):
+    """A destroyed handle's CG state (stream, vectors, captured graphs) serves
+    the next handles: graphs of the same shape are updated in place with the
+    new matrix's pointers, others re-instantiated.  Every solve still follows
+    the oracle."""
+    import gc
+    rep = 10 if precision == T.CG_MIXED else 0
+    for side, seed in ((12, 1), (12, 2), (14, 3), (12, 4)):
+        k = O.lognormal_field(side, side, side, 0.0, 1.0, 4.0, seed)
+        o = O.poisson3d(side, side, side, kappa=k)
+        g = to_gpu(o)
+        n = o.geom["ln_row"]
+        rhs = 2 * O.unit_vector(30 + seed, n) - 1
+        if precision == T.CG_FP64:
+            xo, ito, _ = O.pcg(o, rhs, max_iterations=6, tolerance=1e-30, kind=1)
+            xg, itg, _ = g.cg(rhs, max_iterations=6, tolerance=1e-30, relative=True,
+                              preconditioner=T.PRECOND_JACOBI)
+            assert itg == ito == 6 and rel_max(xg, xo) < 1e-12
+        x, it, rn = g.cg(rhs, tolerance=1e-10, relative=True, precision=precision,
+                         replace_every=rep, preconditioner=T.PRECOND_JACOBI)
+        xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=1)
+        assert abs(it - ito) <= 2
+        A = scipy_of(o)
+        assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+        del g
+        gc.collect()
+
+
 @pytest.mark.parametrize("pre", [T.PRECOND_NONE, T.PRECOND_JACOBI, T.PRECOND_FACTORED])
 def test_converged_iterations_like_oracle(sym, pre):
     o, g = sym
