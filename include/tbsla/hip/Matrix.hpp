@@ -16,11 +16,15 @@
 #ifndef TBSLA_HIP_MATRIX_HPP
 #define TBSLA_HIP_MATRIX_HPP
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../tbsla_hip.h"
@@ -133,6 +137,46 @@ class Matrix {
     check(tbsla_hip_normalize_cols(h_, ds.get(), nullptr));
   }
 
+  // ---- row operations (Matrix.hpp:46-51) ---------------------------------
+  // The buffer is indexed as the reference's format does: ln_row entries
+  // (local rows) for CSR and DENSE, n_row entries (global rows) for ELL and
+  // SCOO (MatrixELL.cpp:790, MatrixSCOO.cpp:703); COO leaves it alone.
+  // SCOO accumulates into the buffer; the others overwrite their rows.
+  void get_row_sums(double *s) {
+    const tbsla_hip_geometry g = geometry();
+    if (g.format == TBSLA_HIP_COO) return;
+    const size_t n = row_sums_len(g);
+    DeviceArray<double> ds(s, n);
+    check(tbsla_hip_get_row_sums(h_, ds.get(), nullptr));
+    ds.to_host(s, n);
+  }
+  void normalize_rows(double *s) {
+    const tbsla_hip_geometry g = geometry();
+    if (g.format == TBSLA_HIP_COO) return;
+    DeviceArray<double> ds(s, row_sums_len(g));
+    check(tbsla_hip_normalize_rows(h_, ds.get(), nullptr));
+  }
+  // CSR only (MatrixCSR.cpp:1285-1334); the other formats leave the
+  // arguments and values untouched, as the reference's empty bodies do.
+  void get_row_max_abs(double *max_abs) {
+    const tbsla_hip_geometry g = geometry();
+    if (g.format != TBSLA_HIP_CSR) return;
+    const size_t n = (size_t)g.ln_row;
+    DeviceArray<double> dm(n);
+    check(tbsla_hip_get_row_max_abs(h_, dm.get(), nullptr));
+    dm.to_host(max_abs, n);
+  }
+  void apply_exponential(double *max_abs, int base = -1) {
+    const tbsla_hip_geometry g = geometry();
+    if (g.format != TBSLA_HIP_CSR) return;
+    if (!max_abs) {
+      check(tbsla_hip_apply_exponential(h_, nullptr, base, nullptr));
+      return;
+    }
+    DeviceArray<double> dm(max_abs, (size_t)g.ln_row);
+    check(tbsla_hip_apply_exponential(h_, dm.get(), base, nullptr));
+  }
+
   // ---- drivers (src/mpi/include/tbsla/mpi/Matrix.hpp:31-33), one block ---
   double *page_rank(double beta, double epsilon, int max_iterations, int &nb_iterations_done) {
     const size_t n = (size_t)geometry().ln_col;
@@ -177,6 +221,26 @@ class Matrix {
     check(tbsla_hip_fill_cqmat(h_, n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed,
                                nullptr));
   }
+
+  // Matrix.hpp:43-44.  The reference's generators for these draw their
+  // entries per OpenMP thread, so their output depends on the thread count
+  // (MatrixCSR.cpp:906-925) and has no fixed result to reproduce; COO and
+  // DENSE implement them as empty stubs (MatrixCOO.cpp:356-379).  Here they
+  // throw, as tbsla::mpi::Matrix does for operations it does not provide.
+  void fill_random(int, int, double, unsigned int = 1, int = 0, int = 0, int = 1, int = 1) {
+    throw std::runtime_error("tbsla_hip: fill_random is not provided (thread-count-dependent "
+                             "generator); use fill_cqmat or read");
+  }
+  void fill_brain(int, int, int *, std::vector<std::vector<double> >,
+                  std::vector<std::unordered_map<int, std::vector<int> > >, unsigned int,
+                  int = 0, int = 0, int = 1, int = 1) {
+    throw std::runtime_error("tbsla_hip: fill_brain is not provided (thread-count-dependent "
+                             "generator); use fill_cqmat or read");
+  }
+  // Matrix.hpp:39.  First-touch placement of host arrays over NUMA nodes
+  // (MatrixCSR.cpp:1234-1268) has no counterpart for arrays resident in
+  // device memory: a no-op.
+  void NUMAinit() {}
 
   // ---- getters (Matrix.hpp:21-32) ----------------------------------------
   int get_n_row() const { return geometry().n_row; }
@@ -303,6 +367,54 @@ class Matrix {
     return os << "\n";
   }
 
+  // Matrix.hpp:37: the block as an n_row x n_col dense array, printed as
+  // tbsla::utils::array::print_dense_matrix does (src/cpp/utils/array.cpp:
+  // 85-104: fixed notation, precision = the widest integer part, width
+  // precision + 3, two spaces after each value).  Entries sit at their
+  // global row and column; DENSE prints its description instead, as
+  // MatrixDENSE.cpp:59-61 does.
+  std::ostream &print_as_dense(std::ostream &os) {
+    const tbsla_hip_geometry g = geometry();
+    if (g.format == TBSLA_HIP_DENSE) return print(os);
+    std::vector<int> a, b;
+    std::vector<double> v;
+    exported(g, a, b, v);
+    const size_t nr = (size_t)g.n_row, nc = (size_t)g.n_col;
+    std::vector<double> d(nr * nc, 0.0);
+    auto put = [&](long long r, long long c, double x) {
+      if (r >= 0 && r < (long long)nr && c >= 0 && c < (long long)nc) d[r * nc + c] += x;
+    };
+    switch (g.format) {
+      case TBSLA_HIP_CSR:
+        for (int i = 0; i < g.ln_row; i++)
+          for (int j = a[i]; j < a[i + 1]; j++) put((long long)g.f_row + i, b[j], v[j]);
+        break;
+      case TBSLA_HIP_ELL:
+        for (int i = 0; i < g.ln_row; i++)
+          for (int j = 0; j < g.max_col; j++) {
+            const size_t k = (size_t)i * g.max_col + j;
+            if (k < v.size() && v[k] != 0.0) put((long long)g.f_row + i, a[k], v[k]);
+          }
+        break;
+      default:  // COO, SCOO: global triplets
+        for (size_t k = 0; k < v.size(); k++) put(a[k], b[k], v[k]);
+        break;
+    }
+    int max_p = 1;
+    for (double x : d) {
+      if (x == 0.0 || !std::isfinite(x)) continue;  // no integer part to widen for
+      const int lr = (int)(std::log10(std::fabs(x)) + 1);
+      if (lr > max_p) max_p = lr;
+    }
+    os << std::fixed;
+    os.precision(max_p);
+    for (size_t i = 0; i < nr; i++) {
+      for (size_t j = 0; j < nc; j++) os << std::setw(max_p + 3) << d[i * nc + j] << "  ";
+      os << std::endl;
+    }
+    return os;
+  }
+
   tbsla_hip_geometry geometry() const {
     tbsla_hip_geometry g{};
     check(tbsla_hip_get_geometry(h_, &g));
@@ -313,6 +425,9 @@ class Matrix {
  protected:
   tbsla_hip_matrix_t h_ = nullptr;
 
+  static size_t row_sums_len(const tbsla_hip_geometry &g) {
+    return (size_t)(g.format == TBSLA_HIP_ELL || g.format == TBSLA_HIP_SCOO ? g.n_row : g.ln_row);
+  }
   static size_t in_len(const tbsla_hip_geometry &g) { return (size_t)g.ln_col; }
   static size_t out_len(const tbsla_hip_geometry &g) {
     return (size_t)(g.format == TBSLA_HIP_COO ? g.n_row : g.ln_row);
@@ -362,6 +477,32 @@ class MatrixCSR : public Matrix {
   MatrixCSR() : Matrix(TBSLA_HIP_CSR) {}
   explicit MatrixCSR(const MatrixCOO &m) : Matrix(TBSLA_HIP_CSR) {
     check(tbsla_hip_convert(h_, TBSLA_HIP_CSR, m.handle(), nullptr));
+  }
+  // MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283, block form
+  // src/mpi/matrix/MatrixCSR.cpp:11-30): C = A B with B ln_col x cols_B and
+  // C ln_row x cols_B, row-major host arrays; C is overwritten.
+  void dense_multiply(const double *B, double *C, int cols_B) const {
+    const tbsla_hip_geometry g = geometry();
+    const size_t nb = (size_t)g.ln_col * (size_t)cols_B, nc = (size_t)g.ln_row * (size_t)cols_B;
+    DeviceArray<double> dB(B, nb), dC(nc);
+    check(tbsla_hip_spmm(h_, dB.get(), dC.get(), cols_B, nullptr));
+    dC.to_host(C, nc);
+  }
+  void dense_multiply_device(const double *d_B, double *d_C, int cols_B,
+                             void *stream = nullptr) const {
+    check(tbsla_hip_spmm(h_, d_B, d_C, cols_B, stream));
+  }
+  // The attention step's row softmax fused into one pass
+  // (tbsla_attention_mpi.cpp:183-191): the result of get_row_max_abs,
+  // apply_exponential(max_abs, base), get_row_sums and normalize_rows in
+  // turn.  max_abs and s (ln_row doubles each, nullable) receive the row
+  // maxima and sums.
+  void softmax_rows(int base = -1, double *max_abs = nullptr, double *s = nullptr) {
+    const size_t n = (size_t)geometry().ln_row;
+    DeviceArray<double> dm(n), ds(n);
+    check(tbsla_hip_softmax_rows(h_, base, dm.get(), ds.get(), nullptr));
+    if (max_abs) dm.to_host(max_abs, n);
+    if (s) ds.to_host(s, n);
   }
 };
 class MatrixELL : public Matrix {

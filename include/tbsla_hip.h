@@ -205,6 +205,43 @@ int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *stream);
 int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s, void *stream);
 int tbsla_hip_make_stochastic(tbsla_hip_matrix_t m, double *d_s, void *stream);
 
+/* ---- row operations ------------------------------------------------------ */
+/* Matrix::get_row_sums / normalize_rows (Matrix.hpp:46-47).  Each format
+ * keeps the reference's indexing, which differs between formats:
+ *   CSR   (MatrixCSR.cpp:1147-1190): d_s[i], i < ln_row, is set to row i's
+ *         sum; normalize divides by d_s[i], or sets 0 where d_s[i] == 0;
+ *   ELL   (MatrixELL.cpp:783-801): d_s[f_row + i] is set to the sum over
+ *         all max_col slots (padding included); normalize divides every
+ *         slot by it;
+ *   SCOO  (MatrixSCOO.cpp:701-711): d_s[row[k]] += values[k] (global row
+ *         index, not zeroed first); normalize divides values[k] by
+ *         d_s[row[k]];
+ *   DENSE (MatrixDENSE.cpp:292-310): d_s[i] set; normalize divides by it;
+ *   COO   (MatrixCOO.cpp:382-388): nothing.
+ * So d_s holds ln_row doubles for CSR and DENSE and n_row (at least
+ * f_row + ln_row) for ELL and SCOO.  Sums follow the reference's serial
+ * loop order and are bit-identical to it. */
+int tbsla_hip_get_row_sums(tbsla_hip_matrix_t m, double *d_s, void *stream);
+int tbsla_hip_normalize_rows(tbsla_hip_matrix_t m, const double *d_s, void *stream);
+/* Matrix::get_row_max_abs / apply_exponential (Matrix.hpp:50-51); only CSR
+ * acts (MatrixCSR.cpp:1285-1334), the other formats are no-ops as in the
+ * reference.  get_row_max_abs: d_max_abs[i] (ln_row doubles) = the largest
+ * |z| over row i's nonzero values, 0 for an empty row.  apply_exponential:
+ * every nonzero z of row i becomes exp(z - d_max_abs[i]) when base <= 0,
+ * pow(base, z - d_max_abs[i]) otherwise; with d_max_abs null, exp(z) or
+ * pow(base, z).  Device exp/pow are within a few ulp of the C library's. */
+int tbsla_hip_get_row_max_abs(tbsla_hip_matrix_t m, double *d_max_abs, void *stream);
+int tbsla_hip_apply_exponential(tbsla_hip_matrix_t m, const double *d_max_abs, int base,
+                                void *stream);
+/* The attention step's row softmax (tbsla_attention_mpi.cpp:183-191) on one
+ * CSR block, fused: the same result, bit for bit, as get_row_max_abs,
+ * apply_exponential(max_abs, base), get_row_sums and normalize_rows called
+ * in turn (the row reductions over the process row are identities on one
+ * block), in one pass over the values.  d_max_abs and d_s (ln_row doubles
+ * each, nullable) receive the row maxima and the row sums.  CSR only. */
+int tbsla_hip_softmax_rows(tbsla_hip_matrix_t m, int base, double *d_max_abs, double *d_s,
+                           void *stream);
+
 /* tbsla::mpi::Matrix::page_rank (src/mpi/matrix/Matrix.cpp:540-602) on this
  * block: b starts at 1; per iteration y = A b_t, b = (beta y + S(1-beta)/n)/S
  * with S = sum(b_t), error = sum|b - b_t| / NC; stops when error < epsilon
@@ -325,7 +362,7 @@ int tbsla_hip_precond_net_train_step(tbsla_hip_precond_net_t net, tbsla_hip_matr
 /* ---- SpMM and personalised PageRank ------------------------------------- */
 /* MatrixCSR::dense_multiply (MatrixCSR.cpp:1270-1283; block form
  * src/mpi/matrix/MatrixCSR.cpp:11-30): C = A B, B ln_col x k and C
- * ln_row x k row-major device arrays, k <= 256.  CSR only.  Products are
+ * ln_row x k row-major device arrays, any k >= 1.  CSR only.  Products are
  * accumulated in the reference's order without fused multiply-add, so C is
  * bit-identical to a serial run of the reference loop. */
 int tbsla_hip_spmm(tbsla_hip_matrix_t m, const double *d_B, double *d_C, int k, void *stream);

@@ -112,6 +112,10 @@ def lib() -> C.CDLL:
         _lib.orc_get_col_sums.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_normalize_cols.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_make_stochastic.argtypes = [C.c_int, C.c_void_p, dp]
+        _lib.orc_get_row_sums.argtypes = [C.c_int, C.c_void_p, dp]
+        _lib.orc_normalize_rows.argtypes = [C.c_int, C.c_void_p, dp]
+        _lib.orc_get_row_max_abs.argtypes = [C.c_int, C.c_void_p, dp]
+        _lib.orc_apply_exponential.argtypes = [C.c_int, C.c_void_p, dp, C.c_int]
         _lib.orc_page_rank.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, dp,
                                        dp]
         _lib.orc_page_rank_source.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
@@ -234,6 +238,17 @@ class Matrix:
         return (_arr(s.values, n, np.float64).reshape(g.ln_row, g.ln_col) if n else
                 np.zeros((g.ln_row, g.ln_col)),)
 
+    def set_values(self, v) -> None:
+        """Overwrite the value array in place (same layout as arrays())."""
+        cur = self.arrays()[-1]
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == cur.size, (v.size, cur.size)
+        if not self.s.values:  # a block without arrays reads as zeros
+            assert not np.any(v)
+            return
+        if v.size:
+            np.ctypeslib.as_array(self.s.values, shape=(v.size,))[:] = v
+
     @property
     def result_len(self) -> int:
         return lib().orc_result_len(self.fmt, self.ptr)
@@ -279,6 +294,37 @@ class Matrix:
         s = np.zeros(max(self.s.g.ln_col, 1))
         lib().orc_get_col_sums(self.fmt, self.ptr, _dp(s))
         return s[: self.s.g.ln_col]
+
+    def row_sums_len(self) -> int:
+        g = self.s.g
+        return g.n_row if self.fmt in (ELL, SCOO) else g.ln_row
+
+    def get_row_sums(self, s=None) -> np.ndarray:
+        """SCOO accumulates into ``s`` (zeros if None); a copy is returned."""
+        n = self.row_sums_len()
+        out = np.zeros(max(n, 1)) if s is None else np.array(s, dtype=np.float64, copy=True)
+        assert out.size >= n
+        lib().orc_get_row_sums(self.fmt, self.ptr, _dp(out))
+        return out[:n] if s is None else out
+
+    def normalize_rows(self, s) -> None:
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        if s.size == 0:
+            s = np.zeros(1)
+        lib().orc_normalize_rows(self.fmt, self.ptr, _dp(s))
+
+    def get_row_max_abs(self) -> np.ndarray:
+        n = self.s.g.ln_row
+        m = np.zeros(max(n, 1))
+        lib().orc_get_row_max_abs(self.fmt, self.ptr, _dp(m))
+        return m[:n]
+
+    def apply_exponential(self, max_abs=None, base: int = -1) -> None:
+        if max_abs is None:
+            lib().orc_apply_exponential(self.fmt, self.ptr, None, base)
+        else:
+            m = np.ascontiguousarray(max_abs, dtype=np.float64)
+            lib().orc_apply_exponential(self.fmt, self.ptr, _dp(m), base)
 
     def page_rank(self, beta=0.85, epsilon=1e-10, max_iterations=10000):
         b = np.zeros(max(self.s.g.ln_col, 1))
@@ -468,6 +514,10 @@ def ref() -> C.CDLL:
         _ref.ref_AAxpAx.argtypes = [vp, dp, dp, dp, C.c_int]
         _ref.ref_AAxpAxpx.argtypes = [vp, dp, dp, dp, C.c_int]
         _ref.ref_make_stochastic.argtypes = [vp, dp]
+        _ref.ref_get_row_sums.argtypes = [vp, dp]
+        _ref.ref_normalize_rows.argtypes = [vp, dp]
+        _ref.ref_get_row_max_abs.argtypes = [vp, dp]
+        _ref.ref_apply_exponential.argtypes = [vp, dp, C.c_int]
         _ref.ref_cqmat_value.argtypes = [C.c_ulong] * 4 + [C.c_double, C.c_uint,
                                                            C.POINTER(C.c_long),
                                                            C.POINTER(C.c_long)]
@@ -557,6 +607,35 @@ class RefMatrix:
         s = np.zeros(max(ln_col, 1))
         ref().ref_make_stochastic(self.h, _dp(s))
         return s[:ln_col]
+
+    def set_values(self, v) -> None:
+        """Overwrite the object's value array in place (layout of arrays())."""
+        cur = self.arrays()[-1]
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == cur.size, (v.size, cur.size)
+        if v.size:
+            np.ctypeslib.as_array(self._ptrs()[0], shape=(v.size,))[:] = v
+
+    def get_row_sums(self, s: np.ndarray) -> np.ndarray:
+        out = np.array(s, dtype=np.float64, copy=True)
+        ref().ref_get_row_sums(self.h, _dp(out))
+        return out
+
+    def normalize_rows(self, s: np.ndarray) -> None:
+        s = np.array(s, dtype=np.float64, copy=True)
+        ref().ref_normalize_rows(self.h, _dp(s))
+
+    def get_row_max_abs(self, n: int) -> np.ndarray:
+        m = np.zeros(max(n, 1))
+        ref().ref_get_row_max_abs(self.h, _dp(m))
+        return m[:n]
+
+    def apply_exponential(self, max_abs=None, base: int = -1) -> None:
+        if max_abs is None:
+            ref().ref_apply_exponential(self.h, None, base)
+        else:
+            m = np.array(max_abs, dtype=np.float64, copy=True)
+            ref().ref_apply_exponential(self.h, _dp(m), base)
 
     def write(self) -> bytes:
         p = C.POINTER(C.c_char)()

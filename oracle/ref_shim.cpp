@@ -208,6 +208,27 @@ void ref_make_stochastic(void *h, double *s) {
   static_cast<Matrix *>(h)->make_stochastic(s);
 }
 
+// Row operations (Matrix.hpp:46-51), through the virtual interface.
+void ref_get_row_sums(void *h, double *s) {
+  Quiet quiet;
+  static_cast<Matrix *>(h)->get_row_sums(s);
+}
+
+void ref_normalize_rows(void *h, double *s) {
+  Quiet quiet;
+  static_cast<Matrix *>(h)->normalize_rows(s);
+}
+
+void ref_get_row_max_abs(void *h, double *m) {
+  Quiet quiet;
+  static_cast<Matrix *>(h)->get_row_max_abs(m);
+}
+
+void ref_apply_exponential(void *h, double *m, int base) {
+  Quiet quiet;
+  static_cast<Matrix *>(h)->apply_exponential(m, base);
+}
+
 // values_generation::cqmat_value, for single-index checks.
 void ref_cqmat_value(unsigned long k, unsigned long nr, unsigned long nc, unsigned long c,
                      double q, unsigned seed_mult, long *row, long *col) {

@@ -991,6 +991,111 @@ void orc_make_stochastic(int fmt, void *m, double *s) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Row operations                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* get_row_sums: CSR (MatrixCSR.cpp:1147-1163: s[i] for local rows, zeroed
+ * first), ELL (MatrixELL.cpp:783-792: s[i + f_row], every slot), SCOO
+ * (MatrixSCOO.cpp:701-705: s[row[k]] += values[k]), DENSE
+ * (MatrixDENSE.cpp:292-301: s[i]); COO a no-op (MatrixCOO.cpp:382-384).
+ * A block without arrays reads as all zeros. */
+void orc_get_row_sums(int fmt, const void *m, double *s) {
+  if (fmt == ORC_CSR) {
+    const orc_csr *a = (const orc_csr *)m;
+    for (int i = 0; i < a->g.ln_row; i++) s[i] = 0.0;
+    if (!a->rowptr) return;
+    for (int i = 0; i < a->g.ln_row; i++) {
+      double sum = 0;
+      for (int j = a->rowptr[i]; j < a->rowptr[i + 1]; j++) sum += a->values[j];
+      s[i] = sum;
+    }
+  } else if (fmt == ORC_ELL) {
+    const orc_ell *a = (const orc_ell *)m;
+    for (int i = 0; i < a->g.ln_row; i++) {
+      double sum = 0;
+      if (a->values)
+        for (int j = 0; j < a->max_col; j++) sum += a->values[(size_t)i * a->max_col + j];
+      s[i + a->g.f_row] = sum;
+    }
+  } else if (fmt == ORC_SCOO) {
+    const orc_coo *a = (const orc_coo *)m;
+    for (long k = 0; k < a->g.nnz; k++) s[a->row[k]] += a->values[k];
+  } else if (fmt == ORC_DENSE) {
+    const orc_dense *a = (const orc_dense *)m;
+    for (int i = 0; i < a->g.ln_row; i++) {
+      double sum = 0;
+      if (a->values)
+        for (int j = 0; j < a->g.ln_col; j++) sum += a->values[(size_t)i * a->g.ln_col + j];
+      s[i] = sum;
+    }
+  }
+}
+
+/* normalize_rows: CSR (MatrixCSR.cpp:1177-1190: divide, or 0 where the sum
+ * is 0), ELL (MatrixELL.cpp:794-801: every slot by s[i + f_row]), SCOO
+ * (MatrixSCOO.cpp:707-711), DENSE (MatrixDENSE.cpp:303-310); COO a no-op. */
+void orc_normalize_rows(int fmt, void *m, const double *s) {
+  if (fmt == ORC_CSR) {
+    orc_csr *a = (orc_csr *)m;
+    if (!a->rowptr) return;
+    for (int i = 0; i < a->g.ln_row; i++)
+      for (int j = a->rowptr[i]; j < a->rowptr[i + 1]; j++) {
+        if (s[i] != 0) a->values[j] /= s[i];
+        else a->values[j] = 0;
+      }
+  } else if (fmt == ORC_ELL) {
+    orc_ell *a = (orc_ell *)m;
+    if (!a->values) return;
+    for (int i = 0; i < a->g.ln_row; i++)
+      for (int j = 0; j < a->max_col; j++) a->values[(size_t)i * a->max_col + j] /= s[i + a->g.f_row];
+  } else if (fmt == ORC_SCOO) {
+    orc_coo *a = (orc_coo *)m;
+    for (long k = 0; k < a->g.nnz; k++) a->values[k] /= s[a->row[k]];
+  } else if (fmt == ORC_DENSE) {
+    orc_dense *a = (orc_dense *)m;
+    if (!a->values) return;
+    for (int i = 0; i < a->g.ln_row; i++)
+      for (int j = 0; j < a->g.ln_col; j++) a->values[(size_t)i * a->g.ln_col + j] /= s[i];
+  }
+}
+
+/* get_row_max_abs: CSR only (MatrixCSR.cpp:1285-1315): the largest |z| over
+ * the nonzero values of each row, 0 for an empty row; std::max keeps its
+ * first argument when the second is NaN, as the comparison below.  Every
+ * other format is a no-op (MatrixELL.cpp:835, MatrixSCOO.cpp:740,
+ * MatrixCOO.cpp:398, MatrixDENSE.cpp:320). */
+void orc_get_row_max_abs(int fmt, const void *m, double *max_abs) {
+  if (fmt != ORC_CSR) return;
+  const orc_csr *a = (const orc_csr *)m;
+  for (int i = 0; i < a->g.ln_row; i++) max_abs[i] = 0.0;
+  if (!a->rowptr) return;
+  for (int i = 0; i < a->g.ln_row; i++) {
+    double mx = 0.0;
+    for (int j = a->rowptr[i]; j < a->rowptr[i + 1]; j++) {
+      double z = a->values[j];
+      if (z != 0) mx = (mx < fabs(z)) ? fabs(z) : mx;
+    }
+    max_abs[i] = mx;
+  }
+}
+
+/* apply_exponential: CSR only (MatrixCSR.cpp:1318-1334): each nonzero z
+ * becomes exp(z - max_abs[i]) for base <= 0, pow(base, z - max_abs[i])
+ * otherwise; max_abs NULL means no shift.  Other formats: no-ops. */
+void orc_apply_exponential(int fmt, void *m, const double *max_abs, int base) {
+  if (fmt != ORC_CSR) return;
+  orc_csr *a = (orc_csr *)m;
+  if (!a->rowptr) return;
+  for (int i = 0; i < a->g.ln_row; i++)
+    for (int j = a->rowptr[i]; j < a->rowptr[i + 1]; j++) {
+      double z = a->values[j];
+      if (z == 0) continue;
+      double e = max_abs ? z - max_abs[i] : z;
+      a->values[j] = base <= 0 ? exp(e) : pow((double)base, e);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* PageRank                                                                  */
 /* ------------------------------------------------------------------------ */
 

@@ -118,6 +118,11 @@ void orc_csr_Ax_omp(const orc_csr *m, double *r, const double *v);
 void orc_get_col_sums(int fmt, const void *m, double *s);
 void orc_normalize_cols(int fmt, void *m, const double *s);
 void orc_make_stochastic(int fmt, void *m, double *s);
+/* row operations (Matrix.hpp:46-51) */
+void orc_get_row_sums(int fmt, const void *m, double *s);
+void orc_normalize_rows(int fmt, void *m, const double *s);
+void orc_get_row_max_abs(int fmt, const void *m, double *max_abs);
+void orc_apply_exponential(int fmt, void *m, const double *max_abs, int base);
 int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_iterations,
                   double *b, double *err_out);
 int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, int max_iterations,

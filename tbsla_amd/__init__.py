@@ -97,6 +97,11 @@ SIGNATURES = [
     ("tbsla_hip_get_col_sums", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_normalize_cols", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_make_stochastic", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_get_row_sums", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_normalize_rows", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_get_row_max_abs", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_apply_exponential", C.c_int, [_vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_softmax_rows", C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
                                       C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
@@ -413,6 +418,55 @@ class Matrix:
         ds = _dev(np.zeros(n) if s is None else s)
         check(lib().tbsla_hip_make_stochastic(self.h, ds.ptr, None))
         return ds.to_numpy(np.float64, n)
+
+    # -- row operations ----------------------------------------------------
+    def row_sums_len(self) -> int:
+        """Length of the row-sum vector: ELL and SCOO index it by global row
+        (MatrixELL.cpp:790, MatrixSCOO.cpp:703), CSR and DENSE by local row."""
+        g = self.geometry()
+        return g.n_row if self.fmt in (ELL, SCOO) else g.ln_row
+
+    def get_row_sums(self, s=None) -> np.ndarray:
+        """Matrix::get_row_sums; SCOO accumulates into ``s`` (zeros if None)."""
+        n = self.row_sums_len()
+        s = np.zeros(n) if s is None else np.asarray(s, np.float64)
+        if s.size < n:
+            raise ValueError(f"row sums need {n} values, got {s.size}")
+        ds = _dev(s)
+        check(lib().tbsla_hip_get_row_sums(self.h, ds.ptr, None))
+        return ds.to_numpy(np.float64, s.size)
+
+    def normalize_rows(self, s) -> None:
+        s = np.asarray(s, np.float64)
+        if self.fmt != COO and s.size < self.row_sums_len():
+            raise ValueError(f"row sums need {self.row_sums_len()} values, got {s.size}")
+        ds = _dev(s)
+        check(lib().tbsla_hip_normalize_rows(self.h, ds.ptr, None))
+
+    def get_row_max_abs(self) -> np.ndarray:
+        """Matrix::get_row_max_abs (CSR; zeros from the no-op formats)."""
+        n = self.geometry().ln_row
+        dm = _dev(np.zeros(n))
+        check(lib().tbsla_hip_get_row_max_abs(self.h, dm.ptr, None))
+        return dm.to_numpy(np.float64, n)
+
+    def apply_exponential(self, max_abs=None, base: int = -1) -> None:
+        """Matrix::apply_exponential: base <= 0 uses exp; max_abs None: no shift."""
+        if max_abs is None:
+            check(lib().tbsla_hip_apply_exponential(self.h, None, base, None))
+            return
+        max_abs = np.asarray(max_abs, np.float64)
+        if self.fmt == CSR and max_abs.size < self.geometry().ln_row:
+            raise ValueError("max_abs needs ln_row values")
+        dm = _dev(max_abs)
+        check(lib().tbsla_hip_apply_exponential(self.h, dm.ptr, base, None))
+
+    def softmax_rows(self, base: int = -1):
+        """The attention step's fused row softmax (CSR); returns (max_abs, sums)."""
+        n = self.geometry().ln_row
+        dm, ds = DeviceBuffer(max(n, 1) * 8), DeviceBuffer(max(n, 1) * 8)
+        check(lib().tbsla_hip_softmax_rows(self.h, base, dm.ptr, ds.ptr, None))
+        return dm.to_numpy(np.float64, n), ds.to_numpy(np.float64, n)
 
     def page_rank(self, beta=0.85, epsilon=1e-10, max_iterations=10000):
         n = self.geometry().ln_col

@@ -43,6 +43,8 @@ CPP_PROGRAMS = [
     # (source, output, rpath relative to the output)
     (os.path.join(PKG, "apps", "tbsla_perf_hip.cpp"), os.path.join(PKG, "bin", "tbsla_perf_hip"),
      "$ORIGIN/.."),
+    (os.path.join(PKG, "apps", "tbsla_attention_hip.cpp"),
+     os.path.join(PKG, "bin", "tbsla_attention_hip"), "$ORIGIN/.."),
     (os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"),
      os.path.join(ROOT, "tests", "cpp", "test_dropin"), "$ORIGIN/../../tbsla_amd"),
 ]

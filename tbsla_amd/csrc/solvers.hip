@@ -224,6 +224,22 @@ struct DevBuf {
 
 }  // namespace tbsla_hip
 
+namespace tbsla_hip {
+
+int values_changed(tbsla_hip_matrix_s *m, hipStream_t s) {
+  int win = 0, sl = 0;  // the split layout copies values: rebuild it
+  if (split_params(m, &win, &sl)) TBSLA_TRY(split_build(m, win, sl, s));
+  if (m->values32) {  // keep the FP32 copy in step
+    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+    hipFree(m->values32);
+    m->values32 = nullptr;
+  }
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  return TBSLA_HIP_SUCCESS;
+}
+
+}  // namespace tbsla_hip
+
 using namespace tbsla_hip;
 
 extern "C" int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *stream) {
@@ -275,14 +291,7 @@ extern "C" int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s,
                                                        m->colidx, m->f_col, m->values, d_s);
     TBSLA_HIP_LAUNCH_CHECK();
   }
-  int win = 0, sl = 0;  // the split layout copies values: rebuild it
-  if (split_params(m, &win, &sl)) TBSLA_TRY(split_build(m, win, sl, s));
-  if (m->values32) {  // keep the FP32 copy in step
-    hipFree(m->values32);
-    m->values32 = nullptr;
-  }
-  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-  return TBSLA_HIP_SUCCESS;
+  return values_changed(m, s);
 }
 
 extern "C" int tbsla_hip_make_stochastic(tbsla_hip_matrix_t m, double *d_s, void *stream) {

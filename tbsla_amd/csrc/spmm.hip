@@ -18,7 +18,7 @@ namespace tbsla_hip {
 
 namespace {
 
-constexpr int kMaxTiles = 4;  // k <= 256
+constexpr int kMaxTiles = 4;  // columns per launch row: 256; wider B in chunks (grid y)
 
 template <int KT>
 __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, int f_col, int ln_col,
@@ -32,6 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, in
       (int)(((long long)blockIdx.x * kBlock + threadIdx.x) / kWave));
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= n_rows) return;
+  const int k0 = blockIdx.y * (kMaxTiles * kWave);  // this launch row's column chunk
   double acc[KT];
 #pragma unroll
   for (int t = 0; t < KT; t++) acc[t] = 0.0;
@@ -43,14 +44,14 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, in
     const double *b = B + (size_t)c * k;
 #pragma unroll
     for (int t = 0; t < KT; t++) {
-      const int col = lane + t * kWave;
+      const int col = k0 + lane + t * kWave;
       if (col < k) acc[t] = acc[t] + v * b[col];
     }
   }
   double *out = C + (size_t)row * k;
 #pragma unroll
   for (int t = 0; t < KT; t++) {
-    const int col = lane + t * kWave;
+    const int col = k0 + lane + t * kWave;
     if (col < k) out[col] = acc[t];
   }
 }
@@ -299,12 +300,13 @@ extern "C" int tbsla_hip_spmm(tbsla_hip_matrix_t m, const double *d_B, double *d
                               void *stream) {
   if (!m || !d_B || !d_C) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
   if (m->format != TBSLA_HIP_CSR) return fail(TBSLA_HIP_ERR_UNSUPPORTED, "SpMM needs a CSR block");
-  if (k < 1 || k > kMaxTiles * kWave) return fail(TBSLA_HIP_ERR_INVALID, "k must be in 1..256");
+  if (k < 1) return fail(TBSLA_HIP_ERR_INVALID, "k must be at least 1");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long rows = m->ln_row;
   if (rows == 0) return TBSLA_HIP_SUCCESS;
-  const unsigned grid = grid_for(rows * kWave);
-  const int tiles = (k + kWave - 1) / kWave;
+  const int chunk = kMaxTiles * kWave;
+  const dim3 grid(grid_for(rows * kWave), (unsigned)((k + chunk - 1) / chunk));
+  const int tiles = k > chunk ? kMaxTiles : (k + kWave - 1) / kWave;
   switch (tiles) {
     case 1: k_csr_spmm<1><<<grid, kBlock, 0, s>>>(rows, k, m->f_col, m->ln_col, m->rowptr, m->colidx, m->values, d_B, d_C); break;
     case 2: k_csr_spmm<2><<<grid, kBlock, 0, s>>>(rows, k, m->f_col, m->ln_col, m->rowptr, m->colidx, m->values, d_B, d_C); break;

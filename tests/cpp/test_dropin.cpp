@@ -147,6 +147,75 @@ static int test_roundtrip() {
   return 0;
 }
 
+// Row operations on cdiag blocks (values 1): row sums count the entries of
+// each row (closed form); after normalize_rows every nonempty row sums to 1;
+// CSR's max_abs is 1 on nonempty rows, exp(z - 1) maps every entry to 1 and
+// pow(2, z) to 2.
+static int test_row_ops(int nr, int nc, int c) {
+  std::vector<double> cnt(nr, 0.0);
+  for (int i = 0; i < nr; i++) {
+    if (i - c >= 0 && i - c < nc) cnt[i] += 1;
+    if (c > 0 && i + c < nc) cnt[i] += 1;
+  }
+  std::vector<double> ones(nc, 1.0);
+  for (int fmt = TBSLA_HIP_COO; fmt <= TBSLA_HIP_DENSE; fmt++) {
+    Matrix m(fmt);
+    m.fill_cdiag(nr, nc, c);
+    std::vector<double> s(nr, 0.0);
+    m.get_row_sums(s.data());
+    for (int i = 0; i < nr; i++)
+      if (s[i] != (fmt == TBSLA_HIP_COO ? 0.0 : cnt[i])) return 20 + fmt;
+    if (fmt == TBSLA_HIP_COO) continue;
+    m.normalize_rows(s.data());
+    if (fmt == TBSLA_HIP_CSR) {
+      std::unique_ptr<double[]> r(m.spmv(ones.data()));
+      for (int i = 0; i < nr; i++)
+        if (r[i] != (cnt[i] ? 1.0 : 0.0)) return 30;
+      m.fill_cdiag(nr, nc, c);
+      std::vector<double> mx(nr, -1.0);
+      m.get_row_max_abs(mx.data());
+      for (int i = 0; i < nr; i++)
+        if (mx[i] != (cnt[i] ? 1.0 : 0.0)) return 31;
+      m.apply_exponential(mx.data(), -1);
+      std::unique_ptr<double[]> r1(m.spmv(ones.data()));
+      for (int i = 0; i < nr; i++)
+        if (r1[i] != cnt[i]) return 32;
+      m.apply_exponential(nullptr, 2);  // 2^1 = 2
+      std::unique_ptr<double[]> r2(m.spmv(ones.data()));
+      for (int i = 0; i < nr; i++)
+        if (r2[i] != 2 * cnt[i]) return 33;
+    }
+  }
+  return 0;
+}
+
+// print_as_dense, NUMAinit and the generator stubs.
+static int test_misc_virtuals() {
+  MatrixCSR m;
+  m.fill_cdiag(3, 3, 1);
+  std::ostringstream os;
+  m.print_as_dense(os);
+  if (os.str() != " 0.0   1.0   0.0  \n 1.0   0.0   1.0  \n 0.0   1.0   0.0  \n") {
+    std::fprintf(stderr, "print_as_dense gave:\n%s", os.str().c_str());
+    return 40;
+  }
+  MatrixCOO coo;
+  coo.fill_cdiag(3, 3, 1);
+  std::ostringstream os2;
+  coo.print_as_dense(os2);
+  if (os2.str() != os.str()) return 41;
+  m.NUMAinit();
+  std::vector<double> v = {1.0, 2.0, 3.0};
+  std::unique_ptr<double[]> r(m.spmv(v.data()));
+  if (r[0] != 2.0 || r[1] != 4.0 || r[2] != 2.0) return 42;
+  try {
+    m.fill_random(10, 10, 0.1);
+    return 43;
+  } catch (const std::runtime_error &) {
+  }
+  return 0;
+}
+
 int main() {
   int nr, nc, c;
   for (int i = 0; i <= 12; i++) {
@@ -175,7 +244,16 @@ int main() {
         RUN(test_cqmat(20, 30, 2 * i, q, s), "cqmat");
         RUN(test_cqmat(30, 20, 2 * i, q, s), "cqmat");
       }
+  for (int i = 0; i <= 12; i += 4) {
+    nr = 30, nc = 30, c = i;
+    RUN(test_row_ops(nr, nc, c), "row_ops");
+    nr = 20, nc = 30;
+    RUN(test_row_ops(nr, nc, c), "row_ops");
+    nr = 30, nc = 20;
+    RUN(test_row_ops(nr, nc, c), "row_ops");
+  }
   nr = nc = c = 0;
+  RUN(test_misc_virtuals(), "misc_virtuals");
   RUN(test_roundtrip(), "roundtrip");
   std::printf("=== finished without error ===\n");
   return 0;

@@ -16,7 +16,7 @@ import tbsla_amd as T
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("k", [1, 7, 64, 100, 256])
+@pytest.mark.parametrize("k", [1, 7, 64, 100, 256, 300, 512])
 def test_spmm_bit_exact(k):
     for (n, c, q, grid) in [(3000, 12, 0.2, (0, 0, 1, 1)), (8000, 32, 0.1, (1, 2, 4, 4))]:
         o = O.fill_cqmat(O.CSR, n, n, c, q, 0, *grid, value_seed=5)
