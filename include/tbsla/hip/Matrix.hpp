@@ -32,6 +32,11 @@
 namespace tbsla {
 namespace hip {
 
+// Matrix.hpp:59-63: thrown when a Matrix Market header is not supported.
+struct MatrixFormatReadException : public std::exception {
+  const char *what() const throw() { return "This Matrix format import is not implemented!"; }
+};
+
 inline void check(int status) {
   if (status != TBSLA_HIP_SUCCESS)
     throw std::runtime_error(std::string("tbsla_hip: ") + tbsla_hip_status_string(status) +
@@ -462,6 +467,14 @@ class Matrix {
 class MatrixCOO : public Matrix {
  public:
   MatrixCOO() : Matrix(TBSLA_HIP_COO) {}
+  // MatrixCOO::readMM (MatrixCOO.cpp:206-278); an unsupported header throws
+  // MatrixFormatReadException as the reference does; a symmetric file's
+  // off-diagonal entries are kept with their mirrors.
+  void readMM(std::string fname) {
+    const int st = tbsla_hip_read_mm(h_, fname.c_str(), nullptr);
+    if (st == TBSLA_HIP_ERR_UNSUPPORTED) throw MatrixFormatReadException();
+    check(st);
+  }
 };
 
 // Conversion constructors <FMT>(const MatrixCOO&) (MatrixCSR.cpp:49 etc.).

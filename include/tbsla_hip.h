@@ -205,6 +205,24 @@ int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *stream);
 int tbsla_hip_normalize_cols(tbsla_hip_matrix_t m, const double *d_s, void *stream);
 int tbsla_hip_make_stochastic(tbsla_hip_matrix_t m, double *d_s, void *stream);
 
+/* ---- Matrix Market import ----------------------------------------------- */
+/* MatrixCOO::readMM (MatrixCOO.cpp:206-278; the converter app
+ * tbsla_mm_to_binCOO.cpp:10): a "%%MatrixMarket matrix coordinate real
+ * general|symmetric" file into the COO block m (one block covering the
+ * matrix, as MatrixCOO::init).  Entries keep file order, 0-based; in a
+ * symmetric file each off-diagonal entry (r, c) is followed by (c, r) (the
+ * reference overwrites the first with the second, MatrixCOO.cpp:256-269;
+ * here both are kept).  Other headers: TBSLA_HIP_ERR_UNSUPPORTED (the
+ * reference throws MatrixFormatReadException); a missing file, fewer
+ * entries than declared or an index outside the matrix:
+ * TBSLA_HIP_ERR_INVALID.  Convert with tbsla_hip_convert for other formats. */
+int tbsla_hip_read_mm(tbsla_hip_matrix_t m, const char *path, void *stream);
+/* The same parse into host arrays, with no device work: *n_row, *n_col and
+ * *nnz are set; h_rows/h_cols/h_values (nullable, `capacity` entries each)
+ * receive the triplets when given. */
+int tbsla_hip_mm_parse(const char *path, int *n_row, int *n_col, long long *nnz, int *h_rows,
+                       int *h_cols, double *h_values, long long capacity);
+
 /* ---- row operations ------------------------------------------------------ */
 /* Matrix::get_row_sums / normalize_rows (Matrix.hpp:46-47).  Each format
  * keeps the reference's indexing, which differs between formats:

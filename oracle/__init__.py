@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
         _lib.orc_get_col_sums.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_normalize_cols.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_make_stochastic.argtypes = [C.c_int, C.c_void_p, dp]
+        _lib.orc_coo_read_mm.argtypes = [C.c_void_p, C.c_char_p]
+        _lib.orc_coo_read_mm.restype = C.c_int
         _lib.orc_get_row_sums.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_normalize_rows.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_get_row_max_abs.argtypes = [C.c_int, C.c_void_p, dp]
@@ -381,6 +383,22 @@ def fill_cqmat(fmt, n_row, n_col, c, q, seed_mult=1, pr=0, pc=0, NR=1, NC=1,
     return m
 
 
+def coo_read_mm(path) -> tuple[int, "Matrix"]:
+    """MatrixCOO::readMM (symmetric overwrite fixed): (status, COO matrix);
+    status 0, -1 cannot open, -2 header rejected, -3 malformed."""
+    m = Matrix(COO)
+    rc = lib().orc_coo_read_mm(m.ptr, os.fsencode(path))
+    return rc, m
+
+
+def ref_read_mm(path) -> tuple[int, "RefMatrix"]:
+    """The compiled reference's MatrixCOO::readMM (0, or -2 on its
+    MatrixFormatReadException)."""
+    m = RefMatrix(COO)
+    rc = ref().ref_read_mm(m.h, os.fsencode(path))
+    return rc, m
+
+
 def fill_cdiag(fmt, n_row, n_col, cdiag, pr=0, pc=0, NR=1, NC=1) -> Matrix:
     m = Matrix(fmt)
     getattr(lib(), _FILL_CD[fmt])(m.ptr, n_row, n_col, cdiag, pr, pc, NR, NC)
@@ -514,6 +532,8 @@ def ref() -> C.CDLL:
         _ref.ref_AAxpAx.argtypes = [vp, dp, dp, dp, C.c_int]
         _ref.ref_AAxpAxpx.argtypes = [vp, dp, dp, dp, C.c_int]
         _ref.ref_make_stochastic.argtypes = [vp, dp]
+        _ref.ref_read_mm.argtypes = [vp, C.c_char_p]
+        _ref.ref_read_mm.restype = C.c_int
         _ref.ref_get_row_sums.argtypes = [vp, dp]
         _ref.ref_normalize_rows.argtypes = [vp, dp]
         _ref.ref_get_row_max_abs.argtypes = [vp, dp]

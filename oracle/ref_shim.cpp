@@ -229,6 +229,18 @@ void ref_apply_exponential(void *h, double *m, int base) {
   static_cast<Matrix *>(h)->apply_exponential(m, base);
 }
 
+// MatrixCOO::readMM into a COO handle: 0, or -2 when it throws
+// MatrixFormatReadException.
+int ref_read_mm(void *h, const char *path) {
+  Quiet quiet;
+  try {
+    dynamic_cast<MatrixCOO *>(static_cast<Matrix *>(h))->readMM(std::string(path));
+  } catch (const MatrixFormatReadException &) {
+    return -2;
+  }
+  return 0;
+}
+
 // values_generation::cqmat_value, for single-index checks.
 void ref_cqmat_value(unsigned long k, unsigned long nr, unsigned long nc, unsigned long c,
                      double q, unsigned seed_mult, long *row, long *col) {

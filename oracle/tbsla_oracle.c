@@ -28,6 +28,7 @@
 #include "tbsla_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -769,6 +770,111 @@ int orc_coo_alloc(orc_coo *m, int n_row, int n_col, long nnz) {
   m->col = (int *)zalloc((size_t)nnz, sizeof(int));
   m->values = (double *)zalloc((size_t)nnz, sizeof(double));
   return 0;
+}
+
+/* MatrixCOO::readMM (MatrixCOO.cpp:206-278), the symmetric branch's
+ * overwrite (:256-269) fixed: the header's space-separated words
+ * (utils/split.cpp:6-19) must be "%%MatrixMarket matrix coordinate real"
+ * and "general" or "symmetric"; '%' lines are skipped (blank ones too);
+ * "n_row n_col n_values"; then n_values whitespace-separated triples
+ * "row col value", 1-based, read in order, each off-diagonal entry of a
+ * symmetric file followed by its mirror.  Returns 0; -1 when the file cannot
+ * be opened; -2 for a header the reference rejects
+ * (MatrixFormatReadException); -3 for fewer or unparsable entries or an
+ * index outside the matrix. */
+int orc_coo_read_mm(orc_coo *m, const char *path) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  fseek(f, 0, SEEK_END);
+  long size = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  char *buf = (char *)malloc((size_t)size + 1);
+  size_t got = fread(buf, 1, (size_t)size, f);
+  fclose(f);
+  buf[got] = 0;
+  int rc = 0;
+  /* header words */
+  char *eol = strchr(buf, '\n');
+  size_t hl = eol ? (size_t)(eol - buf) : got;
+  if (hl > 0 && buf[hl - 1] == '\r') hl--;
+  char *words[5];
+  int nw = 0;
+  size_t a = 0;
+  char *first = (char *)malloc(hl + 1);
+  memcpy(first, buf, hl);
+  first[hl] = 0;
+  while (a < hl && nw < 5) {
+    size_t b = a;
+    while (b < hl && first[b] != ' ') b++;
+    if (b > a) {
+      first[b] = 0;
+      words[nw++] = first + a;
+    }
+    a = b + 1;
+  }
+  if (nw < 5 || strcmp(words[0], "%%MatrixMarket") || strcmp(words[1], "matrix") ||
+      strcmp(words[2], "coordinate") || strcmp(words[3], "real") ||
+      (strcmp(words[4], "general") && strcmp(words[4], "symmetric"))) {
+    free(first);
+    free(buf);
+    return -2;
+  }
+  const int sym = strcmp(words[4], "symmetric") == 0;
+  free(first);
+  /* comments and blank lines, then the size line */
+  char *p = eol ? eol + 1 : buf + got;
+  for (;;) {
+    if (!*p) {
+      free(buf);
+      return -3;
+    }
+    char *e = strchr(p, '\n');
+    char *k = p;
+    while (*k == ' ' || *k == '\t' || *k == '\r') k++;
+    if (*p != '%' && *k != '\n' && *k != 0) break;
+    if (!e) {
+      free(buf);
+      return -3;
+    }
+    p = e + 1;
+  }
+  char *q;
+  long nr = strtol(p, &q, 10);
+  long nc = strtol(q, &q, 10);
+  long nv = strtol(q, &q, 10);
+  if (nr < 0 || nc < 0 || nv < 0) {
+    free(buf);
+    return -3;
+  }
+  q = strchr(q, '\n');
+  p = q ? q + 1 : buf + got;
+  orc_coo_alloc(m, (int)nr, (int)nc, sym ? 2 * nv : nv);
+  long n = 0;
+  for (long i = 0; i < nv; i++) {
+    char *e1, *e2, *e3;
+    long r = strtol(p, &e1, 10);
+    long c = strtol(e1, &e2, 10);
+    double v = strtod(e2, &e3);
+    if (e1 == p || e2 == e1 || e3 == e2 || r < 1 || r > nr || c < 1 || c > nc) {
+      rc = -3;
+      break;
+    }
+    p = e3;
+    m->row[n] = (int)(r - 1);
+    m->col[n] = (int)(c - 1);
+    m->values[n] = v;
+    n++;
+    if (sym && r != c) {
+      m->row[n] = (int)(c - 1);
+      m->col[n] = (int)(r - 1);
+      m->values[n] = v;
+      n++;
+    }
+  }
+  free(buf);
+  m->g.nnz = n;
+  m->len = n;
+  return rc;
 }
 
 void orc_csr_free(orc_csr *m) {

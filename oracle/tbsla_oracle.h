@@ -98,6 +98,10 @@ int orc_ell_from_coo(orc_ell *dst, const orc_coo *src);
 int orc_scoo_from_coo(orc_coo *dst, const orc_coo *src);
 int orc_dense_from_coo(orc_dense *dst, const orc_coo *src);
 
+/* MatrixCOO::readMM (MatrixCOO.cpp:206-278), symmetric overwrite fixed;
+ * 0, or -1 (cannot open), -2 (header rejected), -3 (malformed) */
+int orc_coo_read_mm(orc_coo *m, const char *path);
+
 int orc_csr_alloc(orc_csr *m, int n_row, int n_col, long nnz);
 int orc_coo_alloc(orc_coo *m, int n_row, int n_col, long nnz);
 void orc_csr_free(orc_csr *m);

@@ -97,6 +97,9 @@ SIGNATURES = [
     ("tbsla_hip_get_col_sums", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_normalize_cols", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_make_stochastic", C.c_int, [_vp, _vp, _vp]),
+    ("tbsla_hip_read_mm", C.c_int, [_vp, C.c_char_p, _vp]),
+    ("tbsla_hip_mm_parse", C.c_int, [C.c_char_p, _ip, _ip, C.POINTER(C.c_longlong), _vp, _vp,
+                                     _vp, C.c_longlong]),
     ("tbsla_hip_get_row_sums", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_normalize_rows", C.c_int, [_vp, _vp, _vp]),
     ("tbsla_hip_get_row_max_abs", C.c_int, [_vp, _vp, _vp]),
@@ -216,6 +219,19 @@ def lognormal_field(nx, ny, nz, mu=0.0, sigma=1.0, corr_len=4.0, seed=0) -> Devi
     d = DeviceBuffer(nx * ny * nz * 8)
     check(lib().tbsla_hip_fill_lognormal_field(d.ptr, nx, ny, nz, mu, sigma, corr_len, seed, None))
     return d
+
+
+def mm_parse(path):
+    """The Matrix Market parse of read_mm into host arrays (no device work):
+    (n_row, n_col, rows, cols, values), 0-based, in file order."""
+    nr, nc, nnz = C.c_int(), C.c_int(), C.c_longlong()
+    p = os.fsencode(path)
+    check(lib().tbsla_hip_mm_parse(p, C.byref(nr), C.byref(nc), C.byref(nnz), None, None, None, 0))
+    n = nnz.value
+    r, c, v = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1))
+    check(lib().tbsla_hip_mm_parse(p, C.byref(nr), C.byref(nc), C.byref(nnz), r.ctypes.data,
+                                   c.ctypes.data, v.ctypes.data, max(n, 1)))
+    return nr.value, nc.value, r[:n], c[:n], v[:n]
 
 
 def uniform_vector(n, seed, lo=0.0, hi=1.0) -> DeviceBuffer:
@@ -566,6 +582,13 @@ class Matrix:
         """Conversion constructor <fmt>(const MatrixCOO&); self must be COO."""
         m = Matrix(fmt)
         check(lib().tbsla_hip_convert(m.h, fmt, self.h, None))
+        return m
+
+    @classmethod
+    def read_mm(cls, path) -> "Matrix":
+        """MatrixCOO::readMM: a Matrix Market coordinate file into a COO block."""
+        m = cls(COO)
+        check(lib().tbsla_hip_read_mm(m.h, os.fsencode(path), None))
         return m
 
     @classmethod

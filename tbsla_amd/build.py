@@ -45,6 +45,8 @@ CPP_PROGRAMS = [
      "$ORIGIN/.."),
     (os.path.join(PKG, "apps", "tbsla_attention_hip.cpp"),
      os.path.join(PKG, "bin", "tbsla_attention_hip"), "$ORIGIN/.."),
+    (os.path.join(PKG, "apps", "tbsla_mm_to_bin_hip.cpp"),
+     os.path.join(PKG, "bin", "tbsla_mm_to_bin_hip"), "$ORIGIN/.."),
     (os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"),
      os.path.join(ROOT, "tests", "cpp", "test_dropin"), "$ORIGIN/../../tbsla_amd"),
 ]
