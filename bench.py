@@ -267,6 +267,53 @@ def formats_section(iters: int = 20):
     return out
 
 
+def attention_section(n: int = 1 << 20, nnz_per_row: int = 10, cols_b: int = 512):
+    """The sparse attention step (tbsla_attention_mpi.cpp:179-227) on one
+    block: n x n CSR with nnz_per_row entries per row at random columns
+    (cqmat, q = 1), values U[0,1); the row softmax fused into one kernel and
+    as the four separate row operations, then C = A B with B n x cols_b.
+    Softmax bytes: values read and written once plus the row pointers
+    (16 nnz + 4 (n + 1)); SpMM bytes: A once, one B row per entry, C written
+    (12 nnz + 4 (n + 1) + 8 nnz cols_b + 8 n cols_b).  Host clock around
+    repeated calls, each of which ends with a stream synchronisation."""
+    import tbsla_amd as T
+    m = T.Matrix(T.CSR).fill_cqmat(n, n, nnz_per_row, 1.0, 0, value_seed=1)
+    g = m.geometry()
+    nnz = g.stored
+    dmax, ds = T.DeviceBuffer(n * 8), T.DeviceBuffer(n * 8)
+    L = T.lib()
+
+    def clock(fn, reps):
+        fn()
+        T.check(L.tbsla_hip_stream_synchronize(None))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        T.check(L.tbsla_hip_stream_synchronize(None))
+        return (time.perf_counter() - t0) / reps * 1e6
+
+    fused_us = clock(lambda: T.check(L.tbsla_hip_softmax_rows(m.h, -1, dmax.ptr, ds.ptr, None)), 20)
+
+    def separate():
+        T.check(L.tbsla_hip_get_row_max_abs(m.h, dmax.ptr, None))
+        T.check(L.tbsla_hip_apply_exponential(m.h, dmax.ptr, -1, None))
+        T.check(L.tbsla_hip_get_row_sums(m.h, ds.ptr, None))
+        T.check(L.tbsla_hip_normalize_rows(m.h, ds.ptr, None))
+    separate_us = clock(separate, 20)
+    soft_bytes = 16 * nnz + 4 * (n + 1)
+    B, Cm = T.DeviceBuffer(n * cols_b * 8), T.DeviceBuffer(n * cols_b * 8)
+    T.check(L.tbsla_hip_fill_unit_vector(B.ptr, n * cols_b, 2, None))
+    spmm_us = clock(lambda: T.check(L.tbsla_hip_spmm(m.h, B.ptr, Cm.ptr, cols_b, None)), 3)
+    spmm_bytes = 12 * nnz + 4 * (n + 1) + 8 * nnz * cols_b + 8 * n * cols_b
+    return {"n": n, "nnz_per_row": nnz_per_row, "cols_B": cols_b, "nnz": nnz,
+            "softmax_fused_us": round(fused_us, 1), "softmax_separate_us": round(separate_us, 1),
+            "softmax_fused_GBps": round(soft_bytes / (fused_us * 1e-6) / 1e9, 1),
+            "softmax_gflops": round(6.0 * nnz / (fused_us * 1e-6) / 1e9, 1),
+            "spmm_us": round(spmm_us, 1),
+            "spmm_GBps": round(spmm_bytes / (spmm_us * 1e-6) / 1e9, 1),
+            "spmm_gflops": round(2.0 * nnz * cols_b / (spmm_us * 1e-6) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,7 +327,8 @@ def main():
     ap.add_argument("--csr-kernel", default=None, metavar="VEC,LANES,ROWS,NT",
                     help="use this CSR row kernel (row layout) instead of the autotuned one")
     ap.add_argument("--no-formats", action="store_true",
-                    help="skip the other formats, grid blocks, PageRank and copy figures")
+                    help="skip the other formats, grid blocks, PageRank, attention and copy "
+                         "figures")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-CG and personalised-PageRank sections")
     ap.add_argument("--batched-systems", type=int, default=512)
@@ -337,6 +385,7 @@ def main():
     extra = {}
     if not args.no_formats:
         extra["cfg2_other"] = formats_section()
+        extra["attention"] = attention_section()
     if not args.no_batched:
         from tbsla_amd import batched
         extra["batched_cg"] = batched.bench(args.batched_systems)

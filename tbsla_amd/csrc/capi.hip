@@ -51,12 +51,16 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
 void release_arrays(tbsla_hip_matrix_s *m) {
   cg_cache_release(m);  // holds graphs that name these arrays
   split_release(m);     // a copy of the entries in another order
-  if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32) hipDeviceSynchronize();
+  if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32 || m->coo_rowptr)
+    hipDeviceSynchronize();
   hipFree(m->rowptr);
   hipFree(m->colidx);
   hipFree(m->rowidx);
   hipFree(m->values);
   hipFree(m->values32);
+  hipFree(m->coo_rowptr);
+  m->coo_rowptr = nullptr;
+  m->coo_row0 = m->coo_nrows = 0;
   m->rowptr = m->colidx = m->rowidx = nullptr;
   m->values = nullptr;
   m->values32 = nullptr;

@@ -212,6 +212,11 @@ struct tbsla_hip_matrix_s {
   void *cg_cache = nullptr;
   // CSR split execution layout (split.hip), when selected
   void *split = nullptr;
+  // COO/SCOO with row-sorted triplets: row pointer over the rows
+  // [coo_row0, coo_row0 + coo_nrows) into the triplet arrays, built on the
+  // first SpMV (spmv.hip)
+  int *coo_rowptr = nullptr;
+  int coo_row0 = 0, coo_nrows = 0;
 };
 
 namespace tbsla_hip {

@@ -7,14 +7,19 @@
 //          reduction.
 //   ELL    one lane per row over column-major slots: every slot load is a
 //          contiguous 64-row stripe.
-//   COO/SCOO  row-sorted triplets: each wavefront reduces a 512-entry chunk
-//          with a segmented scan; rows complete inside a chunk are written
-//          directly, rows crossing chunk borders are summed in chunk order by
-//          a fix-up kernel -- deterministic.  Unsorted triplets use FP64
-//          atomics (unordered, as the reference's OpenMP array reduction).
+//   COO/SCOO  row-sorted triplets: SpMV (which zeroes y first) derives a
+//          row pointer over the triplets' row range once, kept with the
+//          block, and runs the CSR row kernel on the column and value arrays
+//          (the row array is not read again).  Accumulating Ax uses a
+//          segmented scan: each wavefront reduces a 512-entry chunk; rows
+//          complete inside a chunk are written directly, rows crossing chunk
+//          borders are summed in chunk order by a fix-up kernel --
+//          deterministic.  Unsorted triplets use FP64 atomics (unordered, as
+//          the reference's OpenMP array reduction).
 //   DENSE  one wavefront per row (GEMV).
 // Floating point: FMA accumulation; the per-row order differs from the
 // reference's sequential loop only in the CSR/COO/DENSE lane split.
+#include <algorithm>
 #include <climits>
 
 #include "common.hpp"
@@ -175,6 +180,24 @@ __global__ __launch_bounds__(kBlock) void k_coo_atomic(long long nnz, const int 
 }
 
 // MatrixDENSE::Ax (MatrixDENSE.cpp:77-86): r[i] += A[i,:] . x.
+// Row pointer of row-sorted triplets: rp[i] = the first k with
+// rows[k] >= row0 + i, for i in [0, nrows]; entry k writes the pointers of
+// the rows after its predecessor's, up to its own.  A decrease sets *unsorted.
+__global__ __launch_bounds__(kBlock) void k_coo_rowptr(long long n, const int *__restrict__ rows,
+                                                       int row0, int nrows, int *__restrict__ rp,
+                                                       int *__restrict__ unsorted) {
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k <= n;
+       k += (long long)gridDim.x * kBlock) {
+    const long long lo = k == 0 ? -1 : (long long)rows[k - 1] - row0;
+    const long long hi = k == n ? nrows : (long long)rows[k] - row0;
+    if (hi < lo) {
+      *unsorted = 1;
+      continue;
+    }
+    for (long long r = lo + 1 < 0 ? 0 : lo + 1; r <= hi && r <= nrows; r++) rp[r] = (int)k;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_dense_gemv(long long n_rows, long long n_cols,
                                                        const double *__restrict__ A,
                                                        const double *__restrict__ x,
@@ -227,6 +250,56 @@ int launch_axpy_inplace(double *y, const double *x, long long n, hipStream_t s) 
 
 // Lanes per row from the mean row length (two entries per lane per row,
 // clamped to 1..64) unless set; rows per group from the lanes.
+// The row pointer of a row-sorted COO/SCOO block (built once; false when
+// the triplets turn out not to be sorted, which also clears row_sorted).
+int coo_rowptr(tbsla_hip_matrix_s *m, long long n, bool *ok, hipStream_t s) {
+  *ok = false;
+  if (m->coo_rowptr) {
+    *ok = true;
+    return TBSLA_HIP_SUCCESS;
+  }
+  int ends[2] = {0, 0};
+  TBSLA_HIP_CHECK(hipMemcpyAsync(&ends[0], m->rowidx, 4, hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(&ends[1], m->rowidx + n - 1, 4, hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (ends[1] < ends[0]) {
+    m->row_sorted = 0;
+    return TBSLA_HIP_SUCCESS;
+  }
+  const int nrows = ends[1] - ends[0] + 1;
+  int *rp = nullptr, *flag = nullptr;
+  TBSLA_TRY(device_alloc((void **)&rp, (size_t)(nrows + 1) * 4));
+  int st = device_alloc((void **)&flag, 4);
+  if (st) {
+    hipFree(rp);
+    return st;
+  }
+  hipError_t e = hipMemsetAsync(flag, 0, 4, s);
+  unsigned g = (unsigned)std::min<long long>((n + kBlock) / kBlock, 65536);
+  if (e == hipSuccess) {
+    k_coo_rowptr<<<g, kBlock, 0, s>>>(n, m->rowidx, ends[0], nrows, rp, flag);
+    e = hipGetLastError();
+  }
+  int unsorted = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&unsorted, flag, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(flag);
+  if (e != hipSuccess) {
+    hipFree(rp);
+    return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
+  }
+  if (unsorted) {
+    hipFree(rp);
+    m->row_sorted = 0;
+    return TBSLA_HIP_SUCCESS;
+  }
+  m->coo_rowptr = rp;
+  m->coo_row0 = ends[0];
+  m->coo_nrows = nrows;
+  *ok = true;
+  return TBSLA_HIP_SUCCESS;
+}
+
 int pick_csr_lanes(const tbsla_hip_matrix_s *m) {
   if (m->lanes_per_row > 0) return m->lanes_per_row;
   double mean = m->ln_row > 0 ? (double)m->stored / m->ln_row : 1.0;
@@ -285,6 +358,24 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       if (n <= 0 || !m->has_arrays) return TBSLA_HIP_SUCCESS;
       const int row_off = coo ? vect_incr : vect_incr - m->f_row;
       const int col_off = coo ? 0 : -m->f_col;
+      bool rows_ok = false;
+      if (zero_first && m->row_sorted) TBSLA_TRY(coo_rowptr(m, n, &rows_ok, s));
+      if (rows_ok) {
+        // y is zero: the row kernel's stores over the triplets' row range
+        // equal the accumulation.  V, L from the mean row length.
+        const double mean = (double)n / m->coo_nrows;
+        int V = mean >= 16 ? 4 : (mean >= 6 ? 2 : 1);
+        int L = 1;
+        while (L < 64 && L * V * 2 <= mean) L <<= 1;
+        if (V == 2) L = std::min(std::max(L, 4), 32);
+        if (V == 4) L = std::min(std::max(L, 4), 16);
+        const int R = V > 1 ? 1 : (L >= 64 ? 1 : (L >= 32 ? 2 : 4));
+        CsrRowsArgs a{m->coo_nrows, m->coo_rowptr, m->colidx, m->values, d_v, -col_off,
+                      d_r + ((long long)m->coo_row0 + row_off), nullptr, nullptr, nullptr, nullptr};
+        csr_rows_launch<false>(a, V, L, R, 1, s);
+        TBSLA_HIP_LAUNCH_CHECK();
+        return TBSLA_HIP_SUCCESS;
+      }
       if (!m->row_sorted) {
         k_coo_atomic<<<stride_grid(n), kBlock, 0, s>>>(n, m->rowidx, m->colidx, m->values, d_v,
                                                         row_off, col_off, d_r);

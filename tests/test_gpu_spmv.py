@@ -141,6 +141,33 @@ def test_coo_unsorted_import_and_long_rows():
     close(g2.spmv(v), oc.spmv(v), rtol=1e-11)
 
 
+@pytest.mark.parametrize("fmt", [T.COO, T.SCOO])
+def test_coo_row_pointer_path_gaps_and_refill(fmt):
+    """Row-sorted triplets go through a derived row pointer and the CSR row
+    kernel: rows before the first and after the last entry, empty rows in
+    between, a block with f_row > 0, and a refill (the row pointer is
+    rebuilt) all match the oracle exactly on integer data."""
+    rng = np.random.default_rng(2)
+    n = 3000
+    rows = np.sort(rng.choice(np.arange(700, 2300), 9000)).astype(np.int32)
+    rows = rows[(rows < 1000) | (rows > 1400)]  # a gap of 400 empty rows
+    cols = rng.integers(0, n, rows.size).astype(np.int32)
+    vals = rng.integers(-3, 4, rows.size).astype(np.float64)
+    v = np.arange(n, dtype=np.float64)
+    geom = dict(n_row=n, n_col=n, f_row=0, f_col=0, ln_row=n, ln_col=n, pr=0, pc=0, NR=1, NC=1,
+                nnz=int(rows.size))
+    g = T.Matrix.from_arrays(fmt, geom, rows, cols, vals)
+    oc = O.coo_from_arrays(n, n, rows, cols, vals)
+    assert np.array_equal(g.spmv(v), oc.spmv(v))
+    for q, grid in ((0.3, (1, 1, 3, 2)), (0.0, (0, 0, 1, 1))):
+        g.fill_cqmat(2000, 2000, 12, q, 1, *grid)
+        o = O.fill_cqmat(fmt, 2000, 2000, 12, q, 1, *grid)
+        geo = o.geom
+        x = np.arange(geo["n_col"], dtype=np.float64)[geo["f_col"]:] if fmt == T.SCOO else \
+            np.arange(2000, dtype=np.float64)
+        assert np.array_equal(g.spmv(x), o.spmv(x)), (fmt, grid)
+
+
 def test_cfg1_ten_spmvs():
     """Configuration 1: n=100000, c=10, q=0.1, seed 0, values U[0,1) (seed 0),
     x U[0,1) (seed 1); ten repeated SpMVs, each within 1e-12 of the oracle."""
