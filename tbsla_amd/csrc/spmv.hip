@@ -198,16 +198,43 @@ __global__ __launch_bounds__(kBlock) void k_coo_rowptr(long long n, const int *_
   }
 }
 
+// One wavefront per row.  With vec (an even row length, so 16-byte aligned
+// rows, and a 16-byte aligned x) each lane reads two values per load and
+// keeps four loads in flight, with four partial sums added in a fixed
+// order; otherwise one value per load.
 __global__ __launch_bounds__(kBlock) void k_dense_gemv(long long n_rows, long long n_cols,
                                                        const double *__restrict__ A,
                                                        const double *__restrict__ x,
-                                                       double *__restrict__ y) {
+                                                       double *__restrict__ y, int vec) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
   const long long row = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= n_rows) return;
   const double *a = A + (size_t)row * n_cols;
   double sum = 0.0;
-  for (long long j = lane; j < n_cols; j += kWave) sum = fma(ldg_stream(a + j), x[j], sum);
+  if (vec) {
+    constexpr int U = 4, STEP = 2 * kWave;  // doubles per wave load
+    double s[U] = {0.0, 0.0, 0.0, 0.0};
+    long long j = 2 * lane;
+    for (; j + (U - 1) * STEP + 1 < n_cols; j += U * STEP) {
+      v2 av[U], xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        av[u] = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(a + j + u * STEP));
+        xv[u] = *reinterpret_cast<const v2 *>(x + j + u * STEP);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) s[u] = fma(av[u].y, xv[u].y, fma(av[u].x, xv[u].x, s[u]));
+    }
+    for (; j + 1 < n_cols; j += STEP) {
+      const v2 av = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(a + j));
+      const v2 xv = *reinterpret_cast<const v2 *>(x + j);
+      s[0] = fma(av.y, xv.y, fma(av.x, xv.x, s[0]));
+    }
+    sum = (s[0] + s[1]) + (s[2] + s[3]);
+  } else {
+    for (long long j = lane; j < n_cols; j += kWave) sum = fma(ldg_stream(a + j), x[j], sum);
+  }
   sum = group_sum<kWave>(sum);
   if (lane == 0) y[row] += sum;
 }
@@ -403,8 +430,9 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
       // MatrixDENSE.cpp:78-79
       if (m->nnz == 0 || !m->has_arrays || m->ln_row == 0) return TBSLA_HIP_SUCCESS;
+      const int vec = (m->ln_col % 2 == 0) && (reinterpret_cast<uintptr_t>(d_v) % 16 == 0);
       k_dense_gemv<<<grid_for((long long)m->ln_row * kWave), kBlock, 0, s>>>(
-          m->ln_row, m->ln_col, m->values, d_v, d_r);
+          m->ln_row, m->ln_col, m->values, d_v, d_r, vec);
       TBSLA_HIP_LAUNCH_CHECK();
       return TBSLA_HIP_SUCCESS;
     }
