@@ -37,15 +37,45 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmm(long long n_rows, int k, in
 #pragma unroll
   for (int t = 0; t < KT; t++) acc[t] = 0.0;
   const int beg = rowptr[row], end = rowptr[row + 1];
-  for (int j = beg; j < end; j++) {
-    const int c = colidx[j] - f_col;
-    if (c < 0 || c >= ln_col) continue;  // block filter (:19)
-    const double v = vals[j];
-    const double *b = B + (size_t)c * k;
+  int j = beg;
+  // E entries per step: their B rows' loads are in flight together; the
+  // products are still added in entry order
+  constexpr int E = 4;
+  for (; j + E <= end; j += E) {
+    bool in[E];
+    double v[E];
+    const double *bp[E];
 #pragma unroll
-    for (int t = 0; t < KT; t++) {
-      const int col = k0 + lane + t * kWave;
-      if (col < k) acc[t] = acc[t] + v * b[col];
+    for (int e = 0; e < E; e++) {
+      const int c = colidx[j + e] - f_col;
+      in[e] = c >= 0 && c < ln_col;  // block filter (:19)
+      v[e] = vals[j + e];
+      bp[e] = B + (size_t)(in[e] ? c : 0) * k;
+    }
+    double xv[E][KT];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+#pragma unroll
+      for (int t = 0; t < KT; t++) {
+        const int col = k0 + lane + t * kWave;
+        xv[e][t] = col < k ? bp[e][col] : 0.0;
+      }
+#pragma unroll
+    for (int e = 0; e < E; e++)
+#pragma unroll
+      for (int t = 0; t < KT; t++)
+        if (in[e]) acc[t] = acc[t] + v[e] * xv[e][t];
+  }
+  for (; j < end; j++) {
+    const int c = colidx[j] - f_col;
+    if (c >= 0 && c < ln_col) {
+      const double v = vals[j];
+      const double *b = B + (size_t)c * k;
+#pragma unroll
+      for (int t = 0; t < KT; t++) {
+        const int col = k0 + lane + t * kWave;
+        if (col < k) acc[t] = acc[t] + v * b[col];
+      }
     }
   }
   double *out = C + (size_t)row * k;
@@ -304,6 +334,8 @@ extern "C" int tbsla_hip_spmm(tbsla_hip_matrix_t m, const double *d_B, double *d
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long rows = m->ln_row;
   if (rows == 0) return TBSLA_HIP_SUCCESS;
+  if (m->ln_col == 0 || !m->has_arrays || m->stored == 0)  // no products: C = 0
+    return launch_fill(d_C, rows * (long long)k, 0.0, s);
   const int chunk = kMaxTiles * kWave;
   const dim3 grid(grid_for(rows * kWave), (unsigned)((k + chunk - 1) / chunk));
   const int tiles = k > chunk ? kMaxTiles : (k + kWave - 1) / kWave;
