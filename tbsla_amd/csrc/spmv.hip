@@ -389,11 +389,12 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       if (zero_first && m->row_sorted) TBSLA_TRY(coo_rowptr(m, n, &rows_ok, s));
       if (rows_ok) {
         // y is zero: the row kernel's stores over the triplets' row range
-        // equal the accumulation.  V, L from the mean row length.
+        // equal the accumulation.  V from the mean row length, L the
+        // fewest lanes whose V L entries cover a mean row in one trip.
         const double mean = (double)n / m->coo_nrows;
         int V = mean >= 16 ? 4 : (mean >= 6 ? 2 : 1);
         int L = 1;
-        while (L < 64 && L * V * 2 <= mean) L <<= 1;
+        while (L < 64 && L * V < mean) L <<= 1;
         if (V == 2) L = std::min(std::max(L, 4), 32);
         if (V == 4) L = std::min(std::max(L, 4), 16);
         const int R = V > 1 ? 1 : (L >= 64 ? 1 : (L >= 32 ? 2 : 4));
