@@ -339,38 +339,86 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
   const double beta = rz_new / fmax(rz_old, S->near_zero);
   double *pn = a.p[par ^ 1];
   double pap = 0.0;
-  for (int row = row_first; row < r1; row += kBlock / L) {
-    int j, end;
+  constexpr int PASS = kBlock / L;
+  if (row_first < r1) {
     V acc = 0;
-    double own;
-    if (row == row_first) {
 #pragma unroll
-      for (int k = 0; k < P; k++) {
-        if (f.beg + lane + k * L < f.end) {
-          const double v = f.s[k] + beta * f.t[k];
-          if (MIX) acc = fmaf((float)f.v[k], (float)v, (float)acc);
-          else acc = fma((double)f.v[k], v, (double)acc);
-        }
+    for (int k = 0; k < P; k++) {
+      if (f.beg + lane + k * L < f.end) {
+        const double v = f.s[k] + beta * f.t[k];
+        if (MIX) acc = fmaf((float)f.v[k], (float)v, (float)acc);
+        else acc = fma((double)f.v[k], v, (double)acc);
       }
-      j = f.beg + lane + P * L;
-      end = f.end;
-      own = own_z + beta * own_p;
-    } else {
-      j = a.rowptr[row] + lane;
-      end = a.rowptr[row + 1];
-      own = zv[row] + beta * po[row];
     }
-    for (; j < end; j += L) {
+    for (int j = f.beg + lane + P * L; j < f.end; j += L) {
       const int c = a.colidx[j] - a.f_col;
       const double v = zv[c] + beta * po[c];
       if (MIX) acc = fmaf((float)val[j], (float)v, (float)acc);
       else acc = fma((double)val[j], v, (double)acc);
     }
     acc = lane_sum<L>(acc);
+    const double own = own_z + beta * own_p;
     if (lane == 0) {
-      pn[row] = own;
-      a.ap[row] = (double)acc;
+      pn[row_first] = own;
+      a.ap[row_first] = (double)acc;
       pap += own * (double)acc;
+    }
+  }
+  // the later rows two at a time: both rows' extents, then entries, then
+  // gathers are issued before either is summed
+  constexpr int R2 = 2;
+  for (int row = row_first + PASS; row < r1; row += R2 * PASS) {
+    RowPrefetch<P, V> g[R2];
+    double oz[R2], op[R2];
+    int rw[R2];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      rw[r] = min(row + r * PASS, r1 - 1);
+      g[r].beg = a.rowptr[rw[r]];
+      g[r].end = a.rowptr[rw[r] + 1];
+      oz[r] = zv[rw[r]];
+      op[r] = po[rw[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int j = min(g[r].beg + lane + k * L, a.a_nnz - 1);
+        g[r].col[k] = a.colidx[j];
+        g[r].v[k] = val[j];
+      }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        g[r].s[k] = zv[g[r].col[k] - a.f_col];
+        g[r].t[k] = po[g[r].col[k] - a.f_col];
+      }
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      if (row + r * PASS >= r1) break;
+      V acc = 0;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        if (g[r].beg + lane + k * L < g[r].end) {
+          const double v = g[r].s[k] + beta * g[r].t[k];
+          if (MIX) acc = fmaf((float)g[r].v[k], (float)v, (float)acc);
+          else acc = fma((double)g[r].v[k], v, (double)acc);
+        }
+      }
+      for (int j = g[r].beg + lane + P * L; j < g[r].end; j += L) {
+        const int c = a.colidx[j] - a.f_col;
+        const double v = zv[c] + beta * po[c];
+        if (MIX) acc = fmaf((float)val[j], (float)v, (float)acc);
+        else acc = fma((double)val[j], v, (double)acc);
+      }
+      acc = lane_sum<L>(acc);
+      const double own = oz[r] + beta * op[r];
+      if (lane == 0) {
+        pn[rw[r]] = own;
+        a.ap[rw[r]] = (double)acc;
+        pap += own * (double)acc;
+      }
     }
   }
   pap = wg_sum(pap, lds);
@@ -550,7 +598,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   const double rz = t[0], pap = t[1];
   const double alpha = rz / fmax(pap, S->near_zero);
   double rr = 0.0, rzp = 0.0;
-  for (int row = row_first; row < r1; row += kBlock) {
+  // PRE 2/4: the first row, then the later rows two at a time (each pair's
+  // loads issued before either is updated; sums in row order)
+  const int row_end = (PRE == 2 || PRE == 4) ? min(r1, row_first + 1) : r1;
+  for (int row = row_first; row < row_end; row += kBlock) {
     const bool first = row == row_first;
     const double xv = first ? fx : a.x[row];
     const double pv = first ? fp : pn[row];
@@ -636,6 +687,65 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       rzp += ri * zi;
     }
   }
+  if (PRE == 2 || PRE == 4) {
+    constexpr int R2 = 2;
+    for (int row = row_first + kBlock; row < r1; row += R2 * kBlock) {
+      RowPrefetch<P, V> g[R2];
+      double xv[R2], pv[R2], rv[R2], apv[R2];
+      int rw[R2];
+#pragma unroll
+      for (int r = 0; r < R2; r++) {
+        rw[r] = min(row + r * kBlock, r1 - 1);
+        xv[r] = a.x[rw[r]];
+        pv[r] = pn[rw[r]];
+        rv[r] = rc[rw[r]];
+        apv[r] = a.ap[rw[r]];
+        g[r].beg = a.t_rp[rw[r]];
+        g[r].end = a.t_rp[rw[r] + 1];
+      }
+#pragma unroll
+      for (int r = 0; r < R2; r++)
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          const int j = min(g[r].beg + k, a.g_nnz - 1);
+          g[r].col[k] = a.t_ci[j];
+          g[r].v[k] = tv[j];
+        }
+#pragma unroll
+      for (int r = 0; r < R2; r++)
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          g[r].s[k] = rc[g[r].col[k] - a.f_col];
+          g[r].t[k] = a.ap[g[r].col[k] - a.f_col];
+        }
+#pragma unroll
+      for (int r = 0; r < R2; r++) {
+        if (row + r * kBlock >= r1) break;
+        a.x[rw[r]] = xv[r] + alpha * pv[r];
+        const double ri = rv[r] + (-alpha) * apv[r];
+        rn[rw[r]] = ri;
+        rr += ri * ri;
+        V acc = 0;
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+          if (g[r].beg + k < g[r].end) {
+            const double rcv = g[r].s[k] + (-alpha) * g[r].t[k];
+            if (MIX) acc = fmaf((float)g[r].v[k], (float)rcv, (float)acc);
+            else acc = fma((double)g[r].v[k], rcv, (double)acc);
+          }
+        }
+        for (int j = g[r].beg + P; j < g[r].end; j++) {
+          const int c = a.t_ci[j] - a.f_col;
+          const double rcv = rc[c] + (-alpha) * a.ap[c];
+          if (MIX) acc = fmaf((float)tv[j], (float)rcv, (float)acc);
+          else acc = fma((double)tv[j], rcv, (double)acc);
+        }
+        if (MIX) a.t32[rw[r]] = (float)acc;
+        else a.t64[rw[r]] = (double)acc;
+        if (PRE == 4) rzp += (double)acc * (double)acc;
+      }
+    }
+  }
   rr = wg_sum(rr, lds);
   if (PRE == 1 || PRE == 3 || PRE == 4) rzp = wg_sum(rzp, lds);
   if (threadIdx.x == 0) {
@@ -718,30 +828,66 @@ __global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode)
                                        : fl != kInactive;
   if (!part) return;
   double rz = 0.0;
-  for (int row = row_first; row < r1; row += kBlock) {
-    const bool first = row == row_first;
+  if (row_first < r1) {
     V acc = 0;
-    int j, end;
-    if (first) {
 #pragma unroll
-      for (int k = 0; k < P; k++)
-        if (f.beg + k < f.end) {
-          if (MIX) acc = fmaf((float)f.v[k], (float)tg[k], (float)acc);
-          else acc = fma((double)f.v[k], (double)tg[k], (double)acc);
-        }
-      j = f.beg + P;
-      end = f.end;
-    } else {
-      j = a.g_rp[row];
-      end = a.g_rp[row + 1];
-    }
-    for (; j < end; j++) {
+    for (int k = 0; k < P; k++)
+      if (f.beg + k < f.end) {
+        if (MIX) acc = fmaf((float)f.v[k], (float)tg[k], (float)acc);
+        else acc = fma((double)f.v[k], (double)tg[k], (double)acc);
+      }
+    for (int j = f.beg + P; j < f.end; j++) {
       if (MIX) acc = fmaf((float)gv[j], (float)tv[a.g_ci[j] - a.f_col], (float)acc);
       else acc = fma((double)gv[j], (double)tv[a.g_ci[j] - a.f_col], (double)acc);
     }
     const double zi = (double)acc;
-    a.z[row] = zi;
-    rz += (first ? fr : rc[row]) * zi;
+    a.z[row_first] = zi;
+    rz += fr * zi;
+  }
+  // the later rows two at a time (loads of both issued before either sum)
+  constexpr int R2 = 2;
+  for (int row = row_first + kBlock; row < r1; row += R2 * kBlock) {
+    RowPrefetch<P, V> g[R2];
+    double rv[R2];
+    V tq[R2][P];
+    int rw[R2];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      rw[r] = min(row + r * kBlock, r1 - 1);
+      g[r].beg = a.g_rp[rw[r]];
+      g[r].end = a.g_rp[rw[r] + 1];
+      rv[r] = rc[rw[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int j = min(g[r].beg + k, a.g_nnz - 1);
+        g[r].col[k] = a.g_ci[j];
+        g[r].v[k] = gv[j];
+      }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) tq[r][k] = tv[g[r].col[k] - a.f_col];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      if (row + r * kBlock >= r1) break;
+      V acc = 0;
+#pragma unroll
+      for (int k = 0; k < P; k++)
+        if (g[r].beg + k < g[r].end) {
+          if (MIX) acc = fmaf((float)g[r].v[k], (float)tq[r][k], (float)acc);
+          else acc = fma((double)g[r].v[k], (double)tq[r][k], (double)acc);
+        }
+      for (int j = g[r].beg + P; j < g[r].end; j++) {
+        if (MIX) acc = fmaf((float)gv[j], (float)tv[a.g_ci[j] - a.f_col], (float)acc);
+        else acc = fma((double)gv[j], (double)tv[a.g_ci[j] - a.f_col], (double)acc);
+      }
+      const double zi = (double)acc;
+      a.z[rw[r]] = zi;
+      rz += rv[r] * zi;
+    }
   }
   rz = wg_sum(rz, lds);
   if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
