@@ -122,7 +122,7 @@ class Solver:
             o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_JACOBI, rep)
             it, rn = A.cg_device(self.rhs, self.x, o)
         ms = (time.perf_counter() - t0) * 1e3
-        # judged by the true FP64 residual
+        # accepted on the true FP64 residual
         y = T.DeviceBuffer(self.n * 8)
         T.check(T.lib().tbsla_hip_spmv(A.h, y.ptr, self.x.ptr, 0, None))
         b = self.rhs.to_numpy(count=self.n)
