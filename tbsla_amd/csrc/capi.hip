@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+#include <atomic>
+
 #include "common.hpp"
 #include "csr_rows.hpp"
 #include "kernels.hpp"
@@ -51,19 +53,17 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
 void release_arrays(tbsla_hip_matrix_s *m) {
   cg_cache_release(m);  // holds graphs that name these arrays
   split_release(m);     // a copy of the entries in another order
-  if (m->rowptr || m->colidx || m->rowidx || m->values || m->values32 || m->coo_rowptr)
-    hipDeviceSynchronize();
+  if (m->rowptr || m->colidx || m->rowidx || m->values || m->coo_rowptr) hipDeviceSynchronize();
   hipFree(m->rowptr);
   hipFree(m->colidx);
   hipFree(m->rowidx);
   hipFree(m->values);
-  hipFree(m->values32);
   hipFree(m->coo_rowptr);
   m->coo_rowptr = nullptr;
   m->coo_row0 = m->coo_nrows = 0;
   m->rowptr = m->colidx = m->rowidx = nullptr;
   m->values = nullptr;
-  m->values32 = nullptr;
+  m->values_version++;
   m->has_arrays = 0;
   m->stored = 0;
 }
@@ -176,8 +176,10 @@ int tbsla_hip_create(tbsla_hip_matrix_t *out, int format) {
   if (!out) return fail(TBSLA_HIP_ERR_INVALID, "null output pointer");
   if (format < TBSLA_HIP_COO || format > TBSLA_HIP_DENSE)
     return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
+  static std::atomic<unsigned long long> next_uid{1};
   *out = new tbsla_hip_matrix_s();
   (*out)->format = format;
+  (*out)->uid = next_uid++;
   return TBSLA_HIP_SUCCESS;
 }
 

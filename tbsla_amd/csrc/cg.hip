@@ -1172,6 +1172,12 @@ struct CgCache {
   CgCom *h_com = nullptr;  // pinned status copies (2)
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipStream_t own = nullptr;  // used when the caller passes the null stream
+  // FP32 copy of A's values for the mixed form, made from the handle uid
+  // at values version ver; kept with the cache (which a destroyed handle
+  // leaves to the next), so a new matrix of the same size allocates nothing
+  float *a_v32 = nullptr;
+  size_t a_v32_len = 0;
+  unsigned long long a_v32_uid = 0, a_v32_ver = 0;
   // factor G: FP32 copies of G and G^T
   tbsla_hip_matrix_s *factor = nullptr;
   int *t_rp = nullptr, *t_ci = nullptr;
@@ -1245,6 +1251,7 @@ struct CgCache {
     if (own) hipStreamDestroy(own);
     for (auto &g : graphs) g.destroy();
     drop_factor();
+    hipFree(a_v32);
   }
 };
 
@@ -1297,16 +1304,6 @@ void cg_cache_release(tbsla_hip_matrix_s *m) {
     }
   }
   delete c;
-}
-
-int ensure_values32(tbsla_hip_matrix_s *m, hipStream_t s) {
-  if (m->values32) return TBSLA_HIP_SUCCESS;
-  TBSLA_TRY(device_alloc((void **)&m->values32, (size_t)std::max(m->stored, 1LL) * 4));
-  if (m->stored > 0) {
-    k_to_f32<<<sgrid(m->stored), kBlock, 0, s>>>(m->values, m->stored, m->values32);
-    TBSLA_HIP_LAUNCH_CHECK();
-  }
-  return TBSLA_HIP_SUCCESS;
 }
 
 namespace {
@@ -1569,7 +1566,26 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   for (auto &e : c.ev)
     if (!e) TBSLA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (pre == TBSLA_HIP_PRECOND_JACOBI) TBSLA_TRY(c.dinv.ensure((size_t)n));
-  if (mixed) TBSLA_TRY(ensure_values32(m, s));
+  if (mixed) {
+    const size_t need = (size_t)std::max(m->stored, 1LL);
+    if (c.a_v32_len < need) {
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      hipFree(c.a_v32);
+      c.a_v32 = nullptr;
+      c.a_v32_len = 0;
+      TBSLA_TRY(device_alloc((void **)&c.a_v32, need * 4));
+      c.a_v32_len = need;
+      c.a_v32_uid = 0;
+    }
+    if (c.a_v32_uid != m->uid || c.a_v32_ver != m->values_version) {
+      if (m->stored > 0) {
+        k_to_f32<<<sgrid(m->stored), kBlock, 0, s>>>(m->values, m->stored, c.a_v32);
+        TBSLA_HIP_LAUNCH_CHECK();
+      }
+      c.a_v32_uid = m->uid;
+      c.a_v32_ver = m->values_version;
+    }
+  }
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
     if (mixed) TBSLA_TRY(c.t32.ensure((size_t)n));
     else TBSLA_TRY(c.t64.ensure((size_t)n));
@@ -1644,7 +1660,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.rowptr = m->rowptr;
   a.colidx = m->colidx;
   a.val64 = m->values;
-  a.val32 = m->values32;
+  a.val32 = mixed ? c.a_v32 : nullptr;
   a.f_col = m->f_col;
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
     a.g_rp = G->rowptr;

@@ -199,7 +199,6 @@ struct tbsla_hip_matrix_s {
   int *colidx = nullptr;     // CSR: stored;  ELL: max_col * ln_row, column-major
   int *rowidx = nullptr;     // COO/SCOO: stored
   double *values = nullptr;  // CSR/COO/SCOO: stored; ELL: column-major; DENSE: row-major
-  float *values32 = nullptr; // FP32 copy of values for the mixed-precision forms
   // SpMV schedule
   int lanes_per_row = 0;     // CSR: 0 = pick from the mean row length
   int rows_per_group = 0;    // CSR: rows each lane group keeps in flight (0 = pick)
@@ -217,6 +216,10 @@ struct tbsla_hip_matrix_s {
   // first SpMV (spmv.hip)
   int *coo_rowptr = nullptr;
   int coo_row0 = 0, coo_nrows = 0;
+  // identity of the handle and of its values: copies derived from the
+  // values elsewhere (the CG's FP32 copy) are current while both match
+  unsigned long long uid = 0;
+  unsigned long long values_version = 0;
 };
 
 namespace tbsla_hip {

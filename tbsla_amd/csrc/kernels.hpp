@@ -53,11 +53,9 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
 int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s);
 int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices);
 
-// cg.hip: FP32 copy of the values (m->values32), built once
-int ensure_values32(tbsla_hip_matrix_s *m, hipStream_t s);
-
 // solvers.hip: after the values of m changed in place: rebuild the split
-// layout (it holds a copy of the values), drop the FP32 copy, synchronise
+// layout (it holds a copy of the values), mark copies of the old values
+// stale (values_version), synchronise
 int values_changed(tbsla_hip_matrix_s *m, hipStream_t s);
 
 }  // namespace tbsla_hip

@@ -1085,7 +1085,7 @@ extern "C" int tbsla_hip_precond_net_apply(tbsla_hip_precond_net_t net, tbsla_hi
   auto shape = [&](tbsla_hip_matrix_s *M, const int *rp, const int *ci, int **rpo, int **cio,
                    double **vo) -> int {
     const bool reuse = M->format == TBSLA_HIP_CSR && M->has_arrays && M->ln_row == n &&
-                       M->stored == ng && M->rowptr && M->colidx && M->values && !M->values32;
+                       M->stored == ng && M->rowptr && M->colidx && M->values;
     if (reuse) {
       TBSLA_HIP_CHECK(hipMemcpyAsync(M->rowptr, rp, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, s));
       TBSLA_HIP_CHECK(hipMemcpyAsync(M->colidx, ci, (size_t)ng * 4, hipMemcpyDeviceToDevice, s));

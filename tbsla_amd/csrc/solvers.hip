@@ -229,11 +229,7 @@ namespace tbsla_hip {
 int values_changed(tbsla_hip_matrix_s *m, hipStream_t s) {
   int win = 0, sl = 0;  // the split layout copies values: rebuild it
   if (split_params(m, &win, &sl)) TBSLA_TRY(split_build(m, win, sl, s));
-  if (m->values32) {  // keep the FP32 copy in step
-    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-    hipFree(m->values32);
-    m->values32 = nullptr;
-  }
+  m->values_version++;  // copies made from the old values are stale
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   return TBSLA_HIP_SUCCESS;
 }

@@ -293,3 +293,30 @@ def test_poisson_direct_solve():
                          precision=prec, replace_every=50 if prec == T.CG_MIXED else 0)
         assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
         assert rel_max(x, exact) < 1e-6, (pre, prec, rel_max(x, exact))
+
+
+def test_mixed_sees_values_changed_in_place():
+    """The mixed form's FP32 copy of A is kept between solves and remade when
+    the values change: after normalize_rows halves every row (A/2), the
+    mixed solution doubles; a second handle of the same size reusing the
+    first one's CG state converts its own values."""
+    import gc
+    dims = (14, 14, 14)
+    g = T.Matrix(T.CSR).fill_poisson3d(*dims, kappa=T.lognormal_field(*dims, seed=5))
+    A = scipy_of(g)
+    n = A.shape[0]
+    rhs = T.uniform_vector(n, 7, -1.0, 1.0).to_numpy(count=n)
+    opts = dict(tolerance=1e-10, relative=True, preconditioner=T.PRECOND_JACOBI,
+                precision=T.CG_MIXED, replace_every=20)
+    x1, _, _ = g.cg(rhs, **opts)
+    assert np.linalg.norm(rhs - A @ x1) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+    g.normalize_rows(np.full(n, 2.0))
+    x2, _, _ = g.cg(rhs, **opts)
+    assert np.linalg.norm(rhs - (A / 2) @ x2) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+    assert rel_max(x2, 2 * x1) < 1e-7
+    del g
+    gc.collect()
+    h = T.Matrix(T.CSR).fill_poisson3d(*dims, kappa=T.lognormal_field(*dims, seed=6))
+    B = scipy_of(h)
+    x3, _, _ = h.cg(rhs, **opts)
+    assert np.linalg.norm(rhs - B @ x3) < 1e-10 * np.linalg.norm(rhs) * 1.0001
