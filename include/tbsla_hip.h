@@ -166,7 +166,10 @@ int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_matrix_t coo
 /* Device pointers of the block's arrays (owned by the handle; valid until
  * the next fill/import/convert or destroy), in the device layout: CSR
  * a = rowptr, b = colidx; ELL b = columns (column-major); COO/SCOO a = row,
- * b = col; v = values.  For zero-copy interoperation. */
+ * b = col; v = values.  For zero-copy interoperation: read them freely;
+ * writes through them bypass the copies the library derives from the
+ * arrays (the split CSR layout, the mixed CG's FP32 values, the COO row
+ * pointer), so re-import the block after changing them. */
 int tbsla_hip_device_arrays(tbsla_hip_matrix_t m, void **d_a, void **d_b, void **d_v);
 
 /* ---- SpMV and composites ------------------------------------------------ */
