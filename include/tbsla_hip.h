@@ -15,9 +15,10 @@
  *     tbsla_hip_malloc); h_* are host pointers.
  *   - `stream` is a hipStream_t passed as void* (NULL: the null stream).
  *     Work is enqueued on it; functions that return scalars to the host
- *     (iteration counts, norms) synchronise that stream.  The only stream
- *     the library creates is the one a handle's CG uses when it is given
- *     the null stream (which cannot be captured into a hipGraph).
+ *     (iteration counts, norms) synchronise that stream.  The only streams
+ *     the library creates are the one a handle's CG uses and the one a
+ *     batched PageRank call uses when given the null stream (which cannot
+ *     be captured into a hipGraph).
  *   - Thread-compatible: one handle per thread, or external synchronisation.
  *   - Geometry follows the reference exactly (n_row .. NC, nnz), including
  *     the count-pass nnz of CSR fill_cqmat (MatrixCSR.cpp:423-434) that can
@@ -270,6 +271,22 @@ int tbsla_hip_softmax_rows(tbsla_hip_matrix_t m, int base, double *d_max_abs, do
  * doubles; *nb_iterations and *final_error as the reference reports them. */
 int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double epsilon, int max_iterations,
                         double *d_b, int *nb_iterations, double *final_error, void *stream);
+/* Many PageRank instances at once (no counterpart in the reference): the
+ * instances are the square diagonal blocks of the CSR block m at (0, 0),
+ * instance k owning rows [h_row_offsets[k], h_row_offsets[k+1])
+ * (tbsla_hip_block_diag builds such an m from column-stochastic parts).
+ * Each runs tbsla_hip_page_rank's iteration on its own block, with its own
+ * sum, error and stop (error < epsilon, or max_iterations), and its own
+ * final normalisation; one SpMV per iteration serves every instance.  The
+ * iterations go in batches of 16 captured once into a hipGraph (on a
+ * stream of the library's own when `stream` is NULL).  Instance k follows a
+ * tbsla_hip_page_rank of its block alone bit for bit when both SpMVs use
+ * the same row kernel (tbsla_hip_set_csr_kernel).  d_b receives every
+ * instance's vector (ln_row doubles); h_iterations / h_errors (nullable,
+ * ninst entries) the per-instance counts and final errors. */
+int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, const int *h_row_offsets,
+                                double beta, double epsilon, int max_iterations, double *d_b,
+                                int *h_iterations, double *h_errors, void *stream);
 
 enum { TBSLA_HIP_CG_FP64 = 0, TBSLA_HIP_CG_MIXED = 1 };
 enum { TBSLA_HIP_PRECOND_NONE = 0, TBSLA_HIP_PRECOND_JACOBI = 1, TBSLA_HIP_PRECOND_FACTORED = 2 };

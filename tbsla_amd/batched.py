@@ -15,6 +15,10 @@ fastest is kept.  Per-system solve time is the device wall clock from the
 first iteration to the iteration at which that system's test passed.
 
     python -m tbsla_amd.batched --count 512
+
+``page_rank_bench`` runs the same instance sizes as column-stochastic cqmat
+blocks (c uniform in 8..32, q = 0.1) through tbsla_hip_page_rank_batched:
+damping 0.85, each instance until its L1 change is below 1e-10.
 """
 from __future__ import annotations
 
@@ -44,6 +48,44 @@ def build(count: int, seed: int = 0, q: float = 0.1):
     offs = np.concatenate([[0], np.cumsum(n)]).astype(np.int32)
     rhs = T.uniform_vector(int(offs[-1]), 17, -1.0, 1.0)
     return bd, offs, rhs, n, c
+
+
+def page_rank_bench(count: int = 512, seed: int = 0, beta: float = 0.85,
+                    epsilon: float = 1e-10) -> dict:
+    """Batched PageRank over `count` stochastic cqmat instances of uneven size;
+    host clock around the call (which ends synchronised), after one warm-up
+    call.  GB/s counts one SpMV of the whole block per iteration of the
+    slowest instance."""
+    n, c = instance_sizes(count, seed)
+    t0 = time.perf_counter()
+    parts = []
+    for k in range(count):
+        g = T.Matrix(T.CSR).fill_cqmat(int(n[k]), int(n[k]), int(c[k]), 0.1, 0, value_seed=k)
+        g.make_stochastic()
+        parts.append(g)
+    bd = T.Matrix.block_diag(parts)
+    del parts
+    build_s = time.perf_counter() - t0
+    offs = np.concatenate([[0], np.cumsum(n)]).astype(np.int32)
+    g = bd.geometry()
+    # the block's SpMV row kernel from the autotuner (untimed)
+    dx, dy = T.DeviceBuffer(g.ln_row * 8), T.DeviceBuffer(g.ln_row * 8)
+    T.check(T.lib().tbsla_hip_fill_unit_vector(dx.ptr, g.ln_row, 1, None))
+    bd.autotune(dy, dx, 3)
+    del dx, dy
+    bd.page_rank_batched(offs, beta, epsilon, 10000)
+    t0 = time.perf_counter()
+    _, its, errs = bd.page_rank_batched(offs, beta, epsilon, 10000)
+    ms = (time.perf_counter() - t0) * 1e3
+    spmv_bytes = 12 * g.stored + 4 * (g.ln_row + 1) + 16 * g.ln_row
+    return {"instances": count, "rows": int(offs[-1]), "nnz": g.stored,
+            "csr_kernel": bd.get_csr_kernel(), "csr_layout": bd.get_csr_layout()["layout"],
+            "build_s": round(build_s, 2), "batch_ms": round(ms, 3),
+            "instances_per_s": round(count / (ms * 1e-3), 1),
+            "iterations_p50": float(np.median(its)), "iterations_max": int(its.max()),
+            "max_error": float(errs.max()),
+            "ms_per_iteration": round(ms / max(int(its.max()), 1), 4),
+            "spmv_GBps_equivalent": round(spmv_bytes * int(its.max()) / (ms * 1e-3) / 1e9, 1)}
 
 
 def options(A: T.Matrix, lanes: int, max_iterations: int = 5000, tol: float = 1e-8):

@@ -204,6 +204,125 @@ __global__ __launch_bounds__(kBlock) void k_divide(double *__restrict__ a, long 
     a[k] /= d;
 }
 
+// ---- batched PageRank ----------------------------------------------------
+//
+// Instances are the diagonal blocks of one square CSR block.  Instance k
+// owns rows [row0, row0 + n) and the partial slots [part0, part0 + nb) with
+// nb = red_grid(n): its update workgroups walk its rows exactly as the
+// single-block update walks a block of n rows, and its finaliser sums its
+// partials in the same order, so each instance follows the single-block
+// solve of its own block bit for bit (given the same SpMV row kernel).
+
+struct PrInstance {
+  double S;     // sum of the current iterate
+  double err;   // last L1 change
+  int iters;    // iterations done
+  int done;     // converged or at the iteration cap
+  long long row0, n;
+  int part0, nb;
+};
+
+struct PrCommon {
+  int ndone;     // instances finished
+  int all_done;  // every instance finished: the SpMV row kernel returns at once
+};
+
+__global__ __launch_bounds__(kBlock) void k_prb_update(PrInstance *__restrict__ inst,
+                                                       const int *__restrict__ block_inst,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ bt,
+                                                       double *__restrict__ bn, double beta,
+                                                       double *__restrict__ p_err,
+                                                       double *__restrict__ p_sum) {
+  TBSLA_NO_FMA
+  const int gb = blockIdx.x;
+  const PrInstance &I = inst[block_inst[gb]];
+  if (I.done) return;
+  const int b = gb - I.part0;
+  const double S = I.S;
+  const double t_sum = S * ((1 - beta) / (double)I.n);
+  double e = 0.0, sm = 0.0;
+  for (long long k = (long long)b * kBlock + threadIdx.x; k < I.n; k += (long long)I.nb * kBlock) {
+    const long long i = I.row0 + k;
+    double v = y[i];
+    v = beta * v + t_sum;
+    v = v / S;
+    bn[i] = v;
+    e += fabs(v - bt[i]);
+    sm += v;
+  }
+  e = block_sum(e);
+  sm = block_sum(sm);
+  if (threadIdx.x == 0) {
+    p_err[gb] = e;
+    p_sum[gb] = sm;
+  }
+}
+
+// one workgroup per instance
+__global__ __launch_bounds__(kBlock) void k_prb_finalize(PrInstance *__restrict__ inst, int ninst,
+                                                         const double *__restrict__ p_err,
+                                                         const double *__restrict__ p_sum,
+                                                         double epsilon, int max_iterations,
+                                                         PrCommon *__restrict__ com) {
+  PrInstance &I = inst[blockIdx.x];
+  if (I.done) return;
+  double e = sum_partials(p_err + I.part0, I.nb);
+  double sm = sum_partials(p_sum + I.part0, I.nb);
+  if (threadIdx.x == 0) {
+    e /= 1;  // NC = 1: one block per instance
+    I.err = e;
+    I.S = sm;
+    I.iters += 1;
+    if (e < epsilon || I.iters >= max_iterations) {
+      I.done = 1;
+      if (atomicAdd(&com->ndone, 1) + 1 == ninst) com->all_done = 1;
+    }
+  }
+}
+
+// sum of each instance's final iterate (buffer iters % 2), per-workgroup partials
+__global__ __launch_bounds__(kBlock) void k_prb_norm_partial(const PrInstance *__restrict__ inst,
+                                                             const int *__restrict__ block_inst,
+                                                             const double *__restrict__ b0,
+                                                             const double *__restrict__ b1,
+                                                             double *__restrict__ part) {
+  const int gb = blockIdx.x;
+  const PrInstance &I = inst[block_inst[gb]];
+  const double *src = (I.iters % 2) ? b1 : b0;
+  const int b = gb - I.part0;
+  double v = 0.0;
+  for (long long k = (long long)b * kBlock + threadIdx.x; k < I.n; k += (long long)I.nb * kBlock)
+    v += src[I.row0 + k];
+  v = block_sum(v);
+  if (threadIdx.x == 0) part[gb] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prb_norm_total(const PrInstance *__restrict__ inst,
+                                                           const double *__restrict__ part,
+                                                           double *__restrict__ total) {
+  const PrInstance &I = inst[blockIdx.x];
+  const double v = sum_partials(part + I.part0, I.nb);
+  if (threadIdx.x == 0) total[blockIdx.x] = v;
+}
+
+// out = final iterate / its sum (pagerank_norma_end, Matrix.cpp:149-171)
+__global__ __launch_bounds__(kBlock) void k_prb_divide(const PrInstance *__restrict__ inst,
+                                                       const int *__restrict__ block_inst,
+                                                       const double *__restrict__ b0,
+                                                       const double *__restrict__ b1,
+                                                       const double *__restrict__ total,
+                                                       double *__restrict__ out) {
+  const int gb = blockIdx.x;
+  const int k = block_inst[gb];
+  const PrInstance &I = inst[k];
+  const double *src = (I.iters % 2) ? b1 : b0;
+  const double d = total[k];
+  const int b = gb - I.part0;
+  for (long long j = (long long)b * kBlock + threadIdx.x; j < I.n; j += (long long)I.nb * kBlock)
+    out[I.row0 + j] = src[I.row0 + j] / d;
+}
+
 // ---- CG -----------------------------------------------------------------
 
 unsigned stride_grid(long long n) {
@@ -357,5 +476,143 @@ extern "C" int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double eps
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   if (nb_iterations) *nb_iterations = h.iters;
   if (final_error) *final_error = h.err;
+  return TBSLA_HIP_SUCCESS;
+}
+
+extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, const int *h_row_offsets,
+                                           double beta, double epsilon, int max_iterations,
+                                           double *d_b, int *h_iterations, double *h_errors,
+                                           void *stream) {
+  if (!m || !h_row_offsets || !d_b) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  if (m->format != TBSLA_HIP_CSR)
+    return fail(TBSLA_HIP_ERR_UNSUPPORTED, "batched PageRank needs a CSR block");
+  if (ninst < 1 || max_iterations < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad counts");
+  if (m->ln_row != m->ln_col || m->f_row != m->f_col || m->f_row != 0)
+    return fail(TBSLA_HIP_ERR_INVALID, "batched PageRank needs a square block at (0, 0)");
+  if (h_row_offsets[0] != 0 || h_row_offsets[ninst] != m->ln_row)
+    return fail(TBSLA_HIP_ERR_INVALID, "row offsets must run from 0 to the block's rows");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // instance table and the workgroup -> instance map of the update kernels
+  std::vector<PrInstance> hi(ninst);
+  std::vector<int> hb;
+  for (int k = 0; k < ninst; k++) {
+    const long long n = (long long)h_row_offsets[k + 1] - h_row_offsets[k];
+    if (n < 1) return fail(TBSLA_HIP_ERR_INVALID, "every instance needs at least one row");
+    PrInstance &I = hi[k];
+    I.S = (double)n;  // b = 1 (Matrix.cpp:549-552)
+    I.err = 0.0;
+    I.iters = 0;
+    I.done = 0;
+    I.row0 = h_row_offsets[k];
+    I.n = n;
+    I.part0 = (int)hb.size();
+    I.nb = (int)red_grid(n);
+    for (int b = 0; b < I.nb; b++) hb.push_back(k);
+  }
+  const int nblocks = (int)hb.size();
+  const long long rows = m->ln_row;
+  DevBuf<double> bufA, bufB, y, p_err, p_sum, total;
+  DevBuf<PrInstance> inst;
+  DevBuf<int> binst;
+  DevBuf<PrCommon> com;
+  TBSLA_TRY(bufA.alloc(rows));
+  TBSLA_TRY(bufB.alloc(rows));
+  TBSLA_TRY(y.alloc(rows));
+  TBSLA_TRY(p_err.alloc(nblocks));
+  TBSLA_TRY(p_sum.alloc(nblocks));
+  TBSLA_TRY(total.alloc(ninst));
+  TBSLA_TRY(inst.alloc(ninst));
+  TBSLA_TRY(binst.alloc(nblocks));
+  TBSLA_TRY(com.alloc(1));
+  const PrCommon hc{0, 0};
+  TBSLA_HIP_CHECK(hipMemcpyAsync(inst.p, hi.data(), sizeof(PrInstance) * ninst, hipMemcpyHostToDevice, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(binst.p, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(com.p, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
+  TBSLA_TRY(launch_fill(bufA.p, rows, 1.0, s));
+  double *buf[2] = {bufA.p, bufB.p};
+  // one batch of 16 iterations (an even count: every batch starts on bufA)
+  constexpr int kBatch = 16;
+  auto enqueue_batch = [&](hipStream_t q) -> int {
+    for (int i = 0; i < kBatch; i++) {
+      double *bt = buf[i % 2], *bn = buf[(i + 1) % 2];
+      TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, q, &com.p->all_done));
+      k_prb_update<<<nblocks, kBlock, 0, q>>>(inst.p, binst.p, y.p, bt, bn, beta, p_err.p,
+                                              p_sum.p);
+      k_prb_finalize<<<ninst, kBlock, 0, q>>>(inst.p, ninst, p_err.p, p_sum.p, epsilon,
+                                              max_iterations, com.p);
+    }
+    return TBSLA_HIP_SUCCESS;
+  };
+  // the batch is captured once into a hipGraph and replayed (on a stream of
+  // our own when the caller passes the null stream, which cannot capture);
+  // the split layout's launch is enqueued directly
+  hipStream_t q = s;
+  hipStream_t own = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto cleanup = [&]() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (own) {
+      hipStreamSynchronize(own);
+      hipStreamDestroy(own);
+    }
+  };
+  bool graphs = m->split == nullptr;
+  if (graphs && !s) {
+    if (hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess) graphs = false;
+    else {
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      q = own;
+    }
+  }
+  if (graphs) {
+    int st = TBSLA_HIP_SUCCESS;
+    if (hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      cleanup();
+      return fail(TBSLA_HIP_ERR_HIP, "stream capture failed to begin");
+    }
+    st = enqueue_batch(q);
+    hipError_t e = hipStreamEndCapture(q, &graph);
+    if (st == TBSLA_HIP_SUCCESS && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (st != TBSLA_HIP_SUCCESS || e != hipSuccess) {
+      cleanup();
+      return st != TBSLA_HIP_SUCCESS ? st : fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
+    }
+  }
+  PrCommon h{};
+  int launched = 0;
+  while (launched < max_iterations) {
+    hipError_t e = hipSuccess;
+    if (graphs) e = hipGraphLaunch(exec, q);
+    else {
+      const int st = enqueue_batch(q);
+      if (st) {
+        cleanup();
+        return st;
+      }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, com.p, sizeof(h), hipMemcpyDeviceToHost, q);
+    if (e == hipSuccess) e = hipStreamSynchronize(q);
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
+    }
+    launched += kBatch;
+    if (h.all_done) break;
+  }
+  // every instance: b /= sum(b) from its final buffer
+  k_prb_norm_partial<<<nblocks, kBlock, 0, q>>>(inst.p, binst.p, bufA.p, bufB.p, p_sum.p);
+  k_prb_norm_total<<<ninst, kBlock, 0, q>>>(inst.p, p_sum.p, total.p);
+  k_prb_divide<<<nblocks, kBlock, 0, q>>>(inst.p, binst.p, bufA.p, bufB.p, total.p, d_b);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), inst.p, sizeof(PrInstance) * ninst, hipMemcpyDeviceToHost, q);
+  if (e == hipSuccess) e = hipStreamSynchronize(q);
+  cleanup();
+  if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
+  for (int k = 0; k < ninst; k++) {
+    if (h_iterations) h_iterations[k] = hi[k].iters;
+    if (h_errors) h_errors[k] = hi[k].err;
+  }
   return TBSLA_HIP_SUCCESS;
 }

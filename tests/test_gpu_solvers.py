@@ -198,3 +198,49 @@ def test_cg_relative_jacobi_and_mixed():
     # the oracle's plain CG agrees on the iteration count within a couple
     xo, ito, _ = o.cg(rhs, tolerance=1e-8, relative=True)
     assert abs(ito - it0) <= 2
+
+
+def stochastic_parts(sizes, seed=0):
+    """Column-stochastic cqmat CSR blocks of the given (n, c) and their
+    block diagonal."""
+    parts = []
+    for i, (n, c) in enumerate(sizes):
+        g = T.Matrix(T.CSR).fill_cqmat(n, n, c, 0.1, 0, value_seed=seed + i)
+        g.make_stochastic()
+        parts.append(g)
+    offs = np.concatenate([[0], np.cumsum([n for n, _ in sizes])]).astype(np.int32)
+    return parts, T.Matrix.block_diag(parts), offs
+
+
+def test_page_rank_batched_equals_single():
+    """Every instance of the batch follows the single-block solve of its own
+    block bit for bit (same row kernel), iteration count and error included,
+    and the oracle's PageRank within 1e-12 with the same count."""
+    sizes = [(4096, 8), (777, 31), (20000, 16), (3, 2), (9000, 20), (50000, 32)]
+    parts, bd, offs = stochastic_parts(sizes)
+    bd.set_csr_kernel(8, 1, 1, 1)
+    b, its, errs = bd.page_rank_batched(offs, 0.85, 1e-10, 500)
+    for k, (g, (n, c)) in enumerate(zip(parts, sizes)):
+        g.set_csr_kernel(8, 1, 1, 1)
+        bs, it, err = g.page_rank(0.85, 1e-10, 500)
+        assert its[k] == it and errs[k] == err, (k, its[k], it, errs[k], err)
+        assert np.array_equal(b[offs[k]:offs[k + 1]], bs), k
+        o = O.fill_cqmat(O.CSR, n, n, c, 0.1, 0, value_seed=k)
+        o.make_stochastic()
+        bo, ito, _ = o.page_rank(0.85, 1e-10, 500)
+        assert ito == it and rel_max(bs, bo) < 1e-12, (k, ito, it)
+
+
+def test_page_rank_batched_iteration_cap_and_one_instance():
+    sizes = [(3000, 12), (6000, 24)]
+    parts, bd, offs = stochastic_parts(sizes, seed=7)
+    b, its, errs = bd.page_rank_batched(offs, 0.85, 1e-30, 21)  # never converges: capped
+    assert list(its) == [21, 21]
+    for k, g in enumerate(parts):
+        bd_k = b[offs[k]:offs[k + 1]]
+        assert abs(bd_k.sum() - 1.0) < 1e-12
+    one, its1, _ = parts[1].page_rank_batched(np.array([0, 6000], np.int32), 0.85, 1e-10, 500)
+    single, it1, _ = parts[1].page_rank(0.85, 1e-10, 500)
+    assert its1[0] == it1 and np.array_equal(one, single)
+    with pytest.raises(T.TbslaHipError):
+        bd.page_rank_batched(np.array([0, 100, 8000], np.int32))  # offsets must end at ln_row
