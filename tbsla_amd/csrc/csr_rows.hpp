@@ -34,7 +34,24 @@ struct CsrRowsArgs {
   const int *rptr;
   const int *rpos;
   const double *P;
+  // optional row segments (instances of a batch): rows [seg_offs[k],
+  // seg_offs[k+1]); a wavefront whose rows all lie in one segment with
+  // seg_done[k] != 0 returns at once (its rows of y are left as they are)
+  const int *seg_offs = nullptr;
+  int nseg = 0;
+  const int *seg_done = nullptr;
 };
+
+// the segment holding row: the largest k with seg_offs[k] <= row
+__device__ inline int seg_of_row(const int *offs, int nseg, long long row) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= row) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 template <int V, int NT>
 __device__ inline void csr_load(const int *ci, const double *va, int j, int (&c)[V],
@@ -66,6 +83,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_rows(CsrRowsArgs a) {
   const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
   const long long base = wave * (G * R);
   if (base >= a.n_rows) return;
+  if (a.seg_offs) {  // wave-uniform
+    const long long last = min(base + G * R, a.n_rows) - 1;
+    const int k0 = seg_of_row(a.seg_offs, a.nseg, base);
+    if (a.seg_done[k0] && (last < a.seg_offs[k0 + 1])) return;
+  }
   int beg[R], end[R];
   int len = 0;
 #pragma unroll

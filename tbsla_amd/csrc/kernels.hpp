@@ -16,7 +16,8 @@ size_t scan_workspace_bytes(long long n);
 // d_done (device, nullable): CSR and ELL launches return at once when it
 // holds a nonzero value (converged solver iterations queued in a batch).
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
-                bool zero_first, hipStream_t s, const int *d_done = nullptr);
+                bool zero_first, hipStream_t s, const int *d_done = nullptr,
+                const int *seg_offs = nullptr, int nseg = 0, const int *seg_done = nullptr);
 int pick_csr_lanes(const tbsla_hip_matrix_s *m);
 int pick_csr_rows(const tbsla_hip_matrix_s *m, int lanes);
 int pick_csr_vec(const tbsla_hip_matrix_s *m);

@@ -264,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_prb_finalize(PrInstance *__restrict_
                                                          const double *__restrict__ p_err,
                                                          const double *__restrict__ p_sum,
                                                          double epsilon, int max_iterations,
-                                                         PrCommon *__restrict__ com) {
+                                                         PrCommon *__restrict__ com,
+                                                         int *__restrict__ done_flags) {
   PrInstance &I = inst[blockIdx.x];
   if (I.done) return;
   double e = sum_partials(p_err + I.part0, I.nb);
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_prb_finalize(PrInstance *__restrict_
     I.iters += 1;
     if (e < epsilon || I.iters >= max_iterations) {
       I.done = 1;
+      done_flags[blockIdx.x] = 1;  // the SpMV skips the instance's rows from now on
       if (atomicAdd(&com->ndone, 1) + 1 == ninst) com->all_done = 1;
     }
   }
@@ -513,7 +515,7 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   const long long rows = m->ln_row;
   DevBuf<double> bufA, bufB, y, p_err, p_sum, total;
   DevBuf<PrInstance> inst;
-  DevBuf<int> binst;
+  DevBuf<int> binst, d_offs, done_flags;
   DevBuf<PrCommon> com;
   TBSLA_TRY(bufA.alloc(rows));
   TBSLA_TRY(bufB.alloc(rows));
@@ -523,11 +525,21 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   TBSLA_TRY(total.alloc(ninst));
   TBSLA_TRY(inst.alloc(ninst));
   TBSLA_TRY(binst.alloc(nblocks));
+  TBSLA_TRY(d_offs.alloc(ninst + 1));
+  TBSLA_TRY(done_flags.alloc(ninst));
   TBSLA_TRY(com.alloc(1));
   const PrCommon hc{0, 0};
   TBSLA_HIP_CHECK(hipMemcpyAsync(inst.p, hi.data(), sizeof(PrInstance) * ninst, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(binst.p, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(com.p, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(d_offs.p, h_row_offsets, sizeof(int) * (ninst + 1),
+                                 hipMemcpyHostToDevice, s));
+  {
+    const std::vector<int> zeros(ninst, 0);
+    TBSLA_HIP_CHECK(hipMemcpyAsync(done_flags.p, zeros.data(), sizeof(int) * ninst,
+                                   hipMemcpyHostToDevice, s));
+    TBSLA_HIP_CHECK(hipStreamSynchronize(s));  // zeros leaves scope
+  }
   TBSLA_TRY(launch_fill(bufA.p, rows, 1.0, s));
   double *buf[2] = {bufA.p, bufB.p};
   // one batch of 16 iterations (an even count: every batch starts on bufA)
@@ -535,11 +547,12 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   auto enqueue_batch = [&](hipStream_t q) -> int {
     for (int i = 0; i < kBatch; i++) {
       double *bt = buf[i % 2], *bn = buf[(i + 1) % 2];
-      TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, q, &com.p->all_done));
+      TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, q, &com.p->all_done, d_offs.p, ninst,
+                            done_flags.p));
       k_prb_update<<<nblocks, kBlock, 0, q>>>(inst.p, binst.p, y.p, bt, bn, beta, p_err.p,
                                               p_sum.p);
       k_prb_finalize<<<ninst, kBlock, 0, q>>>(inst.p, ninst, p_err.p, p_sum.p, epsilon,
-                                              max_iterations, com.p);
+                                              max_iterations, com.p, done_flags.p);
     }
     return TBSLA_HIP_SUCCESS;
   };

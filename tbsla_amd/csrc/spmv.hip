@@ -346,7 +346,8 @@ int pick_csr_vec(const tbsla_hip_matrix_s *m) { return m->csr_vec > 0 ? m->csr_v
 int pick_csr_nt(const tbsla_hip_matrix_s *m) { return m->csr_nt >= 0 ? m->csr_nt : 1; }
 
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
-                bool zero_first, hipStream_t s, const int *d_done) {
+                bool zero_first, hipStream_t s, const int *d_done, const int *seg_offs, int nseg,
+                const int *seg_done) {
   switch (m->format) {
     case TBSLA_HIP_CSR: {
       // MatrixCSR::Ax returns when nnz == 0 (MatrixCSR.cpp:210-211)
@@ -356,7 +357,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       }
       if (m->split) return split_launch(m, d_r, d_v, d_done, s);
       CsrRowsArgs a{m->ln_row, m->rowptr, m->colidx, m->values, d_v, m->f_col, d_r, d_done,
-                    nullptr, nullptr, nullptr};
+                    nullptr, nullptr, nullptr, seg_offs, nseg, seg_done};
       const int L = pick_csr_lanes(m);
       csr_rows_launch<false>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
       TBSLA_HIP_LAUNCH_CHECK();
