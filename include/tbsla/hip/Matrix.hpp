@@ -183,12 +183,26 @@ class Matrix {
   }
 
   // ---- drivers (src/mpi/include/tbsla/mpi/Matrix.hpp:31-33), one block ---
-  double *page_rank(double beta, double epsilon, int max_iterations, int &nb_iterations_done) {
+  virtual double *page_rank(double beta, double epsilon, int max_iterations,
+                            int &nb_iterations_done) {
     const size_t n = (size_t)geometry().ln_col;
     DeviceArray<double> db(n);
     double err = 0;
     check(tbsla_hip_page_rank(h_, beta, epsilon, max_iterations, db.get(), &nb_iterations_done,
                               &err, nullptr));
+    double *b = new double[n];
+    db.to_host(b, n);
+    return b;
+  }
+  // the max-normalised iteration of tbsla::mpi::MatrixCOO::page_rank
+  // (src/mpi/matrix/MatrixCOO.cpp:108-175); MatrixCOO::page_rank calls it
+  double *page_rank_max_norm(double beta, double epsilon, int max_iterations,
+                             int &nb_iterations_done) {
+    const size_t n = (size_t)geometry().n_col;
+    DeviceArray<double> db(n);
+    double err = 0;
+    check(tbsla_hip_page_rank_max_norm(h_, beta, epsilon, max_iterations, db.get(),
+                                       &nb_iterations_done, &err, nullptr));
     double *b = new double[n];
     db.to_host(b, n);
     return b;
@@ -474,6 +488,12 @@ class MatrixCOO : public Matrix {
     const int st = tbsla_hip_read_mm(h_, fname.c_str(), nullptr);
     if (st == TBSLA_HIP_ERR_UNSUPPORTED) throw MatrixFormatReadException();
     check(st);
+  }
+  // tbsla::mpi::MatrixCOO overrides page_rank with the max-normalised
+  // iteration (src/mpi/matrix/MatrixCOO.cpp:108-175)
+  double *page_rank(double beta, double epsilon, int max_iterations,
+                    int &nb_iterations_done) override {
+    return page_rank_max_norm(beta, epsilon, max_iterations, nb_iterations_done);
   }
 };
 

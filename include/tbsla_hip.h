@@ -271,6 +271,16 @@ int tbsla_hip_softmax_rows(tbsla_hip_matrix_t m, int base, double *d_max_abs, do
  * doubles; *nb_iterations and *final_error as the reference reports them. */
 int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double epsilon, int max_iterations,
                         double *d_b, int *nb_iterations, double *final_error, void *stream);
+/* tbsla::mpi::MatrixCOO::page_rank (src/mpi/matrix/MatrixCOO.cpp:108-175),
+ * the COO override of the above, on a block covering a square matrix (any
+ * COO block; other formats with NR = NC = 1): b starts at 1; per iteration
+ * y = A b_t, T = (T + sum(b_t))(1-beta)/n (T carries over between
+ * iterations as in the reference's reduction, starting at 0), b = beta y + T,
+ * b /= max(b), error = sum|b - b_t|; stops when error < epsilon or after
+ * max_iterations; finally b /= sum(b).  d_b receives n doubles. */
+int tbsla_hip_page_rank_max_norm(tbsla_hip_matrix_t m, double beta, double epsilon,
+                                 int max_iterations, double *d_b, int *nb_iterations,
+                                 double *final_error, void *stream);
 /* Many PageRank instances at once (no counterpart in the reference): the
  * instances are the square diagonal blocks of the CSR block m at (0, 0),
  * instance k owning rows [h_row_offsets[k], h_row_offsets[k+1])

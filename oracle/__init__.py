@@ -122,6 +122,8 @@ def lib() -> C.CDLL:
                                        dp]
         _lib.orc_page_rank_source.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
                                               C.c_int, C.c_int, dp, dp]
+        _lib.orc_page_rank_max_norm.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
+                                                C.c_int, dp, dp]
         _lib.orc_cg.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int, dp]
         _lib.orc_cg_clamp.argtypes = [C.c_int, C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int,
                                       C.c_double, dp]
@@ -334,6 +336,15 @@ class Matrix:
         it = lib().orc_page_rank(self.fmt, self.ptr, beta, epsilon, max_iterations, _dp(b),
                                  C.byref(err))
         return b[: self.s.g.ln_col], it, err.value
+
+    def page_rank_max_norm(self, beta=0.85, epsilon=1e-10, max_iterations=10000):
+        """MatrixCOO's max-normalised PageRank (src/mpi/matrix/MatrixCOO.cpp:108-175)."""
+        n = self.s.g.n_col
+        b = np.zeros(max(n, 1))
+        err = C.c_double()
+        it = lib().orc_page_rank_max_norm(self.fmt, self.ptr, beta, epsilon, max_iterations,
+                                          _dp(b), C.byref(err))
+        return b[:n], it, err.value
 
     def page_rank_source(self, source, beta=0.85, epsilon=1e-10, max_iterations=10000):
         """Personalised PageRank: teleport to `source` only."""

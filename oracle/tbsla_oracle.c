@@ -1277,6 +1277,51 @@ int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, in
   return it;
 }
 
+/* tbsla::mpi::MatrixCOO::page_rank (src/mpi/matrix/MatrixCOO.cpp:108-175)
+ * at one rank, on a block covering a square matrix: b = 1 (:115-118); per
+ * iteration b_t = b, b = A b_t (:121-129), T = (T + sum(b_t))(1-beta)/n --
+ * the OpenMP reduction adds into T's previous value (:130-134), which starts
+ * at 0 here (uninitialised in the reference) -- then b = beta b + T, max over
+ * b (:136-142), b /= max, error = sum |b - b_t| (:144-149); stop when
+ * error < epsilon (:150-152).  Finally b /= sum(b) (:158-168). */
+int orc_page_rank_max_norm(int fmt, const void *m, double beta, double epsilon,
+                           int max_iterations, double *b, double *err_out) {
+  const orc_geom *g = geom_of(fmt, m);
+  int n = g->n_col;
+  double *b_t = (double *)zalloc((size_t)n, sizeof(double));
+  for (int i = 0; i < n; i++) b[i] = 1;
+  int it = 0, converged = 0;
+  double error = 0, T = 0;
+  while (!converged && it < max_iterations) {
+    for (int i = 0; i < n; i++) {
+      b_t[i] = b[i];
+      b[i] = 0;
+    }
+    orc_Ax(fmt, m, b, b_t, 0);
+    double sum = 0;
+    for (int i = 0; i < n; i++) sum += b_t[i];
+    T = (T + sum) * ((1 - beta) / n);
+    double mx = 0;
+    for (int i = 0; i < n; i++) {
+      b[i] = beta * b[i] + T;
+      if (i == 0 || mx < b[i]) mx = b[i];
+    }
+    error = 0;
+    for (int i = 0; i < n; i++) {
+      b[i] = b[i] / mx;
+      error += fabs(b[i] - b_t[i]);
+    }
+    if (error < epsilon) converged = 1;
+    it++;
+  }
+  double tot = 0;
+  for (int i = 0; i < n; i++) tot += b[i];
+  for (int i = 0; i < n; i++) b[i] /= tot;
+  if (err_out) *err_out = error;
+  free(b_t);
+  return it;
+}
+
 /* ------------------------------------------------------------------------ */
 /* CG                                                                        */
 /* ------------------------------------------------------------------------ */

@@ -129,6 +129,8 @@ void orc_get_row_max_abs(int fmt, const void *m, double *max_abs);
 void orc_apply_exponential(int fmt, void *m, const double *max_abs, int base);
 int orc_page_rank(int fmt, const void *m, double beta, double epsilon, int max_iterations,
                   double *b, double *err_out);
+int orc_page_rank_max_norm(int fmt, const void *m, double beta, double epsilon,
+                           int max_iterations, double *b, double *err_out);
 int orc_page_rank_source(int fmt, const void *m, double beta, double epsilon, int max_iterations,
                          int source, double *b, double *err_out);
 int orc_cg(int fmt, const void *m, const double *rhs, double *x, int max_iterations,

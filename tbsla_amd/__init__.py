@@ -107,6 +107,8 @@ SIGNATURES = [
     ("tbsla_hip_softmax_rows", C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     ("tbsla_hip_page_rank", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
                                       C.POINTER(C.c_double), _vp]),
+    ("tbsla_hip_page_rank_max_norm", C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _vp, _ip,
+                                      C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_page_rank_batched", C.c_int, [_vp, C.c_int, _ip, C.c_double, C.c_double, C.c_int,
                                               _vp, _ip, C.POINTER(C.c_double), _vp]),
@@ -492,6 +494,16 @@ class Matrix:
         it, err = C.c_int(), C.c_double()
         check(lib().tbsla_hip_page_rank(self.h, beta, epsilon, max_iterations, db.ptr,
                                         C.byref(it), C.byref(err), None))
+        return db.to_numpy(np.float64, n), it.value, err.value
+
+    def page_rank_max_norm(self, beta=0.85, epsilon=1e-10, max_iterations=10000):
+        """tbsla::mpi::MatrixCOO::page_rank's max-normalised iteration
+        (src/mpi/matrix/MatrixCOO.cpp:108-175); returns (b, iterations, error)."""
+        n = self.geometry().n_col
+        db = DeviceBuffer(max(n, 1) * 8)
+        it, err = C.c_int(), C.c_double()
+        check(lib().tbsla_hip_page_rank_max_norm(self.h, beta, epsilon, max_iterations, db.ptr,
+                                                 C.byref(it), C.byref(err), None))
         return db.to_numpy(np.float64, n), it.value, err.value
 
     def page_rank_batched(self, row_offsets, beta=0.85, epsilon=1e-10, max_iterations=10000):

@@ -204,6 +204,119 @@ __global__ __launch_bounds__(kBlock) void k_divide(double *__restrict__ a, long 
     a[k] /= d;
 }
 
+// ---- max-normalised PageRank ---------------------------------------------
+//
+// tbsla::mpi::MatrixCOO::page_rank (src/mpi/matrix/MatrixCOO.cpp:108-175)
+// on one block covering a square matrix: b = 1; per iteration b_t = b,
+// y = A b_t, T = (T + sum(b_t)) (1 - beta) / n (the reduction adds into the
+// previous iteration's T, :131-135; T starts at 0 here, the reference leaves
+// it uninitialised), b = beta y + T, b /= max(b), err = sum |b - b_t|;
+// finally b /= sum(b).  max(b) is taken from the extremes of y:
+// fl(beta y + T) is monotone in y (nondecreasing for beta >= 0,
+// nonincreasing otherwise), so max_k fl(beta y_k + T) is fl(beta y_max + T)
+// (or y_min), the same double the reference's loop finds.
+
+struct PrmState {
+  double S;     // sum of the current iterate b_t
+  double T;     // teleportation term of the current iteration
+  double maxv;  // max of beta y + T
+  double err;   // last L1 change
+  int iters;
+  int done;
+};
+
+// partial max and min of y
+__global__ __launch_bounds__(kBlock) void k_prm_extremes(const double *__restrict__ y, long long n,
+                                                         const PrmState *__restrict__ st,
+                                                         double *__restrict__ p_max,
+                                                         double *__restrict__ p_min) {
+  __shared__ double lmax[kBlock / kWave], lmin[kBlock / kWave];
+  if (st->done) return;
+  double hi = -INFINITY, lo = INFINITY;
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
+       k += (long long)gridDim.x * kBlock) {
+    const double v = y[k];
+    hi = fmax(hi, v);
+    lo = fmin(lo, v);
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    hi = fmax(hi, __shfl_xor(hi, o, kWave));
+    lo = fmin(lo, __shfl_xor(lo, o, kWave));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    lmax[threadIdx.x / kWave] = hi;
+    lmin[threadIdx.x / kWave] = lo;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; w++) {
+      hi = fmax(hi, lmax[w]);
+      lo = fmin(lo, lmin[w]);
+    }
+    p_max[blockIdx.x] = hi;
+    p_min[blockIdx.x] = lo;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prm_max(PrmState *__restrict__ st,
+                                                    const double *__restrict__ p_max,
+                                                    const double *__restrict__ p_min, int nb,
+                                                    double beta) {
+  TBSLA_NO_FMA
+  if (st->done || threadIdx.x != 0) return;
+  double hi = -INFINITY, lo = INFINITY;
+  for (int i = 0; i < nb; i++) {
+    hi = fmax(hi, p_max[i]);
+    lo = fmin(lo, p_min[i]);
+  }
+  const double e = beta >= 0 ? hi : lo;
+  st->maxv = beta * e + st->T;
+}
+
+// b = (beta y + T) / maxv; partials of |b - b_t| and of sum(b)
+__global__ __launch_bounds__(kBlock) void k_prm_update(long long n, const double *__restrict__ y,
+                                                       const double *__restrict__ bt,
+                                                       double *__restrict__ b, double beta,
+                                                       const PrmState *__restrict__ st,
+                                                       double *__restrict__ p_err,
+                                                       double *__restrict__ p_sum) {
+  TBSLA_NO_FMA
+  if (st->done) return;
+  const double T = st->T, mx = st->maxv;
+  double e = 0.0, sm = 0.0;
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n;
+       k += (long long)gridDim.x * kBlock) {
+    double v = beta * y[k] + T;
+    v = v / mx;
+    b[k] = v;
+    e += fabs(v - bt[k]);
+    sm += v;
+  }
+  e = block_sum(e);
+  sm = block_sum(sm);
+  if (threadIdx.x == 0) {
+    p_err[blockIdx.x] = e;
+    p_sum[blockIdx.x] = sm;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prm_finalize(PrmState *__restrict__ st,
+                                                         const double *__restrict__ p_err,
+                                                         const double *__restrict__ p_sum,
+                                                         int nb, double scale, double epsilon) {
+  TBSLA_NO_FMA
+  if (st->done) return;
+  const double e = sum_partials(p_err, nb);
+  const double sm = sum_partials(p_sum, nb);
+  if (threadIdx.x == 0) {
+    st->err = e;
+    st->S = sm;
+    st->T = (st->T + sm) * scale;  // the next iteration's T (:131-135)
+    st->iters += 1;
+    if (e < epsilon) st->done = 1;
+  }
+}
+
 // ---- batched PageRank ----------------------------------------------------
 //
 // Instances are the diagonal blocks of one square CSR block.  Instance k
@@ -475,6 +588,66 @@ extern "C" int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double eps
   k_divide<<<stride_grid(ln_col), kBlock, 0, s>>>(b, ln_col, part.p + 2 * kRedBlocks);
   TBSLA_HIP_LAUNCH_CHECK();
   TBSLA_HIP_CHECK(hipMemcpyAsync(d_b, b, (size_t)ln_col * 8, hipMemcpyDeviceToDevice, s));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (nb_iterations) *nb_iterations = h.iters;
+  if (final_error) *final_error = h.err;
+  return TBSLA_HIP_SUCCESS;
+}
+
+extern "C" int tbsla_hip_page_rank_max_norm(tbsla_hip_matrix_t m, double beta, double epsilon,
+                                            int max_iterations, double *d_b, int *nb_iterations,
+                                            double *final_error, void *stream) {
+  if (!m || !d_b) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
+  const long long n = m->n_col;
+  if (m->n_row != m->n_col ||
+      (m->format != TBSLA_HIP_COO && (m->ln_row != m->n_row || m->ln_col != m->n_col)))
+    return fail(TBSLA_HIP_ERR_UNSUPPORTED,
+                "max-normalised PageRank needs one block covering a square matrix");
+  if (n == 0) {  // nothing to iterate (the reference reads b[0])
+    if (nb_iterations) *nb_iterations = 0;
+    if (final_error) *final_error = 0.0;
+    return TBSLA_HIP_SUCCESS;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevBuf<double> bufA, bufB, y, part;
+  DevBuf<PrmState> st;
+  TBSLA_TRY(bufA.alloc(n));
+  TBSLA_TRY(bufB.alloc(n));
+  TBSLA_TRY(y.alloc(n));
+  TBSLA_TRY(part.alloc(4 * kRedBlocks + 8));
+  TBSLA_TRY(st.alloc(1));
+  TBSLA_TRY(launch_fill(bufA.p, n, 1.0, s));  // b = 1 (:115-118)
+  const double scale = (1 - beta) / (double)n;
+  PrmState h{};
+  h.S = (double)n;
+  h.T = h.S * scale;  // (0 + sum(b_t)) (1 - beta) / n
+  TBSLA_HIP_CHECK(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  const unsigned nb = red_grid(n);
+  double *p_a = part.p, *p_b = part.p + kRedBlocks;
+  double *buf[2] = {bufA.p, bufB.p};
+  int launched = 0;
+  while (launched < max_iterations) {
+    const int batch = std::min(16, max_iterations - launched);
+    for (int i = 0; i < batch; i++, launched++) {
+      double *bt = buf[launched % 2], *bn = buf[(launched + 1) % 2];
+      TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, s, &st.p->done));
+      k_prm_extremes<<<nb, kBlock, 0, s>>>(y.p, n, st.p, p_a, p_b);
+      k_prm_max<<<1, kBlock, 0, s>>>(st.p, p_a, p_b, (int)nb, beta);
+      k_prm_update<<<nb, kBlock, 0, s>>>(n, y.p, bt, bn, beta, st.p, p_a, p_b);
+      k_prm_finalize<<<1, kBlock, 0, s>>>(st.p, p_a, p_b, (int)nb, scale, epsilon);
+    }
+    TBSLA_HIP_LAUNCH_CHECK();
+    TBSLA_HIP_CHECK(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+    if (h.done) break;
+  }
+  double *b = buf[h.iters % 2];
+  // final b /= sum(b) (:159-169)
+  k_partial_sum<<<nb, kBlock, 0, s>>>(b, n, part.p);
+  k_total<<<1, kBlock, 0, s>>>(part.p, (int)nb, part.p + 2 * kRedBlocks);
+  k_divide<<<stride_grid(n), kBlock, 0, s>>>(b, n, part.p + 2 * kRedBlocks);
+  TBSLA_HIP_LAUNCH_CHECK();
+  TBSLA_HIP_CHECK(hipMemcpyAsync(d_b, b, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   if (nb_iterations) *nb_iterations = h.iters;
   if (final_error) *final_error = h.err;

@@ -95,6 +95,44 @@ def test_page_rank_matches_oracle(fmt):
     assert abs(bg.sum() - 1.0) < 1e-12
 
 
+@pytest.mark.parametrize("fmt", [T.COO, T.CSR, T.ELL, T.SCOO, T.DENSE])
+def test_page_rank_max_norm_matches_oracle(fmt):
+    """MatrixCOO's max-normalised PageRank: same iteration count, iterate
+    within a relative 1e-12 of the oracle."""
+    n, c, q = (2000, 12, 0.2) if fmt == T.DENSE else (60000, 16, 0.1)
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, value_seed=0)
+    o.make_stochastic()
+    rp, ci, va = o.arrays()
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+    oc = O.coo_from_arrays(n, n, rows, ci, va)
+    bo, ito, eo = oc.page_rank_max_norm(0.85, 1e-10, 10000)
+    geom = dict(o.geom, nnz=len(rows))
+    g = T.Matrix.from_arrays(T.COO, geom, rows, ci, va)
+    if fmt != T.COO:
+        g = g.to(fmt)
+    bg, itg, eg = g.page_rank_max_norm(0.85, 1e-10, 10000)
+    assert itg == ito, (itg, ito)
+    assert eg < 1e-10 and eo < 1e-10
+    assert rel_max(bg, bo) < 1e-12
+    assert abs(bg.sum() - 1.0) < 1e-12
+
+
+def test_page_rank_max_norm_known_answer_and_geometry():
+    """The 2 x 2 case worked on paper (see the oracle's test); a block that
+    does not cover the matrix is refused."""
+    geom = dict(n_row=2, n_col=2, ln_row=2, ln_col=2, f_row=0, f_col=0, pr=0, pc=0, NR=1,
+                NC=1, nnz=1)
+    g = T.Matrix.from_arrays(T.COO, geom, np.array([0], np.int32), np.array([1], np.int32),
+                             np.array([1.0]))
+    for k, b_want, e_want in [(1, (2 / 3, 1 / 3), 0.5), (3, (21 / 34, 13 / 34), 1 / 21)]:
+        b, it, err = g.page_rank_max_norm(0.5, 1e-300, k)
+        assert it == k and np.allclose(b, b_want, rtol=0, atol=1e-15)
+        assert abs(err - e_want) < 1e-15
+    blk = T.Matrix(T.CSR).fill_cqmat(1000, 1000, 8, 0.1, 0, 0, 0, 2, 2, value_seed=1)
+    with pytest.raises(T.TbslaHipError):
+        blk.page_rank_max_norm()
+
+
 def test_page_rank_max_iterations_and_determinism():
     n = 30000
     g = T.Matrix(T.CSR).fill_cqmat(n, n, 8, 0.3, 2, value_seed=4)

@@ -301,6 +301,47 @@ def test_page_rank_cycle_uniform():
     assert np.allclose(b, 1.0 / n, rtol=0, atol=1e-15)
 
 
+def test_page_rank_max_norm_known_answer():
+    """MatrixCOO's max-normalised PageRank on A = [[0, 1], [0, 0]], beta 0.5,
+    worked on paper: b_t = (1, 1) -> y = (1, 0), T = (0 + 2)/4 -> (1, .5);
+    y = (.5, 0), T = (.5 + 1.5)/4 (the carried T) -> (.75, .5) -> (1, 2/3);
+    y = (2/3, 0), T = (.5 + 5/3)/4 -> (1, 13/21).  Without the carry the
+    third iterate would be (1, 4/7)."""
+    want = {1: ((2 / 3, 1 / 3), 0.5), 2: ((0.6, 0.4), 1 / 6), 3: ((21 / 34, 13 / 34), 1 / 21)}
+    for m in (O.coo_from_arrays(2, 2, [0], [1], [1.0]),
+              O.csr_from_dense(np.array([[0.0, 1.0], [0.0, 0.0]]))):
+        for k, (b_want, e_want) in want.items():
+            b, it, err = m.page_rank_max_norm(0.5, 1e-300, k)
+            assert it == k
+            assert np.allclose(b, b_want, rtol=0, atol=1e-15)
+            assert abs(err - e_want) < 1e-15
+
+
+def test_page_rank_max_norm_fixed_point():
+    """On a column-stochastic A the carried teleport term settles at
+    T = c S / (1 - c), c = (1 - beta)/n, S = sum(b); summing M b = beta A b + T
+    gives the fixed point's max M = beta + (1 - beta) n / (n - 1 + beta), so
+    the result is (M I - beta A)^-1 1 normalised -- a direct solve.  (With
+    T = c S, M would be 1: the plain PageRank vector, 1e-5 away here.)"""
+    n = 300
+    gen = O.fill_cqmat(O.CSR, n, n, 7, 0.2, 3, value_seed=11)
+    m = O.csr_from_dense(_dense_of(gen) + 0.5 * np.eye(n))
+    m.make_stochastic()
+    A = _dense_of(m)
+    beta = 0.85
+    M = beta + (1 - beta) * n / (n - 1 + beta)
+    exact = np.linalg.solve(M * np.eye(n) - beta * A, np.ones(n))
+    exact /= exact.sum()
+    plain = np.linalg.solve(np.eye(n) - beta * A, np.ones(n))
+    plain /= plain.sum()
+    assert np.abs(exact - plain).max() > 1e-6 * exact.max()
+    for mm in (m, O.coo_from_arrays(n, n, *np.nonzero(A), A[np.nonzero(A)])):
+        b, it, err = mm.page_rank_max_norm(beta, 1e-14, 10000)
+        assert err < 1e-14 and it < 10000
+        assert abs(b.sum() - 1.0) < 1e-14
+        assert np.abs(b - exact).max() < 1e-12 * exact.max()
+
+
 def test_cg_solves_laplacian():
     """CG on the 1-D Laplacian tridiag(-1, 2, -1) against a direct solve,
     within n iterations; absolute and relative stopping rules."""
