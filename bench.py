@@ -65,12 +65,18 @@ def measured_traffic(kernel: str):
     return None
 
 
-def csr_kernel_name(k: dict, split: bool) -> str:
-    """The row kernel's name as rocprofv3 prints its template arguments
-    (k_csr_rows<vec, lanes, rows, nontemporal, split>)."""
+def csr_kernel_name(k: dict, layout: int) -> str:
+    """The kernels of one SpMV as rocprofv3 names them: the row kernel with
+    its template arguments (k_csr_rows<vec, lanes, rows, nontemporal,
+    split>), or the diagonal kernel, plus the remote kernels of the split
+    and diagonal layouts."""
+    import tbsla_amd as T
+    if layout == T.CSR_DIAG:
+        return "k_dia_rows + k_remote_products + k_remote_combine"
+    split = layout == T.CSR_SPLIT
     name = "k_csr_rows<%d, %d, %d, %d, %s>" % (k["vec"], k["lanes"], k["rows"], k["nontemporal"],
                                                "true" if split else "false")
-    return name + " + k_remote_products" if split else name
+    return name + " + k_remote_products + k_remote_combine" if split else name
 
 
 def algorithmic_bytes_csr(stored, n_rows, n_cols):
@@ -193,7 +199,8 @@ def formats_section(iters: int = 20):
     """The other cfg2 measurements: SpMV of the ELL, COO, SCOO blocks and of
     the 16384^2 DENSE slice (block (0,0) of a 256 x 256 grid), the CSR blocks
     (0,0) and (0,1) of a 4 x 4 grid, PageRank on the stochastic cfg2 CSR
-    (beta 0.85, epsilon 1e-10), and a 1 GiB device copy.  GB/s over each
+    (beta 0.85, epsilon 1e-10), the Poisson 128^3 SpMV on the row layout and
+    on the autotuner's choice, and a 1 GiB device copy.  GB/s over each
     format's algorithmic bytes (stored entries, x and y once); HIP events
     around `iters` back-to-back SpMVs."""
     import tbsla_amd as T
@@ -253,6 +260,30 @@ def formats_section(iters: int = 20):
                         "note": "host clock around the whole solve (iterations, vector passes, "
                                 "final normalisation), vectors resident"}
     del m, ds
+    # the 7-point Poisson 128^3 stencil: row layout against the autotuner's
+    # choice (the diagonal layout stores its 7 diagonals without indices)
+    side = 128
+    n3 = side ** 3
+    A = T.Matrix(T.CSR).fill_poisson3d(side, side, side)
+    g = A.geometry()
+    nbytes = algorithmic_bytes_csr(g.stored, g.ln_row, g.ln_col)
+    x3, y3 = T.DeviceBuffer(n3 * 8), T.DeviceBuffer(n3 * 8)
+    T.check(T.lib().tbsla_hip_fill_unit_vector(x3.ptr, n3, 2, None))
+    A.autotune(y3, x3, 10)
+    lay = A.get_csr_layout()
+
+    def timed3():
+        A.time_spmv(y3, x3, 2)
+        us = A.time_spmv(y3, x3, iters) / iters * 1e3
+        return {"us": round(us, 1), "GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+                "frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    tuned = timed3()
+    A.set_csr_layout(T.CSR_ROWS)
+    out["poisson128_spmv"] = {"stored": g.stored, "bytes_csr": nbytes,
+                              "autotuned": {"layout": lay["layout"], **tuned},
+                              "row_layout": timed3(),
+                              "note": "GB/s over the CSR algorithmic bytes; layout 2 = diagonal"}
+    del A, x3, y3
     # a plain device copy for reference (read + write)
     B = 1 << 30
     a, b = T.DeviceBuffer(B), T.DeviceBuffer(B)
@@ -367,7 +398,7 @@ def main():
     gbs = nbytes / (per * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": csr_kernel_name(m.get_csr_kernel(), layout["layout"] == T.CSR_SPLIT),
+            "kernel": csr_kernel_name(m.get_csr_kernel(), layout["layout"]),
             "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
     # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch

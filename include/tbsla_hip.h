@@ -441,24 +441,34 @@ int tbsla_hip_set_csr_kernel(tbsla_hip_matrix_t m, int lanes, int rows, int vec,
 int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *vec,
                              int *nontemporal);
 /* CSR / ELL execution layout (no counterpart in the reference; results
- * within rounding of the row-major sum; ELL padding slots count as entries).  TBSLA_HIP_CSR_ROWS: the CSR arrays as
- * they are.  TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in
- * row order; the others are grouped by column slice (slices a multiple of 8,
+ * within rounding of the row-major sum; ELL padding slots count as
+ * entries).  TBSLA_HIP_CSR_ROWS: the CSR arrays as they are.
+ * TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in row order;
+ * the others ("remote") are grouped by column slice (slices a multiple of 8,
  * at most 64) and multiplied by one XCD per slice, whose L2 then holds that
- * slice of x; the row kernel adds both parts.  The split copies the entries
- * once (about 24 bytes per remote entry), is rebuilt by normalize_cols and
- * dropped when the arrays change. */
-enum { TBSLA_HIP_CSR_ROWS = 0, TBSLA_HIP_CSR_SPLIT = 1 };
+ * slice of x; their products are summed per row in slice order and added to
+ * the row kernel's sum.  TBSLA_HIP_CSR_DIAG: the diagonals c - r within
+ * +-window (at most 1048576) holding nonzero entries in at least a quarter
+ * of the rows (up to 64, the fullest) are stored densely, tiled by 64 rows
+ * (8 bytes a slot, no column indices); every other entry -- off those
+ * diagonals, explicit zeros, a second entry on a row's slot -- is remote.
+ * Inf and NaN in x reach the same rows as in the row-major sum.  Both copy
+ * the entries once, are rebuilt by normalize_cols and dropped when the
+ * arrays change; a block without columns cannot take the diagonal layout. */
+enum { TBSLA_HIP_CSR_ROWS = 0, TBSLA_HIP_CSR_SPLIT = 1, TBSLA_HIP_CSR_DIAG = 2 };
 int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int slices,
                              void *stream);
 int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int *slices,
                              long long *remote_entries);
 /* Autotuner (no reference counterpart): times every compiled CSR row
  * kernel (tbsla_hip_set_csr_kernel: lanes x rows x vec x nontemporal) on the
- * row layout and on the split layout (window 64, 8 and 16 slices) on this
- * block with d_v / d_r (`iters` launches each) and keeps the fastest; ELL
- * compares its column-major kernel with the split layout; other formats
- * have a single schedule.  *best_us receives its time per SpMV. */
+ * row layout and on the split layout (window 64, 8 and 16 slices), and the
+ * diagonal layout (window 65536, 8 and 16 slices), on this block with d_v /
+ * d_r (`iters` launches each) and keeps the fastest; a split or diagonal
+ * layout is kept only when it is at least 3 % faster than the row layout.
+ * ELL compares its column-major kernel with the split and diagonal layouts
+ * by the same rule; other formats have a single schedule.  *best_us
+ * receives its time per SpMV. */
 int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                        float *best_us, void *stream);
 

@@ -42,7 +42,7 @@ class CGResult(C.Structure):
 
 CG_FP64, CG_MIXED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_FACTORED = 0, 1, 2
-CSR_ROWS, CSR_SPLIT = 0, 1
+CSR_ROWS, CSR_SPLIT, CSR_DIAG = 0, 1, 2
 FACTOR_AUTO, FACTOR_SEPARATE, FACTOR_FUSED, FACTOR_PRODUCT = 0, 1, 2, 3
 CG_CONVERGED, CG_MAX_ITERATIONS, CG_BREAKDOWN, CG_INACTIVE = 0, 1, 2, 3
 ERR_BREAKDOWN = 6

@@ -486,29 +486,41 @@ int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *v
   return TBSLA_HIP_SUCCESS;
 }
 
+// diagonal layout: offsets searched for dense diagonals (wide enough for a
+// 3-D stencil's +-n^2 diagonals up to n = 256)
+constexpr int kDiagWindow = 65536;
+constexpr float kDerivedGain = 0.97f;
+
 int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                        float *best_us, void *stream) {
   if (!m || iters < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad argument");
   float ms = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (m->format == TBSLA_HIP_ELL && m->has_arrays && m->max_col > 0 && m->ln_row > 0) {
-    // ELL: its column-major kernel, or the split layout (8 or 16 slices)
+    // ELL: its column-major kernel, or the split or diagonal layout (8 or
+    // 16 slices)
     split_release(m);
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
     float best = ms;
-    int best_slices = 0;
-    for (int slices : {8, 16}) {
-      TBSLA_TRY(split_build(m, 64, slices, s));
-      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
-      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
-      if (ms < best) {
-        best = ms;
-        best_slices = slices;
+    const float ell_ms = ms;
+    int best_slices = 0, best_layout = 0, best_window = 0;
+    for (int layout : {TBSLA_HIP_CSR_SPLIT, TBSLA_HIP_CSR_DIAG})
+      for (int slices : {8, 16}) {
+        if (layout == TBSLA_HIP_CSR_DIAG && m->ln_col == 0) break;
+        const int window = layout == TBSLA_HIP_CSR_DIAG ? kDiagWindow : 64;
+        TBSLA_TRY(split_build(m, window, slices, s, layout));
+        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
+        TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+        if (ms < best && ms < ell_ms * kDerivedGain) {
+          best = ms;
+          best_slices = slices;
+          best_layout = layout;
+          best_window = window;
+        }
       }
-    }
     split_release(m);
-    if (best_slices) TBSLA_TRY(split_build(m, 64, best_slices, s));
+    if (best_slices) TBSLA_TRY(split_build(m, best_window, best_slices, s, best_layout));
     if (best_us) *best_us = best * 1000.0f / iters;
     return TBSLA_HIP_SUCCESS;
   }
@@ -524,6 +536,10 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
     int layout = -1, window = 0, slices = 0, L = 0, R = 0, V = 1, NT = 1;
     float ms = 0;
   } best;
+  // a derived layout (a second copy of the entries) is kept only when it
+  // beats the row layout's best time by kDerivedGain
+  float rows_ms = 0;
+  auto derived_wins = [&](float t) { return t < best.ms && t < rows_ms * kDerivedGain; };
   auto sweep = [&](int layout, int window, int slices) -> int {
     for (int V : {1, 2, 4})
       for (int L = 1; L <= 64; L <<= 1)
@@ -536,12 +552,15 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
             m->csr_nt = NT;
             TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));  // warm
             TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
-            if (best.layout < 0 || ms < best.ms) best = Best{layout, window, slices, L, R, V, NT, ms};
+            const bool win = layout == TBSLA_HIP_CSR_ROWS ? (best.layout < 0 || ms < best.ms)
+                                                          : derived_wins(ms);
+            if (win) best = Best{layout, window, slices, L, R, V, NT, ms};
           }
     return TBSLA_HIP_SUCCESS;
   };
   split_release(m);
   TBSLA_TRY(sweep(TBSLA_HIP_CSR_ROWS, 0, 0));
+  rows_ms = best.ms;
   constexpr int kWindow = 64;
   for (int slices : {8, 16}) {
     if (!m->has_arrays || m->ln_row == 0) break;
@@ -551,8 +570,18 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
     if (remote == 0) break;
     TBSLA_TRY(sweep(TBSLA_HIP_CSR_SPLIT, kWindow, slices));
   }
+  // the diagonal layout: its own kernel, one timing per slice count
+  for (int slices : {8, 16}) {
+    if (!m->has_arrays || m->ln_row == 0 || m->ln_col == 0) break;
+    TBSLA_TRY(split_build(m, kDiagWindow, slices, s, TBSLA_HIP_CSR_DIAG));
+    TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
+    TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+    if (derived_wins(ms))
+      best = Best{TBSLA_HIP_CSR_DIAG, kDiagWindow, slices, best.L, best.R, best.V, best.NT, ms};
+  }
   split_release(m);
-  if (best.layout == TBSLA_HIP_CSR_SPLIT) TBSLA_TRY(split_build(m, best.window, best.slices, s));
+  if (best.layout == TBSLA_HIP_CSR_SPLIT || best.layout == TBSLA_HIP_CSR_DIAG)
+    TBSLA_TRY(split_build(m, best.window, best.slices, s, best.layout));
   m->lanes_per_row = best.L;
   m->rows_per_group = best.R;
   m->csr_vec = best.V;

@@ -30,10 +30,8 @@ struct CsrRowsArgs {
   int f_col;
   double *y;
   const int *done;
-  // split layout: the row's remote products P[rpos[rptr[row] ..]]
-  const int *rptr;
-  const int *rpos;
-  const double *P;
+  // split layout: y[row] = the row's sum + add[row] (its remote products)
+  const double *add;
   // optional row segments (instances of a batch): rows [seg_offs[k],
   // seg_offs[k+1]); a wavefront whose rows all lie in one segment with
   // seg_done[k] != 0 returns at once (its rows of y are left as they are)
@@ -134,19 +132,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_rows(CsrRowsArgs a) {
 #pragma unroll
       for (int k = 0; k < V; k++) sum[r] = fma(v[r][k], a.x[c[r][k]], sum[r]);
   }
-  if (SPLIT) {
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const long long row = base + r * G + g;
-      if (row < a.n_rows)
-        for (int j = a.rptr[row] + lane; j < a.rptr[row + 1]; j += L) sum[r] += a.P[a.rpos[j]];
-    }
-  }
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const double s = group_sum<L>(sum[r]);
     const long long row = base + r * G + g;
-    if (lane == 0 && row < a.n_rows) a.y[row] = s;
+    if (lane == 0 && row < a.n_rows) a.y[row] = SPLIT ? s + a.add[row] : s;
   }
 }
 

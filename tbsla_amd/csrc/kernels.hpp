@@ -51,8 +51,9 @@ int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **co
 // split.hip: the split CSR layout (near-diagonal + column-sliced remote)
 int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
                  hipStream_t s);
-int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s);
-int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices);
+int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s,
+                int layout = TBSLA_HIP_CSR_SPLIT);
+int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *layout);
 
 // solvers.hip: after the values of m changed in place: rebuild the split
 // layout (it holds a copy of the values), mark copies of the old values

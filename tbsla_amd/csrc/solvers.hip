@@ -461,8 +461,8 @@ struct DevBuf {
 namespace tbsla_hip {
 
 int values_changed(tbsla_hip_matrix_s *m, hipStream_t s) {
-  int win = 0, sl = 0;  // the split layout copies values: rebuild it
-  if (split_params(m, &win, &sl)) TBSLA_TRY(split_build(m, win, sl, s));
+  int win = 0, sl = 0, lay = 0;  // the split layouts copy values: rebuild
+  if (split_params(m, &win, &sl, &lay)) TBSLA_TRY(split_build(m, win, sl, s, lay));
   m->values_version++;  // copies made from the old values are stale
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   return TBSLA_HIP_SUCCESS;

@@ -4,22 +4,26 @@
 // Why: a row's entries near its diagonal gather x from lines that the next
 // rows reuse, but entries far from it (e.g. cqmat's rewired columns,
 // MatrixCSR.cpp:373-486 with q > 0) gather uniformly over all of x, which is
-// several times one XCD's 4 MB L2; every such gather then pulls a whole line
-// from the MALL or HBM for 8 useful bytes.  Here the remote entries are
-// sorted by (column slice, row) and multiplied by a kernel whose workgroups
-// for slice s all run on XCD s mod 8 (workgroups are dispatched to the eight
-// XCDs round-robin), so each XCD keeps its slice of x in its own L2.  The
-// products go to an array P in that order; the row kernel then adds, per row,
-// its near-diagonal products (streamed CSR) and its remote products P[pos]
-// (row-major list, contiguous per row block).  Every sum is taken in a fixed
-// order, so results are deterministic; they differ from the row-major CSR
-// sum only by rounding.
+// several times one XCD's 4 MB L2; every such gather then pulls a whole
+// 128-byte line from the MALL or HBM for 8 useful bytes.  Here the remote
+// entries are sorted by (column slice, row) and multiplied by a kernel whose
+// workgroups for slice s all run on XCD s mod 8 (workgroups are dispatched
+// to the eight XCDs round-robin), so each XCD keeps its slice of x in its
+// own L2.  The products go to an array P in that order.  A combine kernel
+// then gives each block of kCombineRows rows one workgroup, which reads that
+// block's contiguous run of P in every slice (slice by slice, in order) and
+// writes the row totals R[row]; the row kernel streams the near-diagonal CSR
+// and stores y[row] = its sum + R[row].  Every read of P is sequential.
+// Every sum is taken in a fixed order, so results are deterministic; they
+// differ from the row-major CSR sum only by rounding.
 //
 // Layout (device, owned by the handle):
 //   lrp[ln_row+1], lci[nl], lv[nl]     near-diagonal entries |c - r| <= window
-//   rptr[ln_row+1], rpos[nr]           remote entries per row -> index in P
-//   sc[nr], sv[nr]                     remote entries in (slice, row) order
-//   P[nr]                              their products, rewritten every SpMV
+//   sr[nr], sc[nr], sv[nr]             remote entries in (slice, row) order
+//   boff[slices][nblk+1]               first entry of row block b in slice s
+//   P[nr], R[ln_row]                   products and row totals, rewritten
+//                                      every SpMV
+//   doff[ndiag], dv[tiles]             diagonal layout: stored diagonals
 #include <algorithm>
 #include <vector>
 
@@ -29,22 +33,39 @@
 
 namespace tbsla_hip {
 
+constexpr int kCombineRows = 1024;  // rows per combine workgroup (LDS: 8 KB)
+constexpr int kMaxDiagWindow = 1 << 20;  // diagonal layout: offsets searched
+
+// Diagonal values are stored in tiles of kWave rows: row r's entry on
+// diagonal k sits at (r / kWave) kWave ndiag + k kWave + r % kWave, so one
+// wavefront's loads of a diagonal are one 512-byte piece and its whole slab
+// of diagonals is contiguous.
+__host__ __device__ inline long long dia_at(long long r, int k, int ndiag) {
+  return (r / kWave) * (long long)kWave * ndiag + (long long)k * kWave + (r % kWave);
+}
+
 struct SplitCsr {
-  int window = 0, slices = 0;
-  long long nl = 0, nr = 0;
-  int *lrp = nullptr, *lci = nullptr, *rptr = nullptr, *rpos = nullptr, *sc = nullptr;
-  double *lv = nullptr, *sv = nullptr, *P = nullptr;
+  int layout = 0, window = 0, slices = 0;
+  long long nl = 0, nr = 0, nblk = 0;
+  int ndiag = 0;  // diagonal layout: stored diagonals
+  int *lrp = nullptr, *lci = nullptr, *sr = nullptr, *sc = nullptr, *boff = nullptr;
+  int *doff = nullptr;  // their offsets c - r, ascending
+  double *lv = nullptr, *sv = nullptr, *P = nullptr, *R = nullptr;
+  double *dv = nullptr;  // dv[k n + r]: row r's entry on diagonal k, or 0
   std::vector<long long> soff;  // slice boundaries (host)
   ~SplitCsr() {
     hipDeviceSynchronize();
     hipFree(lrp);
     hipFree(lci);
     hipFree(lv);
-    hipFree(rptr);
-    hipFree(rpos);
+    hipFree(sr);
     hipFree(sc);
     hipFree(sv);
+    hipFree(boff);
     hipFree(P);
+    hipFree(R);
+    hipFree(doff);
+    hipFree(dv);
   }
 };
 
@@ -69,17 +90,37 @@ struct RowSrc {
   }
 };
 
+// Where an entry of row r goes.  Split layout (dmap null): near-diagonal
+// (returns 0) or remote (-1).  Diagonal layout: the slot of diagonal k
+// (returns k + 1) when c - r is a stored diagonal, the value is not zero and
+// the row's slot on it is still free (`used`), else remote; so a zero slot
+// means "no entry" and the DIA kernel skips it.
+__device__ inline int place_of(long long c, long long r, double v, int window,
+                               const int *__restrict__ dmap, unsigned long long &used) {
+  if (!dmap) return is_local(c, r, window) ? 0 : -1;
+  const long long d = c - r;
+  if (d < -window || d > window || !(v != 0.0)) return -1;
+  const int k = dmap[d + window];
+  if (k < 0 || ((used >> k) & 1ULL)) return -1;
+  used |= 1ULL << k;
+  return k + 1;
+}
+
 __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, int window,
                                                         RowSrc src, const int *__restrict__ ci,
+                                                        const double *__restrict__ v,
+                                                        const int *__restrict__ dmap,
                                                         int *__restrict__ lcnt,
                                                         int *__restrict__ rcnt) {
   for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
        r += (long long)gridDim.x * kBlock) {
     int l = 0, q = 0;
+    unsigned long long used = 0;
     const int cnt = src.count(r);
     for (int k = 0; k < cnt; k++) {
-      const long long c = (long long)max(ci[src.at(r, k)] - f_col, 0);
-      if (is_local(c, r, window)) l++;
+      const long long j = src.at(r, k);
+      const long long c = (long long)max(ci[j] - f_col, 0);
+      if (place_of(c, r, v[j], window, dmap, used) >= 0) l++;
       else q++;
     }
     lcnt[r] = l;
@@ -87,31 +128,102 @@ __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, 
   }
 }
 
+// entries per diagonal offset c - r in [-window, window] (nonzero values)
+// over the sampled rows r = 0, step, 2 step, ...; offsets within
+// kLdsHalf of the diagonal are counted in LDS and added once per workgroup
+constexpr int kLdsHalf = 4095;
+__global__ __launch_bounds__(kBlock) void k_diag_hist(long long n, long long step, int f_col,
+                                                      int window, RowSrc src,
+                                                      const int *__restrict__ ci,
+                                                      const double *__restrict__ v,
+                                                      int *__restrict__ hist) {
+  __shared__ int lh[2 * kLdsHalf + 1];
+  for (int i = threadIdx.x; i < 2 * kLdsHalf + 1; i += kBlock) lh[i] = 0;
+  __syncthreads();
+  const long long ns = (n + step - 1) / step;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < ns;
+       i += (long long)gridDim.x * kBlock) {
+    const long long r = i * step;
+    const int cnt = src.count(r);
+    for (int k = 0; k < cnt; k++) {
+      const long long j = src.at(r, k);
+      const long long d = (long long)max(ci[j] - f_col, 0) - r;
+      if (d < -window || d > window || !(v[j] != 0.0)) continue;
+      if (d >= -kLdsHalf && d <= kLdsHalf) atomicAdd(lh + (d + kLdsHalf), 1);
+      else atomicAdd(hist + (d + window), 1);
+    }
+  }
+  __syncthreads();
+  const int lo = max(-window, -kLdsHalf), hi = min(window, kLdsHalf);
+  for (int d = lo + (int)threadIdx.x; d <= hi; d += kBlock)
+    if (lh[d + kLdsHalf]) atomicAdd(hist + (d + window), lh[d + kLdsHalf]);
+}
+
+// y[r] = sum over the stored diagonals k (ascending offset) of
+// dv[k n + r] x[r + doff[k]] (zero slots skipped) + R[r] (when add is not
+// null).  The diagonals go in chunks of eight whose loads are issued
+// together: a last chunk's missing diagonals reread the last one and add
+// nothing.  x is read at the clamped column for every slot (in bounds), and
+// a zero slot's product is not added, so an Inf or NaN in x reaches only
+// the rows that hold an entry on its column.
+__global__ __launch_bounds__(kBlock) void k_dia_rows(long long n, long long ncol, int ndiag,
+                                                     const int *__restrict__ doff,
+                                                     const double *__restrict__ dv,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ add,
+                                                     double *__restrict__ y,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
+  constexpr int U = 8;
+  for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
+       r += (long long)gridDim.x * kBlock) {
+    double sum = 0.0;
+    for (int k = 0; k < ndiag; k += U) {
+      double v[U], xv[U];
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        const int kk = min(k + i, ndiag - 1);
+        v[i] = __builtin_nontemporal_load(dv + dia_at(r, kk, ndiag));
+        xv[i] = x[min(max(r + doff[kk], 0LL), ncol - 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < U; i++)
+        sum = (k + i < ndiag && v[i] != 0.0) ? fma(v[i], xv[i], sum) : sum;
+    }
+    y[r] = add ? sum + add[r] : sum;
+  }
+}
+
 // local entries in row order; remote entries in row-major order with their
 // (slice, row) sort key
 __global__ __launch_bounds__(kBlock) void k_split_fill(
     long long n, long long n_col, int f_col, int window, int slices, int rb, RowSrc src,
-    const int *__restrict__ ci, const double *__restrict__ v,
-    const int *__restrict__ lrp, const int *__restrict__ rptr, int *__restrict__ lci,
-    double *__restrict__ lv, int *__restrict__ rc, double *__restrict__ rv,
-    unsigned long long *__restrict__ keys, int *__restrict__ payload,
+    const int *__restrict__ ci, const double *__restrict__ v, const int *__restrict__ dmap,
+    int ndiag, const int *__restrict__ lrp, const int *__restrict__ rptr, int *__restrict__ lci,
+    double *__restrict__ lv, double *__restrict__ dv, int *__restrict__ rc,
+    double *__restrict__ rv, unsigned long long *__restrict__ keys, int *__restrict__ payload,
     int *__restrict__ scount) {
   for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
        r += (long long)gridDim.x * kBlock) {
-    int pl = lrp[r], pr = rptr[r];
+    int pl = dmap ? 0 : lrp[r], pr = rptr[r];
+    unsigned long long used = 0;
     const int cnt = src.count(r);
     for (int k = 0; k < cnt; k++) {
       const long long j = src.at(r, k);
       // ELL clamps a negative index to 0 (MatrixELL.cpp:198-199)
       const long long c = (long long)max(ci[j] - f_col, 0);
-      if (is_local(c, r, window)) {
+      const double val = v[j];
+      const int at = place_of(c, r, val, window, dmap, used);
+      if (at > 0) {
+        dv[dia_at(r, at - 1, ndiag)] = val;
+      } else if (at == 0) {
         lci[pl] = (int)c;
-        lv[pl] = v[j];
+        lv[pl] = val;
         pl++;
       } else {
         const unsigned long long s = (unsigned long long)(c * slices / n_col);
         rc[pr] = (int)c;
-        rv[pr] = v[j];
+        rv[pr] = val;
         keys[pr] = (s << rb) | (unsigned long long)r;
         payload[pr] = pr;
         atomicAdd(scount + s, 1);
@@ -122,23 +234,78 @@ __global__ __launch_bounds__(kBlock) void k_split_fill(
 }
 
 __global__ __launch_bounds__(kBlock) void k_split_place(long long nr,
-                                                        const int *__restrict__ payload,
+                                                        const unsigned long long *__restrict__ keys,
+                                                        int rb, const int *__restrict__ payload,
                                                         const int *__restrict__ rc,
                                                         const double *__restrict__ rv,
+                                                        int *__restrict__ sr,
                                                         int *__restrict__ sc,
-                                                        double *__restrict__ sv,
-                                                        int *__restrict__ rpos) {
+                                                        double *__restrict__ sv) {
+  const unsigned long long mask = (1ULL << rb) - 1;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < nr;
        e += (long long)gridDim.x * kBlock) {
     const int j = payload[e];
+    sr[e] = (int)(keys[e] & mask);
     sc[e] = rc[j];
     sv[e] = rv[j];
-    rpos[j] = (int)e;
   }
 }
 
+// boff[s (nblk + 1) + b] = the first entry of slice s whose row is at least
+// b kCombineRows (rows ascend within a slice); b = nblk gives the slice end
+__global__ __launch_bounds__(kBlock) void k_block_offsets(const long long *__restrict__ soff,
+                                                          int slices, long long nblk,
+                                                          const int *__restrict__ sr,
+                                                          int *__restrict__ boff) {
+  const long long total = (long long)slices * (nblk + 1);
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kBlock) {
+    const int s = (int)(i / (nblk + 1));
+    const long long row = (i % (nblk + 1)) * kCombineRows;
+    long long lo = soff[s], hi = soff[s + 1];
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (sr[mid] < row) lo = mid + 1;
+      else hi = mid;
+    }
+    boff[i] = (int)lo;
+  }
+}
+
+// R[row] for the rows of block b: per slice in order, each run of one row's
+// products (contiguous, rows ascend) is summed by the thread at its first
+// entry; runs of one slice have distinct rows, so their LDS adds never meet.
+__global__ __launch_bounds__(kBlock) void k_remote_combine(long long n, int slices, long long nblk,
+                                                           const int *__restrict__ boff,
+                                                           const int *__restrict__ sr,
+                                                           const double *__restrict__ P,
+                                                           double *__restrict__ R,
+                                                           const int *__restrict__ done) {
+  __shared__ double acc[kCombineRows];
+  if (done && *done) return;
+  const long long b = blockIdx.x;
+  const long long r0 = b * kCombineRows;
+  for (int i = threadIdx.x; i < kCombineRows; i += kBlock) acc[i] = 0.0;
+  __syncthreads();
+  for (int s = 0; s < slices; s++) {
+    const int e0 = boff[(long long)s * (nblk + 1) + b], e1 = boff[(long long)s * (nblk + 1) + b + 1];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+      const int row = sr[e];
+      if (e == e0 || sr[e - 1] != row) {
+        double t = P[e];
+        for (int k = e + 1; k < e1 && sr[k] == row; k++) t += P[k];
+        acc[row - r0] += t;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < kCombineRows && r0 + i < n; i += kBlock) R[r0 + i] = acc[i];
+}
+
 // P[e] = sv[e] x[sc[e]] for the slices of one pass: workgroup b serves slice
-// pass * 8 + b % 8 (so all of a slice's workgroups land on one XCD), chunk b / 8.
+// pass * 8 + b % 8 (so all of a slice's workgroups land on one XCD), chunk
+// b / 8.  Each thread issues its kRemoteChunk / kBlock loads before the
+// gathers that depend on them.
 __global__ __launch_bounds__(kBlock) void k_remote_products(const long long *__restrict__ soff,
                                                             int pass, int slices,
                                                             const int *__restrict__ sc,
@@ -151,7 +318,20 @@ __global__ __launch_bounds__(kBlock) void k_remote_products(const long long *__r
   if (s >= slices) return;
   const long long b = soff[s] + (long long)(blockIdx.x / 8) * kRemoteChunk;
   const long long e = min(soff[s + 1], b + kRemoteChunk);
-  for (long long k = b + threadIdx.x; k < e; k += kBlock) P[k] = ldg_stream(sv + k) * x[ldg_stream(sc + k)];
+  constexpr int U = kRemoteChunk / kBlock;
+  int c[U];
+  double v[U];
+#pragma unroll
+  for (int i = 0; i < U; i++) {
+    const long long k = b + threadIdx.x + (long long)i * kBlock;
+    c[i] = k < e ? ldg_stream(sc + k) : 0;
+    v[i] = k < e ? ldg_stream(sv + k) : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < U; i++) {
+    const long long k = b + threadIdx.x + (long long)i * kBlock;
+    if (k < e) P[k] = v[i] * x[c[i]];
+  }
 }
 
 unsigned sgrid(long long n) {
@@ -167,11 +347,12 @@ int bits_of(long long v) {
 
 }  // namespace
 
-int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices) {
+int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *layout) {
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
   if (!sp) return 0;
   *window = sp->window;
   *slices = sp->slices;
+  if (layout) *layout = sp->layout;
   return 1;
 }
 
@@ -182,11 +363,11 @@ void split_release(tbsla_hip_matrix_s *m) {
   }
 }
 
-// device copy of the slice offsets lives in the handle's scratch head
+// the device copy of the slice offsets follows P
 int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
                  hipStream_t s) {
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
-  const long long *d_soff = reinterpret_cast<const long long *>(sp->sc == nullptr ? nullptr : sp->P + sp->nr);
+  const long long *d_soff = reinterpret_cast<const long long *>(sp->P + std::max(sp->nr, 1LL));
   if (sp->nr > 0) {
     for (int pass = 0; pass * 8 < sp->slices; pass++) {
       long long chunks = 0;
@@ -197,23 +378,39 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
                                                                   sp->sv, x, sp->P, done);
     }
   }
-  // the near-diagonal entries with the block's CSR schedule, then each
-  // row's remote products (lanes of the group take them in turn)
-  CsrRowsArgs a{m->ln_row, sp->lrp, sp->lci, sp->lv, x, 0, y, done, sp->rptr, sp->rpos, sp->P};
+  if (sp->layout == TBSLA_HIP_CSR_DIAG) {
+    // no remote entries: nothing to combine
+    if (sp->nr > 0)
+      k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(m->ln_row, sp->slices, sp->nblk,
+                                                             sp->boff, sp->sr, sp->P, sp->R, done);
+    k_dia_rows<<<sgrid(m->ln_row), kBlock, 0, s>>>(m->ln_row, std::max(m->ln_col, 1), sp->ndiag,
+                                                   sp->doff, sp->dv, x,
+                                                   sp->nr > 0 ? sp->R : nullptr, y, done);
+    TBSLA_HIP_LAUNCH_CHECK();
+    return TBSLA_HIP_SUCCESS;
+  }
+  k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(m->ln_row, sp->slices, sp->nblk, sp->boff,
+                                                         sp->sr, sp->P, sp->R, done);
+  // the near-diagonal entries with the block's CSR schedule, plus R[row]
+  CsrRowsArgs a{m->ln_row, sp->lrp, sp->lci, sp->lv, x, 0, y, done, sp->R};
   const int L = pick_csr_lanes(m);
   csr_rows_launch<true>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
   TBSLA_HIP_LAUNCH_CHECK();
   return TBSLA_HIP_SUCCESS;
 }
 
-int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
+int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, int layout) {
   split_release(m);
   const long long n = m->ln_row, nc = std::max(m->ln_col, 1);
+  const bool diag = layout == TBSLA_HIP_CSR_DIAG;
   auto *sp = new SplitCsr();
+  sp->layout = layout;
   sp->window = window;
   sp->slices = slices;
+  sp->nblk = (n + kCombineRows - 1) / kCombineRows;
   m->split = sp;
-  int *lcnt = nullptr, *rcnt = nullptr, *rc = nullptr, *pay = nullptr, *scount = nullptr;
+  int *lcnt = nullptr, *rcnt = nullptr, *rptr = nullptr, *rc = nullptr, *pay = nullptr,
+      *scount = nullptr, *hist = nullptr, *dmap = nullptr;
   double *rv = nullptr;
   unsigned long long *keys = nullptr;
   void *ws = nullptr;
@@ -221,12 +418,15 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
     hipStreamSynchronize(s);
     hipFree(lcnt);
     hipFree(rcnt);
+    hipFree(rptr);
     hipFree(rc);
     hipFree(pay);
     hipFree(scount);
     hipFree(rv);
     hipFree(keys);
     hipFree(ws);
+    hipFree(hist);
+    hipFree(dmap);
   };
   auto failed = [&](int st) {
     cleanup();
@@ -237,15 +437,56 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
   if ((st = device_alloc((void **)&lcnt, (size_t)(n + 1) * 4)) ||
       (st = device_alloc((void **)&rcnt, (size_t)(n + 1) * 4)) ||
       (st = device_alloc((void **)&sp->lrp, (size_t)(n + 1) * 4)) ||
-      (st = device_alloc((void **)&sp->rptr, (size_t)(n + 1) * 4)) ||
+      (st = device_alloc((void **)&rptr, (size_t)(n + 1) * 4)) ||
       (st = device_alloc((void **)&scount, (size_t)(slices + 1) * 4)) ||
+      (st = device_alloc((void **)&sp->R, (size_t)n * 8)) ||
+      (st = device_alloc((void **)&sp->boff, (size_t)slices * (size_t)(sp->nblk + 1) * 4)) ||
       (st = device_alloc(&ws, scan_workspace_bytes(n + 1) + 64)))
     return failed(st);
   long long *d_total = reinterpret_cast<long long *>(static_cast<char *>(ws) + scan_workspace_bytes(n + 1));
   RowSrc src{m->rowptr, m->format == TBSLA_HIP_ELL ? m->max_col : 0, n};
-  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, src, m->colidx, lcnt, rcnt);
+  if (diag) {
+    // the stored diagonals: offsets within the window holding at least a
+    // quarter of the rows' worth of nonzero entries (at most 64, the
+    // fullest), ascending
+    const int nb = 2 * window + 1;
+    std::vector<int> h(nb), map(nb, -1);
+    if ((st = device_alloc((void **)&hist, (size_t)nb * 4)) ||
+        (st = device_alloc((void **)&dmap, (size_t)nb * 4)))
+      return failed(st);
+    hipMemsetAsync(hist, 0, (size_t)nb * 4, s);
+    const long long step = std::max(1LL, n / 262144);  // rows sampled: at most 262144
+    k_diag_hist<<<256, kBlock, 0, s>>>(n, step, m->f_col, window, src, m->colidx, m->values, hist);
+    if (hipMemcpyAsync(h.data(), hist, (size_t)nb * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return failed(fail(TBSLA_HIP_ERR_HIP, "diagonal layout: histogram"));
+    std::vector<int> cand;
+    const long long sampled = (n + step - 1) / step;
+    for (int d = 0; d < nb; d++)
+      if ((long long)h[d] * 4 >= sampled && h[d] > 0) cand.push_back(d);
+    std::stable_sort(cand.begin(), cand.end(), [&h](int a, int b) { return h[a] > h[b]; });
+    if (cand.size() > 64) cand.resize(64);
+    std::sort(cand.begin(), cand.end());
+    sp->ndiag = (int)cand.size();
+    // whole tiles of kWave rows
+    const size_t dia_bytes =
+        (size_t)std::max(sp->ndiag, 1) * (size_t)((n + kWave - 1) / kWave) * kWave * 8;
+    std::vector<int> offs(std::max<size_t>(cand.size(), 1), 0);
+    for (size_t k = 0; k < cand.size(); k++) {
+      map[cand[k]] = (int)k;
+      offs[k] = cand[k] - window;
+    }
+    if ((st = device_alloc((void **)&sp->doff, offs.size() * 4)) ||
+        (st = device_alloc((void **)&sp->dv, dia_bytes)))
+      return failed(st);
+    hipMemcpyAsync(dmap, map.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(sp->doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s);
+    hipMemsetAsync(sp->dv, 0, dia_bytes, s);
+  }
+  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, src, m->colidx, m->values, dmap,
+                                            lcnt, rcnt);
   long long tot[2] = {0, 0};
-  int *cnts[2] = {lcnt, rcnt}, *ptrs[2] = {sp->lrp, sp->rptr};
+  int *cnts[2] = {lcnt, rcnt}, *ptrs[2] = {sp->lrp, rptr};
   for (int k = 0; k < 2; k++) {
     if ((st = scan<int>(cnts[k], ptrs[k], n, false, d_total, ws, s))) return failed(st);
     if (hipMemcpyAsync(&tot[k], d_total, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -256,9 +497,10 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
   }
   sp->nl = tot[0];
   sp->nr = tot[1];
-  if ((st = device_alloc((void **)&sp->lci, (size_t)std::max(sp->nl, 1LL) * 4)) ||
-      (st = device_alloc((void **)&sp->lv, (size_t)std::max(sp->nl, 1LL) * 8)) ||
-      (st = device_alloc((void **)&sp->rpos, (size_t)std::max(sp->nr, 1LL) * 4)) ||
+  const long long nl_store = diag ? 1 : std::max(sp->nl, 1LL);  // DIA entries live in dv
+  if ((st = device_alloc((void **)&sp->lci, (size_t)nl_store * 4)) ||
+      (st = device_alloc((void **)&sp->lv, (size_t)nl_store * 8)) ||
+      (st = device_alloc((void **)&sp->sr, (size_t)std::max(sp->nr, 1LL) * 4)) ||
       (st = device_alloc((void **)&sp->sc, (size_t)std::max(sp->nr, 1LL) * 4)) ||
       (st = device_alloc((void **)&sp->sv, (size_t)std::max(sp->nr, 1LL) * 8)) ||
       // P, followed by the device copy of the slice offsets
@@ -271,19 +513,23 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s) {
   hipMemsetAsync(scount, 0, (size_t)(slices + 1) * 4, s);
   const int rb = bits_of(std::max(n, 1LL));
   k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, rb, src, m->colidx,
-                                           m->values, sp->lrp, sp->rptr, sp->lci, sp->lv, rc, rv, keys,
-                                           pay, scount);
+                                           m->values, dmap, sp->ndiag, sp->lrp, rptr, sp->lci, sp->lv, sp->dv,
+                                           rc, rv, keys, pay, scount);
   if (sp->nr > 1 && (st = radix_sort_pairs(keys, pay, sp->nr, rb + bits_of(slices), s))) return failed(st);
-  if (sp->nr > 0) k_split_place<<<sgrid(sp->nr), kBlock, 0, s>>>(sp->nr, pay, rc, rv, sp->sc, sp->sv, sp->rpos);
+  if (sp->nr > 0)
+    k_split_place<<<sgrid(sp->nr), kBlock, 0, s>>>(sp->nr, keys, rb, pay, rc, rv, sp->sr, sp->sc,
+                                                   sp->sv);
   std::vector<int> hc(slices + 1, 0);
   if (hipMemcpyAsync(hc.data(), scount, (size_t)slices * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: slice counts"));
   sp->soff.assign(slices + 1, 0);
   for (int k = 0; k < slices; k++) sp->soff[k + 1] = sp->soff[k] + hc[k];
-  long long *d_soff = reinterpret_cast<long long *>(sp->P + sp->nr);
+  long long *d_soff = reinterpret_cast<long long *>(sp->P + std::max(sp->nr, 1LL));
   if (hipMemcpyAsync(d_soff, sp->soff.data(), (size_t)(slices + 1) * 8, hipMemcpyHostToDevice, s) != hipSuccess)
     return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: offsets"));
+  k_block_offsets<<<sgrid((long long)slices * (sp->nblk + 1)), kBlock, 0, s>>>(
+      d_soff, slices, sp->nblk, sp->sr, sp->boff);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return failed(fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e)));
   cleanup();
@@ -303,19 +549,24 @@ extern "C" int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int wi
     split_release(m);
     return TBSLA_HIP_SUCCESS;
   }
-  if (layout != TBSLA_HIP_CSR_SPLIT) return fail(TBSLA_HIP_ERR_INVALID, "unknown layout");
+  if (layout != TBSLA_HIP_CSR_SPLIT && layout != TBSLA_HIP_CSR_DIAG)
+    return fail(TBSLA_HIP_ERR_INVALID, "unknown layout");
   if (window < 0 || slices < 1 || slices > 64 || slices % 8)
     return fail(TBSLA_HIP_ERR_INVALID, "window >= 0, slices a multiple of 8 up to 64");
+  if (layout == TBSLA_HIP_CSR_DIAG && window > kMaxDiagWindow)
+    return fail(TBSLA_HIP_ERR_INVALID, "diagonal layout: window at most 1048576");
+  if (layout == TBSLA_HIP_CSR_DIAG && m->ln_col == 0)  // its kernel reads x for every slot
+    return fail(TBSLA_HIP_ERR_INVALID, "diagonal layout: block without columns");
   if (!m->has_arrays || m->ln_row == 0 || (m->format == TBSLA_HIP_ELL && m->max_col == 0))
     return fail(TBSLA_HIP_ERR_INVALID, "empty block");
-  return split_build(m, window, slices, static_cast<hipStream_t>(stream));
+  return split_build(m, window, slices, static_cast<hipStream_t>(stream), layout);
 }
 
 extern "C" int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window,
                                         int *slices, long long *remote_entries) {
   if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null handle");
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
-  if (layout) *layout = sp ? TBSLA_HIP_CSR_SPLIT : TBSLA_HIP_CSR_ROWS;
+  if (layout) *layout = sp ? sp->layout : TBSLA_HIP_CSR_ROWS;
   if (window) *window = sp ? sp->window : 0;
   if (slices) *slices = sp ? sp->slices : 0;
   if (remote_entries) *remote_entries = sp ? sp->nr : 0;

@@ -357,7 +357,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       }
       if (m->split) return split_launch(m, d_r, d_v, d_done, s);
       CsrRowsArgs a{m->ln_row, m->rowptr, m->colidx, m->values, d_v, m->f_col, d_r, d_done,
-                    nullptr, nullptr, nullptr, seg_offs, nseg, seg_done};
+                    nullptr, seg_offs, nseg, seg_done};
       const int L = pick_csr_lanes(m);
       csr_rows_launch<false>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
       TBSLA_HIP_LAUNCH_CHECK();
@@ -400,7 +400,7 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         if (V == 4) L = std::min(std::max(L, 4), 16);
         const int R = V > 1 ? 1 : (L >= 64 ? 1 : (L >= 32 ? 2 : 4));
         CsrRowsArgs a{m->coo_nrows, m->coo_rowptr, m->colidx, m->values, d_v, -col_off,
-                      d_r + ((long long)m->coo_row0 + row_off), nullptr, nullptr, nullptr, nullptr};
+                      d_r + ((long long)m->coo_row0 + row_off), nullptr, nullptr};
         csr_rows_launch<false>(a, V, L, R, 1, s);
         TBSLA_HIP_LAUNCH_CHECK();
         return TBSLA_HIP_SUCCESS;

@@ -1,9 +1,12 @@
-"""The split CSR execution layout (tbsla_hip_set_csr_layout) on the GPU.
+"""The split and diagonal CSR execution layouts (tbsla_hip_set_csr_layout)
+on the GPU.
 
-SpMV through the split layout stays within a relative 1e-12 (max-norm) of
-the oracle for every window / slice count / block position, is bitwise
+SpMV through either layout stays within a relative 1e-12 (max-norm) of the
+oracle for every window / slice count / block position, is bitwise
 repeatable, is rebuilt by normalize_cols, and is dropped when the arrays
-change.
+change.  The diagonal layout keeps explicit zeros and duplicate entries
+out of its diagonals, so Inf and NaN in x reach exactly the rows they reach
+in the row-major sum.
 """
 import numpy as np
 import pytest
@@ -48,6 +51,61 @@ def test_split_spmv_matches_oracle(n, c, q, grid, window, slices):
     assert np.array_equal(g.spmv(x), rows)
 
 
+@pytest.mark.parametrize("n,c,q,grid,window,slices", CASES)
+def test_diag_spmv_matches_oracle(n, c, q, grid, window, slices):
+    window = min(window, 65536)
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, *grid, value_seed=3)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, c, q, 0, *grid, value_seed=3)
+    x = O.unit_vector(7, o.geom["ln_col"])
+    rows = g.spmv(x)
+    g.set_csr_layout(T.CSR_DIAG, window, slices)
+    lay = g.get_csr_layout()
+    assert lay["layout"] == T.CSR_DIAG and lay["slices"] == slices
+    y1, y2 = g.spmv(x), g.spmv(x)
+    assert rel_max(y1, o.spmv(x)) < 1e-12
+    assert np.array_equal(y1, y2)
+    g.set_csr_layout(T.CSR_ROWS)
+    assert np.array_equal(g.spmv(x), rows)
+
+
+def test_diag_remote_count_zeros_duplicates_and_nonfinite_x():
+    """Rows r of a 3000 x 3000 block: (r, r) = 1 + r/7; (r, r+1) = 0.0 for
+    odd r, 2.5 otherwise; a second (r, r) = -0.5 for r % 3 == 0; (r, r+500)
+    for r % 5 == 0.  The diagonals 0 and +1 are stored (+500 holds a fifth
+    of the rows, below the quarter threshold); the explicit zeros, the
+    second diagonal entries and the +500 entries are remote."""
+    n = 3000
+    rp, ci, va = [0], [], []
+    for r in range(n):
+        ent = [(r, 1.0 + r / 7.0)]
+        if r + 1 < n:
+            ent.append((r + 1, 0.0 if r % 2 else 2.5))
+        if r % 3 == 0:
+            ent.append((r, -0.5))
+        if r % 5 == 0 and r + 500 < n:
+            ent.append((r + 500, 0.25))
+        for c, v in ent:
+            ci.append(c)
+            va.append(v)
+        rp.append(len(ci))
+    rp, ci, va = np.array(rp, np.int32), np.array(ci, np.int32), np.array(va)
+    o = O.csr_from_arrays(n, n, rp, ci, va)
+    g = T.Matrix.from_arrays(T.CSR, o.geom, rp, ci, va)
+    g.set_csr_layout(T.CSR_DIAG, 1000, 8)
+    zeros = int(np.sum(va == 0.0))
+    dups = len(range(0, n, 3))
+    far = len([r for r in range(0, n, 5) if r + 500 < n])
+    assert g.get_csr_layout()["remote"] == zeros + dups + far
+    for bad in (np.inf, -np.inf, np.nan):
+        x = O.unit_vector(4, n) + 0.5
+        x[[17, 1200, n - 1]] = bad
+        want, got = o.spmv(x), g.spmv(x)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        assert np.array_equal(np.isinf(got), np.isinf(want))
+        fin = np.isfinite(want)
+        assert rel_max(got[fin], want[fin]) < 1e-12
+
+
 def test_split_remote_count_and_normalize():
     n = 40000
     o = O.fill_cqmat(O.CSR, n, n, 16, 0.1, 0, value_seed=1)
@@ -69,13 +127,14 @@ def test_split_remote_count_and_normalize():
     assert g.get_csr_layout()["layout"] == T.CSR_ROWS
 
 
+@pytest.mark.parametrize("layout", [T.CSR_SPLIT, T.CSR_DIAG])
 @pytest.mark.parametrize("grid", [(0, 0, 1, 1), (1, 0, 2, 2)])
-def test_split_ell_matches_oracle(grid):
+def test_split_ell_matches_oracle(grid, layout):
     n = 20000
     o = O.fill_cqmat(O.ELL, n, n, 16, 0.1, 0, *grid, value_seed=5)
     g = T.Matrix(T.ELL).fill_cqmat(n, n, 16, 0.1, 0, *grid, value_seed=5)
     x = O.unit_vector(3, o.geom["ln_col"])
-    g.set_csr_layout(T.CSR_SPLIT, 64, 8)
+    g.set_csr_layout(layout, 64, 8)
     y = g.spmv(x)
     assert rel_max(y, o.spmv(x)) < 1e-12
     assert np.array_equal(y, g.spmv(x))
@@ -86,6 +145,10 @@ def test_split_bad_arguments():
     for w, s in [(-1, 8), (64, 0), (64, 12), (64, 128)]:
         with pytest.raises(T.TbslaHipError):
             g.set_csr_layout(T.CSR_SPLIT, w, s)
+    with pytest.raises(T.TbslaHipError):
+        g.set_csr_layout(T.CSR_DIAG, (1 << 20) + 1, 8)
+    with pytest.raises(T.TbslaHipError):
+        g.set_csr_layout(3, 64, 8)
     e = T.Matrix(T.COO).fill_cqmat(100, 100, 4, 0.1, 0)
     with pytest.raises(T.TbslaHipError):
         e.set_csr_layout(T.CSR_SPLIT, 64, 8)
