@@ -132,12 +132,15 @@ def cg_section(args):
     mixed = dict(precision=T.CG_MIXED, replace_every=50)
     side = 128
     A = T.Matrix(T.CSR).fill_poisson3d(side, side, side)
+    A.tune_layout()  # the autotuned layout (diagonal) is what the CG engine reads
     rhs = T.uniform_vector(side ** 3, 3, -1.0, 1.0)
     out["poisson128"] = {"n": side ** 3, "nnz": A.geometry().stored,
+                         "layout": A.get_csr_layout()["layout"],
                          "plain_fp64": timed(A, rhs, **plain)[0],
                          "mixed": timed(A, rhs, **mixed)[0]}
     del A
     S = T.Matrix(T.CSR).fill_sym_cqmat(1 << 21, 16, 0.05, 0, value_seed=0)
+    S.tune_layout()
     rhs = T.uniform_vector(1 << 21, 4, -1.0, 1.0)
     out["sym_cqmat_2p21"] = {"n": 1 << 21, "nnz": S.geometry().stored,
                              "plain_fp64": timed(S, rhs, **plain)[0],
@@ -148,6 +151,7 @@ def cg_section(args):
         import scipy.sparse as sp
         import scipy.sparse.linalg as spla
         A = T.Matrix(T.CSR).fill_poisson3d(32, 32, 32)
+        A.tune_layout()
         rhs = T.uniform_vector(32 ** 3, 5, -1.0, 1.0)
         rp, ci, va = A.arrays()
         exact = spla.spsolve(sp.csr_matrix((va, ci, rp)).tocsc(), rhs.to_numpy(count=32 ** 3))

@@ -416,6 +416,16 @@ class Matrix:
         check(lib().tbsla_hip_get_csr_kernel(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"lanes": a.value, "rows": b.value, "vec": c.value, "nontemporal": d.value}
 
+    def tune_layout(self, iters: int = 5) -> float:
+        """The autotuner on a unit-value vector of this block's width: picks
+        the SpMV kernel and execution layout (the CG engine then reads the
+        diagonal layout when it holds every entry); returns us per SpMV."""
+        g = self.geometry()
+        dv = DeviceBuffer(max(g.ln_col, 1) * 8)
+        dr = DeviceBuffer(max(g.n_row if self.fmt == COO else g.ln_row, 1) * 8)
+        check(lib().tbsla_hip_fill_unit_vector(dv.ptr, g.ln_col, 1, None))
+        return self.autotune(dr, dv, iters)
+
     def autotune(self, dr: DeviceBuffer, dv: DeviceBuffer, iters: int = 10,
                  stream=None) -> float:
         us = C.c_float()

@@ -108,6 +108,13 @@ struct CgArgs {
   int stall_window;  // > 0: breakdown when |r| has not dropped below |r0| by then
   int a_nnz;         // stored entries of A
   int g_nnz;         // stored entries of G (and of G^T)
+  // A in the diagonal layout (every entry on a stored diagonal): offsets,
+  // tiled values (dia_at), their FP32 copy, count, A's columns
+  const int *doff;
+  const double *dv64;
+  const float *dv32;
+  int ndiag;
+  int ncol;
 };
 
 // Sums of a system's workgroup partials are taken in one fixed order:
@@ -420,6 +427,74 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
         pap += own * (double)acc;
       }
     }
+  }
+  pap = wg_sum(pap, lds);
+  if (threadIdx.x == 0) a.ppap[b] = pap;
+}
+
+// K1 on A in the diagonal layout: the same steps as k_cg_spmv_p (the
+// partials of the previous iteration, the convergence test, beta), then per
+// row p' = z + beta p and (A p')_row = sum over the stored diagonals
+// (ascending offset, zero slots skipped) of A's slot times p' at the slot's
+// column, one row per thread, eight diagonals' loads in flight.
+template <bool MIX, bool ZBUF>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(CgArgs a, int par_g, int check,
+                                                        int check_only) {
+  TBSLA_NO_FMA
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int NP = ZBUF ? 3 : 2;
+  constexpr int U = 8;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ SegShared sh;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  const int par = (par_g ^ S->shift) & 1;
+  const int b0 = S->blk0, b1 = S->blk1;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const double *zv = ZBUF ? a.z : a.r[par];
+  const double *po = a.p[par];
+  const V *dv = MIX ? (const V *)a.dv32 : (const V *)a.dv64;
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], NP, b0, b1, sl);
+  double t[3];
+  if (seg_finish(flag, sl, NP, b0, b1, sh, t) != kRunning) return;
+  const double rz_new = t[0], rz_old = t[1];
+  const double rr = ZBUF ? t[2] : rz_new;
+  if (check) {
+    const int fin = k1_test(a, S, b, b0, par, rz_new, rr, check_only);
+    if (fin != kRunning || check_only) return;
+  }
+  const double beta = rz_new / fmax(rz_old, S->near_zero);
+  double *pn = a.p[par ^ 1];
+  double pap = 0.0;
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
+    V acc = 0;
+    for (int k = 0; k < a.ndiag; k += U) {
+      V v[U];
+      double zs[U], ps[U];
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        const int kk = min(k + i, a.ndiag - 1);
+        const int c = min(max(row + a.doff[kk], 0), a.ncol - 1);
+        v[i] = dv[dia_at(row, kk, a.ndiag)];
+        zs[i] = zv[c];
+        ps[i] = po[c];
+      }
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        if (k + i < a.ndiag && v[i] != (V)0) {
+          const double pc = zs[i] + beta * ps[i];
+          if (MIX) acc = fmaf((float)v[i], (float)pc, (float)acc);
+          else acc = fma((double)v[i], pc, (double)acc);
+        }
+      }
+    }
+    const double own = zv[row] + beta * po[row];
+    pn[row] = own;
+    a.ap[row] = (double)acc;
+    pap += own * (double)acc;
   }
   pap = wg_sum(pap, lds);
   if (threadIdx.x == 0) a.ppap[b] = pap;
@@ -1149,7 +1224,8 @@ struct Buf {
 struct GraphKey {
   CgArgs args;
   const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
-  int L, Lk1, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused, product;
+  int L, Lk1, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused, product,
+      dia;
   bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
 };
 
@@ -1178,6 +1254,11 @@ struct CgCache {
   float *a_v32 = nullptr;
   size_t a_v32_len = 0;
   unsigned long long a_v32_uid = 0, a_v32_ver = 0;
+  // FP32 copy of A's diagonal layout, made from the layout build dv32_build
+  // (ids are unique in the process; a layout is rebuilt when values change)
+  float *dv32 = nullptr;
+  size_t dv32_len = 0;
+  unsigned long long dv32_build = 0;
   // factor G: FP32 copies of G and G^T
   tbsla_hip_matrix_s *factor = nullptr;
   int *t_rp = nullptr, *t_ci = nullptr;
@@ -1252,6 +1333,7 @@ struct CgCache {
     for (auto &g : graphs) g.destroy();
     drop_factor();
     hipFree(a_v32);
+    hipFree(dv32);
   }
 };
 
@@ -1313,6 +1395,7 @@ struct Plan {
   int Lk1;  // lanes per row of K1 (the product form reads A G's longer rows)
   bool fused;    // M = G G^T: z = G (G^T r) in K2
   bool product;  // M = G G^T in the product form (K1 with A G rows, no K3)
+  bool dia;      // K1 reads A's diagonal layout
   long long n;
   int nblocks;
   hipStream_t s;
@@ -1325,6 +1408,17 @@ void launch_k1(const CgArgs &a, const Plan &p, int par, int check, int check_onl
 
 template <int L>
 void launch_k1_l(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
+  if (p.dia) {
+    const bool zbuf = p.pre != 0;
+    if (p.mixed) {
+      if (zbuf) k_cg_spmv_dia<true, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+      else k_cg_spmv_dia<true, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+    } else {
+      if (zbuf) k_cg_spmv_dia<false, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+      else k_cg_spmv_dia<false, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+    }
+    return;
+  }
   if (p.product) {
     if (p.mixed) k_cg_spmv_ag<L, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
     else k_cg_spmv_ag<L, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
@@ -1701,6 +1795,37 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   }
   a.verify = (mixed || o.replace_every > 0) ? 1 : 0;
   a.stall_window = o.stall_window;
+  // A in the diagonal layout with no remote entries: K1 reads its diagonals
+  const int *d_off = nullptr;
+  const double *d_v = nullptr;
+  int ndiag = 0;
+  long long dv_len = 0;
+  unsigned long long build = 0;
+  const bool dia =
+      !product && split_diag_only(m, &d_off, &d_v, &ndiag, &dv_len, &build) && m->ln_col > 0;
+  if (dia) {
+    a.doff = d_off;
+    a.dv64 = d_v;
+    a.ndiag = ndiag;
+    a.ncol = (int)m->ln_col;
+    if (mixed) {
+      if (c.dv32_len < (size_t)dv_len) {
+        TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+        hipFree(c.dv32);
+        c.dv32 = nullptr;
+        c.dv32_len = 0;
+        TBSLA_TRY(device_alloc((void **)&c.dv32, (size_t)dv_len * 4));
+        c.dv32_len = (size_t)dv_len;
+        c.dv32_build = 0;
+      }
+      if (c.dv32_build != build) {
+        k_to_f32<<<sgrid(dv_len), kBlock, 0, s>>>(d_v, dv_len, c.dv32);
+        TBSLA_HIP_LAUNCH_CHECK();
+        c.dv32_build = build;
+      }
+      a.dv32 = c.dv32;
+    }
+  }
   // the prefetches of K1 and K2 read entry 0 of A's and G^T's arrays even
   // when they hold none: a one-entry stand-in then (never used otherwise)
   if (!c.dummy_i.p) {
@@ -1743,6 +1868,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   // us), so the automatic choice is the separate form
   p.fused = pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_FUSED;
   p.product = product;
+  p.dia = dia;
   p.Lk1 = L;
   if (product && o.lanes_per_row <= 0) {
     // the same rule on A G's mean row length (at least L: the workgroup
@@ -1822,6 +1948,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.nblocks = nblocks;
   key.fused = p.fused ? 1 : 0;
   key.product = p.product ? 1 : 0;
+  key.dia = p.dia ? 1 : 0;
   key.spmv_lanes = pick_csr_lanes(m);
   key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
   key.spmv_vec = pick_csr_vec(m);
@@ -1833,7 +1960,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   if (B > 0) {
     auto shape_of = [](const GraphKey &k) {
       return std::make_tuple(k.L, k.Lk1, k.pre, k.mixed, k.replace, k.batch, k.nblocks, k.fused,
-                             k.product, k.args.verify);
+                             k.product, k.args.verify, k.dia);
     };
     size_t pos = c.graphs.size();
     for (size_t q = 0; q < c.graphs.size() && pos == c.graphs.size(); q++)

@@ -40,6 +40,14 @@ int fail(int status, const std::string &msg);
       return st__;                 \
   } while (0)
 
+// Diagonal layout of a CSR block: values stored in tiles of kWave rows; row
+// r's entry on stored diagonal k sits at (r / kWave) kWave ndiag + k kWave +
+// r % kWave, so one wavefront's loads of a diagonal are one 512-byte piece
+// and its whole slab of diagonals is contiguous.
+__host__ __device__ inline long long dia_at(long long r, int k, int ndiag) {
+  return (r / kWave) * (long long)kWave * ndiag + (long long)k * kWave + (r % kWave);
+}
+
 inline unsigned grid_for(long long n, int per_block = kBlock) {
   long long g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;

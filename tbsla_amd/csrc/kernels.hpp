@@ -54,6 +54,12 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
 int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s,
                 int layout = TBSLA_HIP_CSR_SPLIT);
 int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *layout);
+// The diagonal layout's arrays when it holds every entry (no remote part):
+// offsets, tiled values (dia_at), count, values' length, and the build's
+// id (unique in the process: a copy made from one build is stale for any
+// other, even one whose arrays reuse the same addresses); else 0.
+int split_diag_only(const tbsla_hip_matrix_s *m, const int **doff, const double **dv, int *ndiag,
+                    long long *dv_len, unsigned long long *build_id);
 
 // solvers.hip: after the values of m changed in place: rebuild the split
 // layout (it holds a copy of the values), mark copies of the old values

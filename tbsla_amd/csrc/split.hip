@@ -25,6 +25,7 @@
 //                                      every SpMV
 //   doff[ndiag], dv[tiles]             diagonal layout: stored diagonals
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -36,16 +37,9 @@ namespace tbsla_hip {
 constexpr int kCombineRows = 1024;  // rows per combine workgroup (LDS: 8 KB)
 constexpr int kMaxDiagWindow = 1 << 20;  // diagonal layout: offsets searched
 
-// Diagonal values are stored in tiles of kWave rows: row r's entry on
-// diagonal k sits at (r / kWave) kWave ndiag + k kWave + r % kWave, so one
-// wavefront's loads of a diagonal are one 512-byte piece and its whole slab
-// of diagonals is contiguous.
-__host__ __device__ inline long long dia_at(long long r, int k, int ndiag) {
-  return (r / kWave) * (long long)kWave * ndiag + (long long)k * kWave + (r % kWave);
-}
-
 struct SplitCsr {
   int layout = 0, window = 0, slices = 0;
+  unsigned long long build_id = 0;  // unique per build in the process
   long long nl = 0, nr = 0, nblk = 0;
   int ndiag = 0;  // diagonal layout: stored diagonals
   int *lrp = nullptr, *lci = nullptr, *sr = nullptr, *sc = nullptr, *boff = nullptr;
@@ -356,6 +350,18 @@ int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *lay
   return 1;
 }
 
+int split_diag_only(const tbsla_hip_matrix_s *m, const int **doff, const double **dv, int *ndiag,
+                    long long *dv_len, unsigned long long *build_id) {
+  const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  if (!sp || sp->layout != TBSLA_HIP_CSR_DIAG || sp->nr != 0 || sp->ndiag == 0) return 0;
+  *build_id = sp->build_id;
+  *doff = sp->doff;
+  *dv = sp->dv;
+  *ndiag = sp->ndiag;
+  *dv_len = (long long)sp->ndiag * ((m->ln_row + kWave - 1) / kWave) * kWave;
+  return 1;
+}
+
 void split_release(tbsla_hip_matrix_s *m) {
   if (m->split) {
     delete static_cast<SplitCsr *>(m->split);
@@ -403,7 +409,9 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
   split_release(m);
   const long long n = m->ln_row, nc = std::max(m->ln_col, 1);
   const bool diag = layout == TBSLA_HIP_CSR_DIAG;
+  static std::atomic<unsigned long long> builds{0};
   auto *sp = new SplitCsr();
+  sp->build_id = ++builds;
   sp->layout = layout;
   sp->window = window;
   sp->slices = slices;

@@ -138,6 +138,7 @@ def evaluate(net, side: int, seeds, methods=METHODS, tol: float = 1e-8, warm: bo
     factors = (T.Matrix(T.CSR), T.Matrix(T.CSR))  # one pair for all problems of this size
     for k, sd in enumerate(seeds):
         A, rhs = poisson_problem(side, sd)
+        A.tune_layout()  # the 7-point matrix: the CG engine reads its diagonals
         s = Solver(A, rhs, net, factors)
         for m in methods:
             if net is None and m.startswith("learned"):
