@@ -284,9 +284,11 @@ def formats_section(iters: int = 20):
     tuned = timed3()
     A.set_csr_layout(T.CSR_ROWS)
     out["poisson128_spmv"] = {"stored": g.stored, "bytes_csr": nbytes,
+                              "bytes_diagonal_layout": 7 * 8 * n3 + 16 * n3,
                               "autotuned": {"layout": lay["layout"], **tuned},
                               "row_layout": timed3(),
-                              "note": "GB/s over the CSR algorithmic bytes; layout 2 = diagonal"}
+                              "note": "GB/s over the CSR algorithmic bytes; layout 2 = diagonal, "
+                                      "which moves bytes_diagonal_layout"}
     del A, x3, y3
     # a plain device copy for reference (read + write)
     B = 1 << 30
