@@ -13,7 +13,8 @@ against 8 TB/s) and the reference's own OpenMP CPU SpMV (oracle/_ref, the
 reference library compiled from its sources) timed on this host.
 
 The same line carries "cg": time to relative residual 1e-8 (vectors in HBM,
-host clock around each solve, second solve of each kind):
+host clock around each solve, second solve of each kind; every matrix is
+autotuned first, so a stencil's CG reads its diagonal layout):
   * 7-point Poisson 128^3 (k = 1) and the symmetrised cqmat (n = 2^21, c = 16,
     q = 0.05): plain FP64 CG against mixed precision (FP32 matrix, FP64 dots,
     true residual every 50), with the error of both against a sparse direct
