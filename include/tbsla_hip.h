@@ -464,8 +464,10 @@ int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int
  * kernel (tbsla_hip_set_csr_kernel: lanes x rows x vec x nontemporal) on the
  * row layout and on the split layout (window 64, 8 and 16 slices), and the
  * diagonal layout (window 65536, 8 and 16 slices), on this block with d_v /
- * d_r (`iters` launches each) and keeps the fastest; a split or diagonal
- * layout is kept only when it is at least 3 % faster than the row layout.
+ * d_r (`iters` launches each; the three fastest row kernels are timed again
+ * with 3 x `iters` launches before the choice) and keeps the fastest; a
+ * split or diagonal layout is kept only when it is at least 3 % faster than
+ * the row layout.
  * ELL compares its column-major kernel with the split and diagonal layouts
  * by the same rule; other formats have a single schedule.  *best_us
  * receives its time per SpMV. */

@@ -539,6 +539,7 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
   // a derived layout (a second copy of the entries) is kept only when it
   // beats the row layout's best time by kDerivedGain
   float rows_ms = 0;
+  std::vector<Best> rows_trials;  // every row-layout kernel's short trial
   auto derived_wins = [&](float t) { return t < best.ms && t < rows_ms * kDerivedGain; };
   auto sweep = [&](int layout, int window, int slices) -> int {
     for (int V : {1, 2, 4})
@@ -555,11 +556,32 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
             const bool win = layout == TBSLA_HIP_CSR_ROWS ? (best.layout < 0 || ms < best.ms)
                                                           : derived_wins(ms);
             if (win) best = Best{layout, window, slices, L, R, V, NT, ms};
+            if (layout == TBSLA_HIP_CSR_ROWS) rows_trials.push_back(Best{layout, 0, 0, L, R, V, NT, ms});
           }
     return TBSLA_HIP_SUCCESS;
   };
   split_release(m);
   TBSLA_TRY(sweep(TBSLA_HIP_CSR_ROWS, 0, 0));
+  // the three fastest row kernels again with three times the launches (the
+  // short trials differ by a few percent from run to run)
+  {
+    std::stable_sort(rows_trials.begin(), rows_trials.end(),
+                     [](const Best &x, const Best &y) { return x.ms < y.ms; });
+    if (rows_trials.size() > 3) rows_trials.resize(3);
+    for (auto &t : rows_trials) {
+      m->lanes_per_row = t.L;
+      m->rows_per_group = t.R;
+      m->csr_vec = t.V;
+      m->csr_nt = t.NT;
+      TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 3 * iters, &ms, stream));
+      t.ms = ms / 3.0f;
+    }
+    if (!rows_trials.empty()) {
+      best = rows_trials[0];
+      for (const auto &t : rows_trials)
+        if (t.ms < best.ms) best = t;
+    }
+  }
   rows_ms = best.ms;
   constexpr int kWindow = 64;
   for (int slices : {8, 16}) {
