@@ -158,6 +158,31 @@ int tbsla_hip_export(tbsla_hip_matrix_t m, int *h_a, int *h_b, double *h_v, void
  * n_row .. NC, nnz, stored and, for ELL, max_col).  Array roles as export. */
 int tbsla_hip_import(tbsla_hip_matrix_t m, int format, const tbsla_hip_geometry *geometry,
                      const int *h_a, const int *h_b, const double *h_v, void *stream);
+/* ---- binary images ------------------------------------------------------ */
+/* Matrix::write (Matrix.hpp:17; MatrixCSR.cpp:236-261, MatrixELL.cpp:208-228,
+ * MatrixCOO.cpp:144-169, MatrixSCOO.cpp:166-191, MatrixDENSE.cpp:88-105):
+ * the block's binary image, byte for byte the reference's: header
+ * n_row n_col ln_row ln_col f_row f_col (int32), nnz (int64), pr pc NR NC
+ * (int32), ELL max_col (int32), then each array as a size_t count and its
+ * elements in the reference layout (CSR values[nnz] rowptr[ln_row+1]
+ * colidx[nnz]; COO/SCOO values row col, nnz each; ELL values columns,
+ * ln_row*max_col each, row-major; DENSE values ln_row*ln_col).  CSR entries
+ * past rowptr[ln_row] that the nnz field still counts (fill_cqmat's count
+ * pass) are written as zeros (the reference leaves them uninitialised).
+ * h_buf NULL: *written receives the image's size only. */
+int tbsla_hip_write(tbsla_hip_matrix_t m, void *h_buf, size_t capacity, size_t *written,
+                    void *stream);
+/* Matrix::read (Matrix.hpp:18; MatrixCSR.cpp:263-295 and its twins): the
+ * block from an image in the handle's format.  A truncated image or arrays
+ * of inconsistent lengths give TBSLA_HIP_ERR_INVALID. */
+int tbsla_hip_read(tbsla_hip_matrix_t m, const void *h_buf, size_t bytes, void *stream);
+/* The file forms of the reference C interface's C_CPP_MatrixCSR_write(m,
+ * file) and C_CPP_MatrixCSR_read(m, file, seek)
+ * (cpp/interface/c/include/tbsla/cpp/MatrixCSR.h:18-19): the image written
+ * to `path`, or read from `path` starting `seek` bytes in. */
+int tbsla_hip_write_file(tbsla_hip_matrix_t m, const char *path, void *stream);
+int tbsla_hip_read_file(tbsla_hip_matrix_t m, const char *path, long long seek, void *stream);
+
 /* Conversion constructors <FMT>(const MatrixCOO&): MatrixCSR.cpp:49-96,
  * MatrixELL.cpp:25-79, MatrixSCOO.cpp:75-101, MatrixDENSE.cpp:15-37.  The
  * (row, col) sort is stable (the reference's std::sort leaves the order of
@@ -169,9 +194,16 @@ int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_matrix_t coo
  * a = rowptr, b = colidx; ELL b = columns (column-major); COO/SCOO a = row,
  * b = col; v = values.  For zero-copy interoperation: read them freely;
  * writes through them bypass the copies the library derives from the
- * arrays (the split CSR layout, the mixed CG's FP32 values, the COO row
- * pointer), so re-import the block after changing them. */
+ * arrays (the split and diagonal CSR layouts, the mixed CG's FP32 values,
+ * the COO row pointer): call tbsla_hip_values_changed after them. */
 int tbsla_hip_device_arrays(tbsla_hip_matrix_t m, void **d_a, void **d_b, void **d_v);
+/* After writes through tbsla_hip_device_arrays (values or indices): rebuilds
+ * the split/diagonal execution layout from the arrays, drops the COO row
+ * pointer (remade, and the row order checked, at the next SpMV) and marks
+ * every FP32 copy of the old values stale.  Ordered after the earlier work
+ * on `stream`; synchronises it.  No reference counterpart: the reference's
+ * arrays are host members with no derived copies. */
+int tbsla_hip_values_changed(tbsla_hip_matrix_t m, void *stream);
 
 /* ---- SpMV and composites ------------------------------------------------ */
 /* Matrix::Ax (Matrix.hpp:15): CSR and ELL overwrite r (MatrixCSR.cpp:209-233,
@@ -320,22 +352,18 @@ typedef struct {
   int lanes_per_row;     /* lanes per matrix row in the iteration's SpMV (power of two
                             <= 64); 0 picks from the mean row length */
   int stall_window;      /* > 0: breakdown when |r| >= |r0| after that many iterations */
-  int factor_form;       /* _FACTORED: TBSLA_HIP_FACTOR_AUTO picks (the separate form); _SEPARATE applies G^T
-                            in the update kernel and G in a kernel of its own; _FUSED
-                            computes z = G (G^T r) in the update kernel (one kernel
-                            fewer per iteration, G^T rows gathered once per G entry);
-                            the iterates are bitwise the same either way; _PRODUCT
-                            builds A G once per call (rows of A with at most 64
-                            products, else the separate form) and computes
-                            Ap' = (A G) t + beta Ap with r.z = t.t, so no kernel
-                            applies G on its own (a different rounding; Ap is
-                            recomputed in FP64 at every residual replacement) */
+  int factor_form;       /* _FACTORED: how M = G G^T is applied.  TBSLA_HIP_FACTOR_AUTO: from G's
+                            diagonals when its entries lie on at most 8 diagonals of offset
+                            <= 0 (a stencil's lower factor: no column indices read), else
+                            from the CSR rows of G and G^T; _ROWS: always the CSR rows;
+                            _DIAGONALS: the diagonals or TBSLA_HIP_ERR_UNSUPPORTED.  Both
+                            forms sum G^T's and G's rows in ascending column, so they give
+                            bitwise the same iterates */
 } tbsla_hip_cg_options;
 enum {
   TBSLA_HIP_FACTOR_AUTO = 0,
-  TBSLA_HIP_FACTOR_SEPARATE = 1,
-  TBSLA_HIP_FACTOR_FUSED = 2,
-  TBSLA_HIP_FACTOR_PRODUCT = 3
+  TBSLA_HIP_FACTOR_ROWS = 1,
+  TBSLA_HIP_FACTOR_DIAGONALS = 2
 };
 
 /* tbsla::mpi::Matrix::CG (src/mpi/matrix/CG.cpp:32-73) with its Ax call

@@ -43,9 +43,9 @@ class CGResult(C.Structure):
 CG_FP64, CG_MIXED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_FACTORED = 0, 1, 2
 CSR_ROWS, CSR_SPLIT, CSR_DIAG = 0, 1, 2
-FACTOR_AUTO, FACTOR_SEPARATE, FACTOR_FUSED, FACTOR_PRODUCT = 0, 1, 2, 3
+FACTOR_AUTO, FACTOR_ROWS, FACTOR_DIAGONALS = 0, 1, 2
 CG_CONVERGED, CG_MAX_ITERATIONS, CG_BREAKDOWN, CG_INACTIVE = 0, 1, 2, 3
-ERR_BREAKDOWN = 6
+ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_UNSUPPORTED, ERR_RANGE, ERR_BREAKDOWN = 1, 2, 3, 4, 5, 6
 
 
 class Geometry(C.Structure):
@@ -88,6 +88,11 @@ SIGNATURES = [
     ("tbsla_hip_convert", C.c_int, [_vp, C.c_int, _vp, _vp]),
     ("tbsla_hip_device_arrays", C.c_int, [_vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]),
+    ("tbsla_hip_values_changed", C.c_int, [_vp, _vp]),
+    ("tbsla_hip_write", C.c_int, [_vp, _vp, C.c_size_t, C.POINTER(C.c_size_t), _vp]),
+    ("tbsla_hip_read", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("tbsla_hip_write_file", C.c_int, [_vp, C.c_char_p, _vp]),
+    ("tbsla_hip_read_file", C.c_int, [_vp, C.c_char_p, C.c_longlong, _vp]),
     ("tbsla_hip_Ax", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_a_axpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
@@ -399,6 +404,25 @@ class Matrix:
         a, b, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
         check(lib().tbsla_hip_device_arrays(self.h, C.byref(a), C.byref(b), C.byref(v)))
         return a.value, b.value, v.value
+
+    def write(self) -> bytes:
+        """Matrix::write: the block's binary image (the reference's bytes)."""
+        n = C.c_size_t()
+        check(lib().tbsla_hip_write(self.h, None, 0, C.byref(n), None))
+        buf = C.create_string_buffer(max(n.value, 1))
+        check(lib().tbsla_hip_write(self.h, buf, n.value, C.byref(n), None))
+        return buf.raw[: n.value]
+
+    @classmethod
+    def read(cls, fmt: int, image: bytes) -> "Matrix":
+        """Matrix::read: a block of format fmt from a binary image."""
+        m = cls(fmt)
+        check(lib().tbsla_hip_read(m.h, image, len(image), None))
+        return m
+
+    def values_changed(self, stream=None) -> None:
+        """tbsla_hip_values_changed: after writes through device_arrays."""
+        check(lib().tbsla_hip_values_changed(self.h, stream))
 
     def set_csr_schedule(self, lanes: int, rows: int = 0) -> None:
         check(lib().tbsla_hip_set_csr_schedule(self.h, lanes, rows))

@@ -84,18 +84,20 @@ struct CgArgs {
   const double *t_v64;
   const float *t_v32;
   const double *dinv;
-  // product form of M = G G^T: rows of A G (local columns)
-  const int *ag_rp;
-  const int *ag_ci;
-  const double *ag_v64;
-  const float *ag_v32;
-  int ag_nnz;
+  // M = G G^T with G held on A's stored diagonals of offset <= 0 (a stencil
+  // factor): offsets (ascending), tiled values (dia_at) and their FP32 copy;
+  // G^T is read from the same slots (G^T_{i,i-o} = G_{i-o,i})
+  const int *gdoff;
+  const double *gd64;
+  const float *gd32;
+  int gndiag;
   const double *b;
   double *x;
   double *r[2];
   double *p[2];
   double *z;
   double *ap;
+  double *ax;  // A x in FP64 (true residuals)
   double *t64;
   float *t32;
   double *ppap;
@@ -116,6 +118,14 @@ struct CgArgs {
   int ndiag;
   int ncol;
 };
+
+// Every kernel reads its arguments from one CgArgs block in device memory
+// that the solve's cache owns for its whole lifetime: a captured graph then
+// names only that block (and its launch shape), never a matrix or work
+// buffer, so one instantiated graph serves every later solve of the same
+// shape and no graph can refer to memory that has been freed since.
+#define CG_ARGS const CgArgs *__restrict__ pa_
+#define CG_BIND const CgArgs &a = *pa_
 
 // Sums of a system's workgroup partials are taken in one fixed order:
 // thread k adds partials k, k+256, ... in order; waves reduce by xor
@@ -290,9 +300,10 @@ __device__ inline int k1_test(const CgArgs &a, CgSeg *S, int b, int b0, int par,
 // (the count it++ of a passed test is made by K2, which no K1 workgroup
 // races with).  check_only: the final test after the last iteration.
 template <int L, bool MIX, bool ZBUF>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int check,
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CG_ARGS, int par_g, int check,
                                                       int check_only) {
   TBSLA_NO_FMA
+  CG_BIND;
   typedef typename std::conditional<MIX, float, double>::type V;
   constexpr int P = L == 1 ? 8 : 4;
   constexpr int NP = ZBUF ? 3 : 2;
@@ -438,9 +449,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CgArgs a, int par_g, int c
 // (ascending offset, zero slots skipped) of A's slot times p' at the slot's
 // column, one row per thread, eight diagonals' loads in flight.
 template <bool MIX, bool ZBUF>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(CgArgs a, int par_g, int check,
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(CG_ARGS, int par_g, int check,
                                                         int check_only) {
   TBSLA_NO_FMA
+  CG_BIND;
   typedef typename std::conditional<MIX, float, double>::type V;
   constexpr int NP = ZBUF ? 3 : 2;
   constexpr int U = 8;
@@ -500,109 +512,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(CgArgs a, int par_g, int
   if (threadIdx.x == 0) a.ppap[b] = pap;
 }
 
-// K1 of the product form of M = G G^T: z = G t is never stored.  For its
-// rows the kernel forms z_i = (G t)_i (in K3's order), p'_i = z_i + beta p_i
-// and, by the recurrence A p' = A G t + beta A p, Ap'_i = (A G t)_i +
-// beta Ap_i from the rows of the product matrix A G -- one kernel fewer per
-// iteration, no z gathers.  The recurrence is restarted from A p in FP64 at
-// every residual replacement, and beta = 0 (start, restart) drops it.
-template <int L, bool MIX>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_ag(CgArgs a, int par_g, int check,
-                                                       int check_only) {
-  TBSLA_NO_FMA
-  typedef typename std::conditional<MIX, float, double>::type V;
-  constexpr int P = L == 1 ? 8 : 4;
-  __shared__ double lds[kBlock / kWave];
-  __shared__ SegShared sh;
-  const int b = blockIdx.x;
-  CgSeg *S = a.seg + a.block_seg[b];
-  const int par = (par_g ^ S->shift) & 1;
-  const int b0 = S->blk0, b1 = S->blk1;
-  const int r0 = S->row0 + (b - b0) * S->rpb;
-  const int r1 = min(S->row1, r0 + S->rpb);
-  const int lane = threadIdx.x & (L - 1);
-  const V *tv = MIX ? (const V *)a.t32 : (const V *)a.t64;
-  const V *agv = MIX ? (const V *)a.ag_v32 : (const V *)a.ag_v64;
-  const V *gv = MIX ? (const V *)a.g_v32 : (const V *)a.g_v64;
-  const double *po = a.p[par];
-  const int row_first = r0 + (int)threadIdx.x / L;
-  const int rowc = min(row_first, r1 - 1);
-  // as K1: the first row's entries, the partials, then the gathers
-  RowPrefetch<P, V> f;
-  f.beg = a.ag_rp[rowc];
-  f.end = a.ag_rp[rowc + 1];
-  const double own_p = po[rowc], own_ap = a.ap[rowc];
-#pragma unroll
-  for (int k = 0; k < P; k++) {
-    const int j = min(f.beg + lane + k * L, a.ag_nnz - 1);
-    f.col[k] = a.ag_ci[j];
-    f.v[k] = agv[j];
-  }
-  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
-  SegLoad sl;
-  seg_issue(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], 3, b0, b1, sl);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int k = 0; k < P; k++) f.s[k] = (double)tv[f.col[k]];
-  __builtin_amdgcn_sched_barrier(0);  // keep the loads above issued in this order
-  double t[3];
-  if (seg_finish(flag, sl, 3, b0, b1, sh, t) != kRunning) return;
-  const double rz_new = t[0], rz_old = t[1], rr = t[2];
-  if (check) {
-    const int fin = k1_test(a, S, b, b0, par, rz_new, rr, check_only);
-    if (fin != kRunning || check_only) return;
-  }
-  const double beta = rz_new / fmax(rz_old, S->near_zero);
-  double *pn = a.p[par ^ 1];
-  double pap = 0.0;
-  for (int row = row_first; row < r1; row += kBlock / L) {
-    const bool first = row == row_first;
-    int j, end;
-    V acc = 0;
-    if (first) {
-#pragma unroll
-      for (int k = 0; k < P; k++)
-        if (f.beg + lane + k * L < f.end) {
-          if (MIX) acc = fmaf((float)f.v[k], (float)f.s[k], (float)acc);
-          else acc = fma((double)f.v[k], f.s[k], (double)acc);
-        }
-      j = f.beg + lane + P * L;
-      end = f.end;
-    } else {
-      j = a.ag_rp[row] + lane;
-      end = a.ag_rp[row + 1];
-    }
-    for (; j < end; j += L) {
-      if (MIX) acc = fmaf((float)agv[j], (float)tv[a.ag_ci[j]], (float)acc);
-      else acc = fma((double)agv[j], (double)tv[a.ag_ci[j]], (double)acc);
-    }
-    acc = lane_sum<L>(acc);
-    if (lane == 0) {
-      V zacc = 0;
-      for (int jg = a.g_rp[row]; jg < a.g_rp[row + 1]; jg++) {
-        if (MIX) zacc = fmaf((float)gv[jg], (float)tv[a.g_ci[jg] - a.f_col], (float)zacc);
-        else zacc = fma((double)gv[jg], (double)tv[a.g_ci[jg] - a.f_col], (double)zacc);
-      }
-      const double own = (double)zacc + beta * (first ? own_p : po[row]);
-      const double apo = first ? own_ap : a.ap[row];
-      const double apn = (double)acc + (beta != 0.0 ? beta * apo : 0.0);
-      pn[row] = own;
-      a.ap[row] = apn;
-      pap += own * apn;
-    }
-  }
-  pap = wg_sum(pap, lds);
-  if (threadIdx.x == 0) a.ppap[b] = pap;
-}
-
-// K2.  PRE: 0 none, 1 Jacobi, 2 M = G G^T (t = G^T r' here, z = G t in K3),
-// 4 M = G G^T in the product form (t = G^T r' and t.t partials: r.z = t.t),
-// 3 M = G G^T with z = G (G^T r') here: each G entry (row, k) gathers G^T's
-// row k and r', Ap at its columns; t_k is summed exactly as PRE 2 sums it
-// and z as K3 does, so the iterates are bitwise those of PRE 2 + K3.
+// K2.  PRE: 0 none, 1 Jacobi, 2 M = G G^T (t = G^T r' here from G^T's CSR
+// rows, z = G t in K3).
 template <int PRE, bool MIX>
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int count) {
+__global__ __launch_bounds__(kBlock) void k_cg_update(CG_ARGS, int par_g, int count) {
   TBSLA_NO_FMA
+  CG_BIND;
   typedef typename std::conditional<MIX, float, double>::type V;
   constexpr int P = 4;
   __shared__ double lds[kBlock / kWave];
@@ -615,7 +530,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   double *rn = a.r[par ^ 1];
   const double *pn = a.p[par ^ 1];
   const V *tv = MIX ? (const V *)a.t_v32 : (const V *)a.t_v64;
-  const V *gv = MIX ? (const V *)a.g_v32 : (const V *)a.g_v64;
   const int r0 = S->row0 + (b - b0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
   const int row_first = r0 + (int)threadIdx.x;
@@ -626,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   const double fd = PRE == 1 ? a.dinv[rowc] : 0.0;
   RowPrefetch<P, V> f;
   f.beg = f.end = 0;
-  if (PRE == 2 || PRE == 4) {
+  if (PRE == 2) {
     f.beg = a.t_rp[rowc];
     f.end = a.t_rp[rowc + 1];
 #pragma unroll
@@ -636,31 +550,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       f.v[k] = tv[j];
     }
   }
-  // PRE 3: the first row's G entries and the extents of the G^T rows they
-  // name
-  RowPrefetch<P, V> fg;
-  int tb[P], te[P];
-  fg.beg = fg.end = 0;
-  if (PRE == 3) {
-    fg.beg = a.g_rp[rowc];
-    fg.end = a.g_rp[rowc + 1];
-#pragma unroll
-    for (int k = 0; k < P; k++) {
-      const int j = min(fg.beg + k, a.g_nnz - 1);
-      fg.col[k] = a.g_ci[j];
-      fg.v[k] = gv[j];
-    }
-#pragma unroll
-    for (int k = 0; k < P; k++) {
-      tb[k] = a.t_rp[fg.col[k] - a.f_col];
-      te[k] = a.t_rp[fg.col[k] - a.f_col + 1];
-    }
-  }
   const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
   SegLoad sl;
   seg_issue(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, b1, sl);
   __builtin_amdgcn_sched_barrier(0);
-  if (PRE == 2 || PRE == 4) {
+  if (PRE == 2) {
 #pragma unroll
     for (int k = 0; k < P; k++) {
       f.s[k] = rc[f.col[k] - a.f_col];
@@ -673,9 +567,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
   const double rz = t[0], pap = t[1];
   const double alpha = rz / fmax(pap, S->near_zero);
   double rr = 0.0, rzp = 0.0;
-  // PRE 2/4: the first row, then the later rows two at a time (each pair's
+  // PRE 2: the first row, then the later rows two at a time (each pair's
   // loads issued before either is updated; sums in row order)
-  const int row_end = (PRE == 2 || PRE == 4) ? min(r1, row_first + 1) : r1;
+  const int row_end = PRE == 2 ? min(r1, row_first + 1) : r1;
   for (int row = row_first; row < row_end; row += kBlock) {
     const bool first = row == row_first;
     const double xv = first ? fx : a.x[row];
@@ -691,26 +585,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       a.z[row] = zi;
       rzp += ri * zi;
     }
-    if (PRE == 2 || PRE == 4) {
+    if (PRE == 2) {
       // t = G^T r' with r'_c = r_c - alpha Ap_c computed as above
       V acc = 0;
-      int j, end;
-      if (first) {
 #pragma unroll
-        for (int k = 0; k < P; k++) {
-          if (f.beg + k < f.end) {
-            const double rcv = f.s[k] + (-alpha) * f.t[k];
-            if (MIX) acc = fmaf((float)f.v[k], (float)rcv, (float)acc);
-            else acc = fma((double)f.v[k], rcv, (double)acc);
-          }
+      for (int k = 0; k < P; k++) {
+        if (f.beg + k < f.end) {
+          const double rcv = f.s[k] + (-alpha) * f.t[k];
+          if (MIX) acc = fmaf((float)f.v[k], (float)rcv, (float)acc);
+          else acc = fma((double)f.v[k], rcv, (double)acc);
         }
-        j = f.beg + P;
-        end = f.end;
-      } else {
-        j = a.t_rp[row];
-        end = a.t_rp[row + 1];
       }
-      for (; j < end; j++) {
+      for (int j = f.beg + P; j < f.end; j++) {
         const int c = a.t_ci[j] - a.f_col;
         const double rcv = rc[c] + (-alpha) * a.ap[c];
         if (MIX) acc = fmaf((float)tv[j], (float)rcv, (float)acc);
@@ -718,51 +604,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
       }
       if (MIX) a.t32[row] = (float)acc;
       else a.t64[row] = (double)acc;
-      if (PRE == 4) rzp += (double)acc * (double)acc;
-    }
-    if (PRE == 3) {
-      // z = G t' with t'_k = sum over G^T row k of G^T_kc r'_c, r'_c = r_c - alpha Ap_c
-      V zacc = 0;
-      int jg, gend;
-      if (first) {
-#pragma unroll
-        for (int k = 0; k < P; k++) {
-          if (fg.beg + k < fg.end) {
-            V tk = 0;
-            for (int j = tb[k]; j < te[k]; j++) {
-              const int c = a.t_ci[j] - a.f_col;
-              const double rcv = rc[c] + (-alpha) * a.ap[c];
-              if (MIX) tk = fmaf((float)tv[j], (float)rcv, (float)tk);
-              else tk = fma((double)tv[j], rcv, (double)tk);
-            }
-            if (MIX) zacc = fmaf((float)fg.v[k], (float)tk, (float)zacc);
-            else zacc = fma((double)fg.v[k], (double)tk, (double)zacc);
-          }
-        }
-        jg = fg.beg + P;
-        gend = fg.end;
-      } else {
-        jg = a.g_rp[row];
-        gend = a.g_rp[row + 1];
-      }
-      for (; jg < gend; jg++) {
-        const int k = a.g_ci[jg] - a.f_col;
-        V tk = 0;
-        for (int j = a.t_rp[k]; j < a.t_rp[k + 1]; j++) {
-          const int c = a.t_ci[j] - a.f_col;
-          const double rcv = rc[c] + (-alpha) * a.ap[c];
-          if (MIX) tk = fmaf((float)tv[j], (float)rcv, (float)tk);
-          else tk = fma((double)tv[j], rcv, (double)tk);
-        }
-        if (MIX) zacc = fmaf((float)gv[jg], (float)tk, (float)zacc);
-        else zacc = fma((double)gv[jg], (double)tk, (double)zacc);
-      }
-      const double zi = (double)zacc;
-      a.z[row] = zi;
-      rzp += ri * zi;
     }
   }
-  if (PRE == 2 || PRE == 4) {
+  if (PRE == 2) {
     constexpr int R2 = 2;
     for (int row = row_first + kBlock; row < r1; row += R2 * kBlock) {
       RowPrefetch<P, V> g[R2];
@@ -817,16 +661,111 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a, int par_g, int c
         }
         if (MIX) a.t32[rw[r]] = (float)acc;
         else a.t64[rw[r]] = (double)acc;
-        if (PRE == 4) rzp += (double)acc * (double)acc;
       }
     }
   }
   rr = wg_sum(rr, lds);
-  if (PRE == 1 || PRE == 3 || PRE == 4) rzp = wg_sum(rzp, lds);
+  if (PRE == 1) rzp = wg_sum(rzp, lds);
   if (threadIdx.x == 0) {
     a.prr[par][b] = rr;
     if (PRE == 0) a.prz[par][b] = rr;
-    if (PRE == 1 || PRE == 3 || PRE == 4) a.prz[par][b] = rzp;
+    if (PRE == 1) a.prz[par][b] = rzp;
+    if (b == b0) {
+      S->true_res = 0;
+      if (count) S->iters += 1;  // the test K1 made on this iteration passed
+    }
+  }
+}
+
+// Slot k of G's diagonal layout for row `src` (0 outside the block): G^T's
+// row i holds G_{i-o_k, i} at column i - o_k, i.e. slot (i - o_k, k).
+template <typename V>
+__device__ inline V gd_slot(const V *__restrict__ gd, long long src, int k, int ndg, long long n) {
+  return src < n ? gd[dia_at(src, k, ndg)] : (V)0;
+}
+
+// K2 with M = G G^T and G on diagonals: t_i = sum over G^T's row i in
+// ascending column (descending offset o_k <= 0) of G_{i-o_k, i} r'_{i-o_k},
+// zero slots skipped -- the order and the products of K2's CSR form, so the
+// two forms give bitwise the same iterates.  Rows go two per thread and
+// trip, all their loads issued before either row is updated.
+template <bool MIX, int KD>
+__global__ __launch_bounds__(kBlock) void k_cg_update_gdia(CG_ARGS, int par_g, int count) {
+  TBSLA_NO_FMA
+  CG_BIND;
+  typedef typename std::conditional<MIX, float, double>::type V;
+  // KD: at most that many diagonals of G (4 or 8, chosen on the host)
+  constexpr int R2 = 2;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ SegShared sh;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  const int par = (par_g ^ S->shift) & 1;
+  const int b0 = S->blk0, b1 = S->blk1;
+  const double *rc = a.r[par];
+  double *rn = a.r[par ^ 1];
+  const double *pn = a.p[par ^ 1];
+  const V *gd = MIX ? (const V *)a.gd32 : (const V *)a.gd64;
+  const int ndg = a.gndiag;
+  const long long n = a.ncol;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  int off[KD];
+#pragma unroll
+  for (int k = 0; k < KD; k++) off[k] = k < ndg ? a.gdoff[k] : 0;
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.ppap, nullptr, 2, b0, b1, sl);
+  double t[3];
+  if (seg_finish(flag, sl, 2, b0, b1, sh, t) != kRunning) return;
+  const double rz = t[0], pap = t[1];
+  const double alpha = rz / fmax(pap, S->near_zero);
+  double rr = 0.0;
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += R2 * kBlock) {
+    double xv[R2], pv[R2], rv[R2], apv[R2], rs[R2][KD], as[R2][KD];
+    V gv[R2][KD];
+    int rw[R2];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      rw[r] = min(row + r * kBlock, r1 - 1);
+      xv[r] = a.x[rw[r]];
+      pv[r] = pn[rw[r]];
+      rv[r] = rc[rw[r]];
+      apv[r] = a.ap[rw[r]];
+#pragma unroll
+      for (int k = 0; k < KD; k++) {
+        if (k < ndg) {
+          const long long src = (long long)rw[r] - off[k];
+          const long long sc = src < n ? src : rw[r];
+          gv[r][k] = gd_slot(gd, src, k, ndg, n);
+          rs[r][k] = rc[sc];
+          as[r][k] = a.ap[sc];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      if (row + r * kBlock >= r1) break;
+      a.x[rw[r]] = xv[r] + alpha * pv[r];
+      const double ri = rv[r] + (-alpha) * apv[r];
+      rn[rw[r]] = ri;
+      rr += ri * ri;
+      V acc = 0;
+#pragma unroll
+      for (int k = KD - 1; k >= 0; k--) {
+        if (k < ndg && gv[r][k] != (V)0) {
+          const double rcv = rs[r][k] + (-alpha) * as[r][k];
+          if (MIX) acc = fmaf((float)gv[r][k], (float)rcv, (float)acc);
+          else acc = fma((double)gv[r][k], rcv, (double)acc);
+        }
+      }
+      if (MIX) a.t32[rw[r]] = (float)acc;
+      else a.t64[rw[r]] = (double)acc;
+    }
+  }
+  rr = wg_sum(rr, lds);
+  if (threadIdx.x == 0) {
+    a.prr[par][b] = rr;
     if (b == b0) {
       S->true_res = 0;
       if (count) S->iters += 1;  // the test K1 made on this iteration passed
@@ -861,8 +800,9 @@ __device__ bool mode_select(const CgSeg *S, int mode, int par_g, int *s_flag, in
 // first row's entries of G and the values of t they pick are loaded before
 // the done flag is shared (as in K1 and K2).
 template <bool MIX>
-__global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode) {
+__global__ __launch_bounds__(kBlock) void k_cg_gz(CG_ARGS, int par_g, int mode) {
   TBSLA_NO_FMA
+  CG_BIND;
   typedef typename std::conditional<MIX, float, double>::type V;
   constexpr int P = 4;
   __shared__ double lds[kBlock / kWave];
@@ -968,12 +908,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_gz(CgArgs a, int par_g, int mode)
   if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
 }
 
-// t = G^T r for the current residual (replacement, verification, start);
-// TT (the product form): also the t.t partials, r.z of M = G G^T.
-template <bool MIX, bool TT>
-__global__ __launch_bounds__(kBlock) void k_cg_gtr(CgArgs a, int par_g, int mode) {
+// t = G^T r for the current residual (replacement, verification, start).
+template <bool MIX>
+__global__ __launch_bounds__(kBlock) void k_cg_gtr(CG_ARGS, int par_g, int mode) {
   TBSLA_NO_FMA
-  __shared__ double lds[kBlock / kWave];
+  CG_BIND;
   __shared__ int s_flag;
   const int b = blockIdx.x;
   CgSeg *S = a.seg + a.block_seg[b];
@@ -982,60 +921,155 @@ __global__ __launch_bounds__(kBlock) void k_cg_gtr(CgArgs a, int par_g, int mode
   const double *rc = a.r[rsel];
   const int r0 = S->row0 + (b - S->blk0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
-  double tt = 0.0;
   for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
     if (MIX) {
       float acc = 0.0f;
       for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++)
         acc = fmaf(a.t_v32[j], (float)rc[a.t_ci[j] - a.f_col], acc);
       a.t32[row] = acc;
-      if (TT) tt += (double)acc * (double)acc;
     } else {
       double acc = 0.0;
       for (int j = a.t_rp[row]; j < a.t_rp[row + 1]; j++)
         acc = fma(a.t_v64[j], rc[a.t_ci[j] - a.f_col], acc);
       a.t64[row] = acc;
-      if (TT) tt += acc * acc;
     }
-  }
-  if (TT) {
-    tt = wg_sum(tt, lds);
-    if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = tt;
   }
 }
 
-// ap = A p (FP64 matrix) for the current p of the running systems (the
-// product form's Ap recurrence restarts from it at each residual
-// replacement).
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_cg_ap(CgArgs a, int par_g) {
+// k_cg_gtr with G on diagonals (G^T's row in ascending column, zero slots
+// skipped, as k_cg_update_gdia).
+template <bool MIX>
+__global__ __launch_bounds__(kBlock) void k_cg_gtr_gdia(CG_ARGS, int par_g, int mode) {
+  TBSLA_NO_FMA
+  CG_BIND;
+  typedef typename std::conditional<MIX, float, double>::type V;
   __shared__ int s_flag;
   const int b = blockIdx.x;
   CgSeg *S = a.seg + a.block_seg[b];
   int rsel;
-  if (!mode_select(S, kAfterIter, par_g, &s_flag, &rsel)) return;
-  const double *pc = a.p[rsel];
+  if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
+  const double *rc = a.r[rsel];
+  const V *gd = MIX ? (const V *)a.gd32 : (const V *)a.gd64;
+  const int ndg = a.gndiag;
+  const long long n = a.ncol;
   const int r0 = S->row0 + (b - S->blk0) * S->rpb;
   const int r1 = min(S->row1, r0 + S->rpb);
-  const int lane = threadIdx.x & (L - 1);
-  for (int row = r0 + (int)threadIdx.x / L; row < r1; row += kBlock / L) {
-    double acc = 0.0;
-    for (int j = a.rowptr[row] + lane; j < a.rowptr[row + 1]; j += L)
-      acc = fma(a.val64[j], pc[a.colidx[j] - a.f_col], acc);
-    acc = lane_sum<L>(acc);
-    if (lane == 0) a.ap[row] = acc;
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += kBlock) {
+    V acc = 0;
+    for (int k = ndg - 1; k >= 0; k--) {
+      const long long src = (long long)row - a.gdoff[k];
+      const V g = gd_slot(gd, src, k, ndg, n);
+      if (g != (V)0) {
+        if (MIX) acc = fmaf((float)g, (float)rc[src], (float)acc);
+        else acc = fma((double)g, rc[src], (double)acc);
+      }
+    }
+    if (MIX) a.t32[row] = (float)acc;
+    else a.t64[row] = (double)acc;
   }
 }
 
-// r = b - ax (ax = A x in FP64, or 0 at a zero start) into the selected
+// K3 with G on diagonals: z_i = sum over G's row i in ascending column
+// (ascending offset) of G_{i,i+o_k} t_{i+o_k}, zero slots skipped; r.z
+// partials.  Rows two per thread and trip, loads of both issued first.
+template <bool MIX, int KD>
+__global__ __launch_bounds__(kBlock) void k_cg_gz_gdia(CG_ARGS, int par_g, int mode) {
+  TBSLA_NO_FMA
+  CG_BIND;
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int R2 = 2;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ int s_flag;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  int rsel;
+  if (!mode_select(S, mode, par_g, &s_flag, &rsel)) return;
+  const double *rc = a.r[rsel];
+  const V *gd = MIX ? (const V *)a.gd32 : (const V *)a.gd64;
+  const V *tv = MIX ? (const V *)a.t32 : (const V *)a.t64;
+  const int ndg = a.gndiag;
+  const int r0 = S->row0 + (b - S->blk0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  int off[KD];
+#pragma unroll
+  for (int k = 0; k < KD; k++) off[k] = k < ndg ? a.gdoff[k] : 0;
+  double rz = 0.0;
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += R2 * kBlock) {
+    V gv[R2][KD], tq[R2][KD];
+    double rv[R2];
+    int rw[R2];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      rw[r] = min(row + r * kBlock, r1 - 1);
+      rv[r] = rc[rw[r]];
+#pragma unroll
+      for (int k = 0; k < KD; k++) {
+        if (k < ndg) {
+          const int c = rw[r] + off[k];
+          gv[r][k] = gd[dia_at(rw[r], k, ndg)];
+          tq[r][k] = tv[c >= 0 ? c : rw[r]];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      if (row + r * kBlock >= r1) break;
+      V acc = 0;
+#pragma unroll
+      for (int k = 0; k < KD; k++)
+        if (k < ndg && gv[r][k] != (V)0) {
+          if (MIX) acc = fmaf((float)gv[r][k], (float)tq[r][k], (float)acc);
+          else acc = fma((double)gv[r][k], (double)tq[r][k], (double)acc);
+        }
+      const double zi = (double)acc;
+      a.z[rw[r]] = zi;
+      rz += rv[r] * zi;
+    }
+  }
+  rz = wg_sum(rz, lds);
+  if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
+}
+
+// G's entries into its diagonal layout (slots zeroed first): the slot of
+// offset col - row among the offsets given; *bad set when an entry lies on
+// none of them or two entries of a row share one (G is then applied from its
+// CSR rows).
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_g_to_dia(long long n, int f_col, int f_row,
+                                                     const int *__restrict__ rp,
+                                                     const int *__restrict__ ci,
+                                                     const double *__restrict__ v,
+                                                     const int *__restrict__ off, int ndg,
+                                                     V *__restrict__ gd, int *bad) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    unsigned used = 0;
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+      const long long o = (long long)(ci[j] - f_col) - (i + f_row - f_col);
+      int k = -1;
+      for (int q = 0; q < ndg; q++)
+        if (off[q] == o) k = q;
+      if (k < 0 || (used >> k) & 1u) {
+        atomicOr(bad, 1);
+        continue;
+      }
+      used |= 1u << k;
+      gd[dia_at(i, k, ndg)] = (V)v[j];
+    }
+  }
+}
+
+// r = b - ax (ax = A x in FP64 when use_ax, else 0: a zero start) into the selected
 // buffer, z = D^-1 r for Jacobi, and the r.r (r.z) partials.  At the start
 // it also zeroes x (unless an initial guess is given) and p, seeds the
 // "old r.z" slot with +inf (so the first beta is 0) and leaves b.b partials
 // in ppap.
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void k_cg_residual(CgArgs a, const double *__restrict__ ax,
-                                                        int par_g, int mode, int zero_x) {
+__global__ __launch_bounds__(kBlock) void k_cg_residual(CG_ARGS, int use_ax, int par_g, int mode,
+                                                        int zero_x) {
   TBSLA_NO_FMA
+  CG_BIND;
+  const double *ax = use_ax ? a.ax : nullptr;
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
@@ -1080,8 +1114,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(CgArgs a, const double *
 }
 
 // Per-system setup after the start residual: threshold, |r0|, counters.
-__global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int relative,
+__global__ __launch_bounds__(kBlock) void k_cg_start(CG_ARGS, double tol, int relative,
                                                      double near_zero) {
+  CG_BIND;
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
@@ -1110,7 +1145,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs a, double tol, int r
 // After verification: accept (true |r| below the threshold) or resume at
 // the next global iteration j from the replaced residual (only j's parity
 // matters).
-__global__ __launch_bounds__(kBlock) void k_cg_vcheck(CgArgs a, int j) {
+__global__ __launch_bounds__(kBlock) void k_cg_vcheck(CG_ARGS, int j) {
+  CG_BIND;
   __shared__ double lds[kBlock / kWave];
   __shared__ int s_flag;
   const int b = blockIdx.x;
@@ -1150,8 +1186,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_vcheck(CgArgs a, int j) {
 // the engine's workgroup layout (workgroups of other systems return at
 // once); L lanes per row.
 template <int L>
-__global__ __launch_bounds__(kBlock) void k_cg_ax(CgArgs a, int par_g, int mode,
-                                                  double *__restrict__ ax) {
+__global__ __launch_bounds__(kBlock) void k_cg_ax(CG_ARGS, int par_g, int mode) {
+  CG_BIND;
+  double *ax = a.ax;
   __shared__ int s_flag;
   const int b = blockIdx.x;
   CgSeg *S = a.seg + a.block_seg[b];
@@ -1201,6 +1238,8 @@ template <typename T>
 struct Buf {
   T *p = nullptr;
   size_t n = 0;
+  // grows only; a reallocation waits for the device first (work in flight
+  // may still read the old buffer)
   int ensure(size_t count) {
     if (count <= n && p) return TBSLA_HIP_SUCCESS;
     if (p) {
@@ -1220,18 +1259,22 @@ struct Buf {
   }
 };
 
-// What a captured graph bakes in: every kernel argument and the launch plan.
+// What a captured graph bakes in besides the argument block's address: the
+// launch plan (kernel variants, grid, iteration indices).  Pointers are not
+// part of it -- every kernel reads them from the argument block at run time.
 struct GraphKey {
-  CgArgs args;
-  const void *spmv_rowptr, *spmv_colidx, *spmv_values, *spmv_split;
-  int L, Lk1, pre, mixed, replace, batch, nblocks, spmv_lanes, spmv_rows, spmv_vec, spmv_nt, fused, product,
-      dia;
-  bool operator==(const GraphKey &o) const { return std::memcmp(this, &o, sizeof o) == 0; }
+  int L, pre, mixed, replace, batch, nblocks, verify, dia, gdia;
+  bool operator==(const GraphKey &o) const {
+    return L == o.L && pre == o.pre && mixed == o.mixed && replace == o.replace &&
+           batch == o.batch && nblocks == o.nblocks && verify == o.verify && dia == o.dia &&
+           gdia == o.gdia;
+  }
 };
 
 }  // namespace
 
-// Work vectors, partials and graphs kept by a matrix handle between solves.
+// Work vectors, partials, the argument block and graphs kept by a matrix
+// handle between solves.
 struct CgCache {
   int device = 0;
   Buf<double> b, x, r0, r1, p0, p1, z, ap, t64, dinv, ax;
@@ -1239,13 +1282,17 @@ struct CgCache {
   Buf<double> ppap, prr0, prr1, prz0, prz1;
   Buf<CgSeg> seg;
   Buf<CgCom> com;
-
   Buf<int> block_seg;
   // stand-ins for A's or G^T's entry arrays when they hold no entry: one
   // column index (the block's first column) and one zero value
   Buf<int> dummy_i;
   Buf<double> dummy_d;
+  // the kernels' argument block (one CgArgs, allocated once and never moved:
+  // captured graphs name it) and its pinned staging copy
+  Buf<CgArgs> args;
+  CgArgs *h_args = nullptr;
   CgCom *h_com = nullptr;  // pinned status copies (2)
+  int *h_probe = nullptr;  // pinned: G's probe rows (extents, columns) and the dia check
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipStream_t own = nullptr;  // used when the caller passes the null stream
   // FP32 copy of A's values for the mixed form, made from the handle uid
@@ -1259,32 +1306,19 @@ struct CgCache {
   float *dv32 = nullptr;
   size_t dv32_len = 0;
   unsigned long long dv32_build = 0;
-  // factor G: FP32 copies of G and G^T
-  tbsla_hip_matrix_s *factor = nullptr;
+  // factor G in its CSR form: G^T's rows (when not given) and FP32 copies,
+  // remade on every call (G may have been rewritten in place)
   int *t_rp = nullptr, *t_ci = nullptr;
   double *t_v64 = nullptr;
   float *g_v32 = nullptr, *t_v32 = nullptr;
   size_t f32_len = 0;
-  // product form: A G (rebuilt per call into buffers kept while large
-  // enough, so a graph captured for one problem serves the next of the same
-  // size), FP32 copy for the mixed form
-  int *ag_rp = nullptr, *ag_ci = nullptr;
-  double *ag_v64 = nullptr;
-  float *ag_v32 = nullptr;
-  long long ag_nnz = 0, ag_cap = 0, ag_rows = 0;
-  void drop_product() {
-    hipFree(ag_rp);
-    hipFree(ag_ci);
-    hipFree(ag_v64);
-    hipFree(ag_v32);
-    ag_rp = ag_ci = nullptr;
-    ag_v64 = nullptr;
-    ag_v32 = nullptr;
-    ag_nnz = ag_cap = ag_rows = 0;
-  }
-  // instantiated graphs, one set per plan shape (solver, precision,
-  // preconditioner, batch, layout), most recently used first: exec[0] a
-  // plain batch, exec[1] a batch ending with a replacement step
+  // factor G on diagonals: offsets, slots (FP64 or FP32), the check flag
+  Buf<int> gdoff;
+  Buf<double> gd64;
+  Buf<float> gd32;
+  Buf<int> gbad;
+  // instantiated graphs, one set per launch plan, most recently used first:
+  // exec[0] a plain batch, exec[1] a batch ending with a replacement step
   struct GraphSet {
     GraphKey key;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -1311,26 +1345,29 @@ struct CgCache {
     t_v64 = nullptr;
     g_v32 = t_v32 = nullptr;
     f32_len = 0;
-    factor = nullptr;
-    drop_product();
   }
   ~CgCache() {
     hipDeviceSynchronize();
+    for (auto &g : graphs) g.destroy();
     Buf<double> *bufs[] = {&b, &x, &r0, &r1, &p0, &p1, &z, &ap, &t64, &dinv, &ax,
-                           &ppap, &prr0, &prr1, &prz0, &prz1};
+                           &ppap, &prr0, &prr1, &prz0, &prz1, &gd64};
     for (auto *q : bufs) q->release();
     t32.release();
+    gd32.release();
     seg.release();
     com.release();
-
     block_seg.release();
     dummy_i.release();
     dummy_d.release();
+    gdoff.release();
+    gbad.release();
+    args.release();
+    if (h_args) hipHostFree(h_args);
     if (h_com) hipHostFree(h_com);
+    if (h_probe) hipHostFree(h_probe);
     for (auto &e : ev)
       if (e) hipEventDestroy(e);
     if (own) hipStreamDestroy(own);
-    for (auto &g : graphs) g.destroy();
     drop_factor();
     hipFree(a_v32);
     hipFree(dv32);
@@ -1341,7 +1378,9 @@ struct CgCache {
 // stream, events, pinned status words, work vectors and instantiated graphs
 // are reused (creating them costs 1-2 ms per new matrix, a third of a 64^3
 // solve).  At most kSpareCaches, per device, under a mutex; never freed at
-// process exit (the runtime may be gone by then).
+// process exit (the runtime may be gone by then).  A reused cache's graphs
+// name only its own argument block, so they serve the next handle as they
+// are.
 constexpr size_t kSpareCaches = 2;
 
 std::mutex &spare_mutex() {
@@ -1392,170 +1431,213 @@ namespace {
 
 struct Plan {
   int L, pre, mixed, replace, verify, stall;
-  int Lk1;  // lanes per row of K1 (the product form reads A G's longer rows)
-  bool fused;    // M = G G^T: z = G (G^T r) in K2
-  bool product;  // M = G G^T in the product form (K1 with A G rows, no K3)
-  bool dia;      // K1 reads A's diagonal layout
+  bool dia;   // K1 reads A's diagonal layout
+  bool gdia;  // M = G G^T applied from G's diagonals
+  int gkd;    // their kernels' capacity: 4 or 8 diagonals
   long long n;
   int nblocks;
+  const CgArgs *args;  // device argument block
   hipStream_t s;
 };
 
 template <int L, bool MIX, bool ZBUF>
-void launch_k1(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
-  k_cg_spmv_p<L, MIX, ZBUF><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+void launch_k1(const Plan &p, int par, int check, int check_only) {
+  k_cg_spmv_p<L, MIX, ZBUF><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
 }
 
 template <int L>
-void launch_k1_l(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
+void launch_k1_l(const Plan &p, int par, int check, int check_only) {
+  const bool zbuf = p.pre != 0;
   if (p.dia) {
-    const bool zbuf = p.pre != 0;
     if (p.mixed) {
-      if (zbuf) k_cg_spmv_dia<true, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
-      else k_cg_spmv_dia<true, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+      if (zbuf) k_cg_spmv_dia<true, true><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
+      else k_cg_spmv_dia<true, false><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
     } else {
-      if (zbuf) k_cg_spmv_dia<false, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
-      else k_cg_spmv_dia<false, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
+      if (zbuf) k_cg_spmv_dia<false, true><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
+      else k_cg_spmv_dia<false, false><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
     }
     return;
   }
-  if (p.product) {
-    if (p.mixed) k_cg_spmv_ag<L, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
-    else k_cg_spmv_ag<L, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, check, check_only);
-    return;
-  }
-  const bool zbuf = p.pre != 0;
   if (p.mixed) {
-    if (zbuf) launch_k1<L, true, true>(a, p, par, check, check_only);
-    else launch_k1<L, true, false>(a, p, par, check, check_only);
+    if (zbuf) launch_k1<L, true, true>(p, par, check, check_only);
+    else launch_k1<L, true, false>(p, par, check, check_only);
   } else {
-    if (zbuf) launch_k1<L, false, true>(a, p, par, check, check_only);
-    else launch_k1<L, false, false>(a, p, par, check, check_only);
+    if (zbuf) launch_k1<L, false, true>(p, par, check, check_only);
+    else launch_k1<L, false, false>(p, par, check, check_only);
   }
 }
 
-void launch_k1_any(const CgArgs &a, const Plan &p, int par, int check, int check_only) {
-  switch (p.Lk1) {
-    case 1: launch_k1_l<1>(a, p, par, check, check_only); break;
-    case 2: launch_k1_l<2>(a, p, par, check, check_only); break;
-    case 4: launch_k1_l<4>(a, p, par, check, check_only); break;
-    case 8: launch_k1_l<8>(a, p, par, check, check_only); break;
-    case 16: launch_k1_l<16>(a, p, par, check, check_only); break;
-    case 32: launch_k1_l<32>(a, p, par, check, check_only); break;
-    default: launch_k1_l<64>(a, p, par, check, check_only); break;
+void launch_k1_any(const Plan &p, int par, int check, int check_only) {
+  switch (p.L) {
+    case 1: launch_k1_l<1>(p, par, check, check_only); break;
+    case 2: launch_k1_l<2>(p, par, check, check_only); break;
+    case 4: launch_k1_l<4>(p, par, check, check_only); break;
+    case 8: launch_k1_l<8>(p, par, check, check_only); break;
+    case 16: launch_k1_l<16>(p, par, check, check_only); break;
+    case 32: launch_k1_l<32>(p, par, check, check_only); break;
+    default: launch_k1_l<64>(p, par, check, check_only); break;
   }
 }
 
-void launch_k2(const CgArgs &a, const Plan &p, int par, int count, bool fused) {
+void launch_k2(const Plan &p, int par, int count) {
   const int g = p.nblocks;
   if (p.pre == 0) {
-    if (p.mixed) k_cg_update<0, true><<<g, kBlock, 0, p.s>>>(a, par, count);
-    else k_cg_update<0, false><<<g, kBlock, 0, p.s>>>(a, par, count);
+    if (p.mixed) k_cg_update<0, true><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+    else k_cg_update<0, false><<<g, kBlock, 0, p.s>>>(p.args, par, count);
   } else if (p.pre == 1) {
-    if (p.mixed) k_cg_update<1, true><<<g, kBlock, 0, p.s>>>(a, par, count);
-    else k_cg_update<1, false><<<g, kBlock, 0, p.s>>>(a, par, count);
-  } else if (fused) {
-    if (p.mixed) k_cg_update<3, true><<<g, kBlock, 0, p.s>>>(a, par, count);
-    else k_cg_update<3, false><<<g, kBlock, 0, p.s>>>(a, par, count);
-  } else if (p.product) {
-    if (p.mixed) k_cg_update<4, true><<<g, kBlock, 0, p.s>>>(a, par, count);
-    else k_cg_update<4, false><<<g, kBlock, 0, p.s>>>(a, par, count);
+    if (p.mixed) k_cg_update<1, true><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+    else k_cg_update<1, false><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+  } else if (p.gdia) {
+    if (p.gkd == 4) {
+      if (p.mixed) k_cg_update_gdia<true, 4><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+      else k_cg_update_gdia<false, 4><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+    } else {
+      if (p.mixed) k_cg_update_gdia<true, 8><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+      else k_cg_update_gdia<false, 8><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+    }
   } else {
-    if (p.mixed) k_cg_update<2, true><<<g, kBlock, 0, p.s>>>(a, par, count);
-    else k_cg_update<2, false><<<g, kBlock, 0, p.s>>>(a, par, count);
+    if (p.mixed) k_cg_update<2, true><<<g, kBlock, 0, p.s>>>(p.args, par, count);
+    else k_cg_update<2, false><<<g, kBlock, 0, p.s>>>(p.args, par, count);
   }
 }
 
-void launch_gz(const CgArgs &a, const Plan &p, int par, int mode) {
-  if (p.mixed) k_cg_gz<true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
-  else k_cg_gz<false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
-}
-
-void launch_gtr(const CgArgs &a, const Plan &p, int par, int mode) {
-  if (p.product) {
-    if (p.mixed) k_cg_gtr<true, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
-    else k_cg_gtr<false, true><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+void launch_gz(const Plan &p, int par, int mode) {
+  if (p.gdia) {
+    if (p.gkd == 4) {
+      if (p.mixed) k_cg_gz_gdia<true, 4><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+      else k_cg_gz_gdia<false, 4><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+    } else {
+      if (p.mixed) k_cg_gz_gdia<true, 8><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+      else k_cg_gz_gdia<false, 8><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+    }
   } else {
-    if (p.mixed) k_cg_gtr<true, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
-    else k_cg_gtr<false, false><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode);
+    if (p.mixed) k_cg_gz<true><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+    else k_cg_gz<false><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
   }
 }
 
-// the product form's A p (FP64) for the running systems
-void launch_ap(const CgArgs &a, const Plan &p, int par) {
+void launch_gtr(const Plan &p, int par, int mode) {
+  if (p.gdia) {
+    if (p.mixed) k_cg_gtr_gdia<true><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+    else k_cg_gtr_gdia<false><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+  } else {
+    if (p.mixed) k_cg_gtr<true><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+    else k_cg_gtr<false><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode);
+  }
+}
+
+void launch_residual(const Plan &p, int use_ax, int par, int mode, int zero_x) {
+  if (p.pre == 1) k_cg_residual<1><<<p.nblocks, kBlock, 0, p.s>>>(p.args, use_ax, par, mode, zero_x);
+  else if (p.pre == 0) k_cg_residual<0><<<p.nblocks, kBlock, 0, p.s>>>(p.args, use_ax, par, mode, zero_x);
+  else k_cg_residual<2><<<p.nblocks, kBlock, 0, p.s>>>(p.args, use_ax, par, mode, zero_x);
+}
+
+// ax = A x with the FP64 matrix, for the systems the mode selects
+void launch_ax(const Plan &p, int par, int mode) {
   switch (p.L) {
-    case 1: k_cg_ap<1><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    case 2: k_cg_ap<2><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    case 4: k_cg_ap<4><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    case 8: k_cg_ap<8><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    case 16: k_cg_ap<16><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    case 32: k_cg_ap<32><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
-    default: k_cg_ap<64><<<p.nblocks, kBlock, 0, p.s>>>(a, par); break;
+    case 1: k_cg_ax<1><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    case 2: k_cg_ax<2><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    case 4: k_cg_ax<4><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    case 8: k_cg_ax<8><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    case 16: k_cg_ax<16><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    case 32: k_cg_ax<32><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
+    default: k_cg_ax<64><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, mode); break;
   }
 }
 
-void launch_residual(const CgArgs &a, const Plan &p, const double *ax, int par, int mode,
-                     int zero_x) {
-  if (p.pre == 1) k_cg_residual<1><<<p.nblocks, kBlock, 0, p.s>>>(a, ax, par, mode, zero_x);
-  else if (p.pre == 0) k_cg_residual<0><<<p.nblocks, kBlock, 0, p.s>>>(a, ax, par, mode, zero_x);
-  else k_cg_residual<2><<<p.nblocks, kBlock, 0, p.s>>>(a, ax, par, mode, zero_x);
-}
-
-// r = b - A x with the FP64 matrix, for the systems the mode selects
-void launch_ax(const CgArgs &a, const Plan &p, int par, int mode, double *ax) {
-  switch (p.L) {
-    case 1: k_cg_ax<1><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    case 2: k_cg_ax<2><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    case 4: k_cg_ax<4><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    case 8: k_cg_ax<8><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    case 16: k_cg_ax<16><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    case 32: k_cg_ax<32><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-    default: k_cg_ax<64><<<p.nblocks, kBlock, 0, p.s>>>(a, par, mode, ax); break;
-  }
-}
-
-int true_residual(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double *ax, int par,
-                  int mode) {
-  (void)m;
-  launch_ax(a, p, par, mode, ax);
-  launch_residual(a, p, ax, par, mode, 0);
+// r = b - A x (FP64), then z = M r for the systems the mode selects
+void true_residual(const Plan &p, int par, int mode) {
+  launch_ax(p, par, mode);
+  launch_residual(p, 1, par, mode, 0);
   if (p.pre == 2) {
-    launch_gtr(a, p, par, mode);
-    if (!p.product) launch_gz(a, p, par, mode);
+    launch_gtr(p, par, mode);
+    launch_gz(p, par, mode);
   }
-  // the product form's Ap recurrence restarts from the true A p
-  if (p.product && mode == kAfterIter) launch_ap(a, p, par);
-  return TBSLA_HIP_SUCCESS;
 }
 
-// iteration i (global): K1 (tested when i > 0), K2, K3 (unless K2 made z); then, after every
+// iteration i (global): K1 (tested when i > 0), K2, K3; then, after every
 // replace_every-th iteration (i a multiple of it), the true FP64 residual
 // replaces the recurrence's.  tail: also settle the systems waiting for
 // their true-residual test (a replacement step does it anyway: the same
 // FP64 product serves both), so they wait at most one batch, without a host
 // round trip.
-int enqueue_iteration(tbsla_hip_matrix_s *m, const CgArgs &a, const Plan &p, double *ax, int i,
-                      bool tail) {
+void enqueue_iteration(const Plan &p, int i, bool tail) {
   const int par = i & 1;
-  launch_k1_any(a, p, par, i > 0, 0);
+  launch_k1_any(p, par, i > 0, 0);
   const bool replace = p.replace > 0 && i > 0 && i % p.replace == 0;
-  // a replacement step recomputes z from the true residual: K2 then needs
-  // no z of its own
-  launch_k2(a, p, par, i > 0, p.fused && !replace);  // product form: PRE 4 (t.t) or, replacing, 2
-  if (replace) {
-    TBSLA_TRY(true_residual(m, a, p, ax, par, kAfterIter));
-  } else if (p.pre == 2 && !p.fused) {
-    launch_gz(a, p, par, kAfterIter);
-  }
+  launch_k2(p, par, i > 0);
+  // a replacement step recomputes z from the true residual: no K3 of its own
+  if (replace) true_residual(p, par, kAfterIter);
+  else if (p.pre == 2) launch_gz(p, par, kAfterIter);
   if (p.verify && (replace || tail)) {
-    launch_ax(a, p, par, kVerify, ax);
-    launch_residual(a, p, ax, par, kVerify, 0);
-    if (p.pre == 2) {
-      launch_gtr(a, p, par, kVerify);
-      if (!p.product) launch_gz(a, p, par, kVerify);
-    }
-    k_cg_vcheck<<<p.nblocks, kBlock, 0, p.s>>>(a, (i + 1) & 1);
+    true_residual(p, par, kVerify);
+    k_cg_vcheck<<<p.nblocks, kBlock, 0, p.s>>>(p.args, (i + 1) & 1);
   }
+}
+
+// G's diagonals for the diagonal form of M = G G^T: the offsets of the
+// entries of three probe rows (at most 8 distinct, all <= 0: G lower
+// triangular), then every entry placed on one of them.  *ndg: the count of
+// diagonals, or 0 (G is then applied from its CSR rows) when the probe or
+// the placement fails.
+int build_gdia(CgCache &c, const tbsla_hip_matrix_s *G, bool mixed, hipStream_t s, int *ndg_out) {
+  *ndg_out = 0;
+  const long long n = G->ln_row;
+  if (n < 1 || G->stored < 1) return TBSLA_HIP_SUCCESS;
+  constexpr int kProbe = 3, kMaxDiag = 8;
+  if (!c.h_probe)
+    TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_probe, 64 * sizeof(int), hipHostMallocDefault));
+  int *h = c.h_probe;
+  const long long rows[kProbe] = {n / 4, n / 2, (3 * n) / 4};
+  for (int q = 0; q < kProbe; q++)
+    TBSLA_HIP_CHECK(hipMemcpyAsync(h + 2 * q, G->rowptr + rows[q], 2 * sizeof(int),
+                                   hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  int len[kProbe], total = 0;
+  for (int q = 0; q < kProbe; q++) {
+    len[q] = h[2 * q + 1] - h[2 * q];
+    if (len[q] < 0 || len[q] > kMaxDiag) return TBSLA_HIP_SUCCESS;
+  }
+  int beg[kProbe];
+  for (int q = 0; q < kProbe; q++) beg[q] = h[2 * q];
+  for (int q = 0; q < kProbe; q++) {
+    if (len[q] > 0)
+      TBSLA_HIP_CHECK(hipMemcpyAsync(h + 8 + total, G->colidx + beg[q], len[q] * sizeof(int),
+                                     hipMemcpyDeviceToHost, s));
+    total += len[q];
+  }
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int> off;
+  for (int q = 0, at = 8; q < kProbe; q++)
+    for (int k = 0; k < len[q]; k++, at++) {
+      const long long o = (long long)(h[at] - G->f_col) - rows[q];
+      if (o > 0 || o < -(long long)INT32_MAX) return TBSLA_HIP_SUCCESS;
+      if (std::find(off.begin(), off.end(), (int)o) == off.end()) off.push_back((int)o);
+    }
+  if (off.empty() || off.size() > (size_t)kMaxDiag) return TBSLA_HIP_SUCCESS;
+  std::sort(off.begin(), off.end());
+  const int ndg = (int)off.size();
+  const size_t slots = (size_t)((n + kWave - 1) / kWave) * kWave * ndg;
+  TBSLA_TRY(c.gdoff.ensure(kMaxDiag));
+  TBSLA_TRY(c.gbad.ensure(1));
+  for (int k = 0; k < ndg; k++) h[32 + k] = off[k];
+  TBSLA_HIP_CHECK(hipMemcpyAsync(c.gdoff.p, h + 32, ndg * sizeof(int), hipMemcpyHostToDevice, s));
+  TBSLA_HIP_CHECK(hipMemsetAsync(c.gbad.p, 0, sizeof(int), s));
+  if (mixed) {
+    TBSLA_TRY(c.gd32.ensure(slots));
+    TBSLA_HIP_CHECK(hipMemsetAsync(c.gd32.p, 0, slots * sizeof(float), s));
+    k_g_to_dia<float><<<sgrid(n), kBlock, 0, s>>>(n, G->f_col, G->f_row, G->rowptr, G->colidx,
+                                                  G->values, c.gdoff.p, ndg, c.gd32.p, c.gbad.p);
+  } else {
+    TBSLA_TRY(c.gd64.ensure(slots));
+    TBSLA_HIP_CHECK(hipMemsetAsync(c.gd64.p, 0, slots * sizeof(double), s));
+    k_g_to_dia<double><<<sgrid(n), kBlock, 0, s>>>(n, G->f_col, G->f_row, G->rowptr, G->colidx,
+                                                   G->values, c.gdoff.p, ndg, c.gd64.p, c.gbad.p);
+  }
+  TBSLA_HIP_LAUNCH_CHECK();
+  TBSLA_HIP_CHECK(hipMemcpyAsync(h + 48, c.gbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  if (!h[48]) *ndg_out = ndg;
   return TBSLA_HIP_SUCCESS;
 }
 
@@ -1595,7 +1677,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown preconditioner");
   if (o.precision != TBSLA_HIP_CG_FP64 && o.precision != TBSLA_HIP_CG_MIXED)
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "unknown precision");
-  if (o.factor_form < TBSLA_HIP_FACTOR_AUTO || o.factor_form > TBSLA_HIP_FACTOR_PRODUCT)
+  if (o.factor_form < TBSLA_HIP_FACTOR_AUTO || o.factor_form > TBSLA_HIP_FACTOR_DIAGONALS)
     return fail(TBSLA_HIP_ERR_INVALID, "unknown factor_form");
   tbsla_hip_matrix_s *G = o.factor;
   const int pre = o.preconditioner;
@@ -1603,6 +1685,9 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     if (!G || G->format != TBSLA_HIP_CSR || G->ln_row != n || G->ln_col != n ||
         G->f_row != m->f_row || G->f_col != m->f_col)
       return fail(TBSLA_HIP_ERR_INVALID, "factor G must be a CSR block of the matrix's shape");
+    tbsla_hip_matrix_s *GT = o.factor_t;
+    if (GT && (GT->format != TBSLA_HIP_CSR || GT->ln_row != n || GT->stored != G->stored))
+      return fail(TBSLA_HIP_ERR_INVALID, "factor_t must be the CSR transpose of the factor");
   }
   const bool mixed = o.precision == TBSLA_HIP_CG_MIXED;
   const int max_it = o.max_iterations;
@@ -1654,9 +1739,10 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   for (auto *v : parts) TBSLA_TRY(v->ensure((size_t)nblocks));
   TBSLA_TRY(c.seg.ensure(nsys));
   TBSLA_TRY(c.com.ensure(1));
-
+  TBSLA_TRY(c.args.ensure(1));
   TBSLA_TRY(c.block_seg.ensure(nblocks));
   if (!c.h_com) TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_com, 2 * sizeof(CgCom), hipHostMallocDefault));
+  if (!c.h_args) TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_args, sizeof(CgArgs), hipHostMallocDefault));
   for (auto &e : c.ev)
     if (!e) TBSLA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (pre == TBSLA_HIP_PRECOND_JACOBI) TBSLA_TRY(c.dinv.ensure((size_t)n));
@@ -1680,72 +1766,51 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
       c.a_v32_ver = m->values_version;
     }
   }
+  // M = G G^T: from G's diagonals when its entries lie on at most 8 of
+  // offset <= 0 (a stencil factor; no column indices read), else from the
+  // CSR rows of G and G^T
+  bool gdia = false;
+  int gndiag = 0;
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
     if (mixed) TBSLA_TRY(c.t32.ensure((size_t)n));
     else TBSLA_TRY(c.t64.ensure((size_t)n));
-    tbsla_hip_matrix_s *GT = o.factor_t;
-    if (GT && (GT->format != TBSLA_HIP_CSR || GT->ln_row != n || GT->stored != G->stored))
-      return fail(TBSLA_HIP_ERR_INVALID, "factor_t must be the CSR transpose of the factor");
-    // the transpose and the FP32 copies are rebuilt on every call (G may
-    // have been rewritten in place since the last solve); their buffers are
-    // kept while large enough
-    if (!GT) {
-      hipFree(c.t_rp);
-      hipFree(c.t_ci);
-      hipFree(c.t_v64);
-      c.t_rp = c.t_ci = nullptr;
-      c.t_v64 = nullptr;
-      TBSLA_TRY(csr_transpose_arrays(G, &c.t_rp, &c.t_ci, &c.t_v64, s));
+    if (o.factor_form != TBSLA_HIP_FACTOR_ROWS) {
+      TBSLA_TRY(build_gdia(c, G, mixed, s, &gndiag));
+      gdia = gndiag > 0;
+      if (!gdia && o.factor_form == TBSLA_HIP_FACTOR_DIAGONALS)
+        return fail(TBSLA_HIP_ERR_UNSUPPORTED,
+                    "factor G is not held on at most 8 diagonals of offset <= 0");
     }
-    if (mixed) {
-      if (c.f32_len < (size_t)std::max(G->stored, 1LL)) {
-        hipFree(c.g_v32);
-        hipFree(c.t_v32);
-        c.g_v32 = c.t_v32 = nullptr;
-        c.f32_len = 0;
-        TBSLA_TRY(device_alloc((void **)&c.g_v32, (size_t)std::max(G->stored, 1LL) * 4));
-        TBSLA_TRY(device_alloc((void **)&c.t_v32, (size_t)std::max(G->stored, 1LL) * 4));
-        c.f32_len = (size_t)std::max(G->stored, 1LL);
-      }
-      if (G->stored > 0) {
-        k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(G->values, G->stored, c.g_v32);
-        k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(GT ? GT->values : c.t_v64, G->stored,
-                                                     c.t_v32);
-      }
-    }
-    c.factor = G;
-  }
-  // the product form: A G, when every row of A has at most 64 products
-  // (else the separate form)
-  bool product = false;
-  if (pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_PRODUCT) {
-    int *rp = nullptr, *ci = nullptr;
-    double *v = nullptr;
-    long long nnz = 0;
-    bool fits = false;
-    TBSLA_TRY(csr_product_arrays(m, G, &rp, &ci, &v, &nnz, &fits, s));
-    if (fits && nnz > 0) {
-      product = true;
-      if (c.ag_rp && c.ag_rows >= n && c.ag_cap >= nnz) {
-        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_rp, rp, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, s));
-        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_ci, ci, (size_t)nnz * 4, hipMemcpyDeviceToDevice, s));
-        TBSLA_HIP_CHECK(hipMemcpyAsync(c.ag_v64, v, (size_t)nnz * 8, hipMemcpyDeviceToDevice, s));
+    if (!gdia) {
+      // the transpose and the FP32 copies are rebuilt on every call (G may
+      // have been rewritten in place since the last solve)
+      tbsla_hip_matrix_s *GT = o.factor_t;
+      if (!GT) {
         TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-        hipFree(rp);
-        hipFree(ci);
-        hipFree(v);
-      } else {
-        c.drop_product();
-        c.ag_rp = rp;
-        c.ag_ci = ci;
-        c.ag_v64 = v;
-        c.ag_rows = n;
-        c.ag_cap = nnz;
+        hipFree(c.t_rp);
+        hipFree(c.t_ci);
+        hipFree(c.t_v64);
+        c.t_rp = c.t_ci = nullptr;
+        c.t_v64 = nullptr;
+        TBSLA_TRY(csr_transpose_arrays(G, &c.t_rp, &c.t_ci, &c.t_v64, s));
       }
-      c.ag_nnz = nnz;
       if (mixed) {
-        if (!c.ag_v32) TBSLA_TRY(device_alloc((void **)&c.ag_v32, (size_t)c.ag_cap * 4));
-        k_to_f32<<<sgrid(nnz), kBlock, 0, s>>>(c.ag_v64, nnz, c.ag_v32);
+        if (c.f32_len < (size_t)std::max(G->stored, 1LL)) {
+          TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+          hipFree(c.g_v32);
+          hipFree(c.t_v32);
+          c.g_v32 = c.t_v32 = nullptr;
+          c.f32_len = 0;
+          TBSLA_TRY(device_alloc((void **)&c.g_v32, (size_t)std::max(G->stored, 1LL) * 4));
+          TBSLA_TRY(device_alloc((void **)&c.t_v32, (size_t)std::max(G->stored, 1LL) * 4));
+          c.f32_len = (size_t)std::max(G->stored, 1LL);
+        }
+        if (G->stored > 0) {
+          k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(G->values, G->stored, c.g_v32);
+          k_to_f32<<<sgrid(G->stored), kBlock, 0, s>>>(GT ? GT->values : c.t_v64, G->stored,
+                                                       c.t_v32);
+          TBSLA_HIP_LAUNCH_CHECK();
+        }
       }
     }
   }
@@ -1757,14 +1822,21 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.val32 = mixed ? c.a_v32 : nullptr;
   a.f_col = m->f_col;
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
-    a.g_rp = G->rowptr;
-    a.g_ci = G->colidx;
-    a.g_v64 = G->values;
-    a.g_v32 = c.g_v32;
-    a.t_rp = o.factor_t ? o.factor_t->rowptr : c.t_rp;
-    a.t_ci = o.factor_t ? o.factor_t->colidx : c.t_ci;
-    a.t_v64 = o.factor_t ? o.factor_t->values : c.t_v64;
-    a.t_v32 = c.t_v32;
+    if (gdia) {
+      a.gdoff = c.gdoff.p;
+      a.gd64 = c.gd64.p;
+      a.gd32 = c.gd32.p;
+      a.gndiag = gndiag;
+    } else {
+      a.g_rp = G->rowptr;
+      a.g_ci = G->colidx;
+      a.g_v64 = G->values;
+      a.g_v32 = c.g_v32;
+      a.t_rp = o.factor_t ? o.factor_t->rowptr : c.t_rp;
+      a.t_ci = o.factor_t ? o.factor_t->colidx : c.t_ci;
+      a.t_v64 = o.factor_t ? o.factor_t->values : c.t_v64;
+      a.t_v32 = c.t_v32;
+    }
   }
   a.dinv = c.dinv.p;
   a.b = c.b.p;
@@ -1775,6 +1847,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.p[1] = c.p1.p;
   a.z = c.z.p;
   a.ap = c.ap.p;
+  a.ax = c.ax.p;
   a.t64 = c.t64.p;
   a.t32 = c.t32.p;
   a.ppap = c.ppap.p;
@@ -1785,29 +1858,20 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   a.seg = c.seg.p;
   a.com = c.com.p;
   a.block_seg = c.block_seg.p;
-
-  if (product) {
-    a.ag_rp = c.ag_rp;
-    a.ag_ci = c.ag_ci;
-    a.ag_v64 = c.ag_v64;
-    a.ag_v32 = c.ag_v32;
-    a.ag_nnz = (int)c.ag_nnz;
-  }
   a.verify = (mixed || o.replace_every > 0) ? 1 : 0;
   a.stall_window = o.stall_window;
+  a.ncol = (int)m->ln_col;
   // A in the diagonal layout with no remote entries: K1 reads its diagonals
   const int *d_off = nullptr;
   const double *d_v = nullptr;
   int ndiag = 0;
   long long dv_len = 0;
   unsigned long long build = 0;
-  const bool dia =
-      !product && split_diag_only(m, &d_off, &d_v, &ndiag, &dv_len, &build) && m->ln_col > 0;
+  const bool dia = split_diag_only(m, &d_off, &d_v, &ndiag, &dv_len, &build) && m->ln_col > 0;
   if (dia) {
     a.doff = d_off;
     a.dv64 = d_v;
     a.ndiag = ndiag;
-    a.ncol = (int)m->ln_col;
     if (mixed) {
       if (c.dv32_len < (size_t)dv_len) {
         TBSLA_HIP_CHECK(hipStreamSynchronize(s));
@@ -1834,15 +1898,13 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     const double zero = 0.0;
     TBSLA_HIP_CHECK(hipMemcpy(c.dummy_d.p, &zero, sizeof zero, hipMemcpyHostToDevice));
   }
-  const int fcol = m->f_col;
-  TBSLA_HIP_CHECK(hipMemcpyAsync(c.dummy_i.p, &fcol, sizeof fcol, hipMemcpyHostToDevice, s));
   a.a_nnz = (int)std::max(m->stored, 1LL);
   if (m->stored == 0) {
     a.colidx = c.dummy_i.p;
     a.val64 = c.dummy_d.p;
     a.val32 = (const float *)c.dummy_d.p;
   }
-  if (pre == TBSLA_HIP_PRECOND_FACTORED) {
+  if (pre == TBSLA_HIP_PRECOND_FACTORED && !gdia) {
     a.g_nnz = (int)std::max(G->stored, 1LL);
     if (G->stored == 0) {
       a.g_ci = c.dummy_i.p;
@@ -1861,27 +1923,23 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   p.replace = std::max(0, o.replace_every);
   p.verify = a.verify;
   p.stall = o.stall_window;
-  // the fused form saves a kernel per iteration at the price of gathering
-  // each G^T row once per G entry, the product form reads A G's longer rows;
-  // on MI355X both cost more than the kernel they save (7-point Poisson,
-  // 64^3: 38 and 52 vs 31-37 us per iteration; 128^3: 286 and 410 vs 200-216
-  // us), so the automatic choice is the separate form
-  p.fused = pre == TBSLA_HIP_PRECOND_FACTORED && o.factor_form == TBSLA_HIP_FACTOR_FUSED;
-  p.product = product;
   p.dia = dia;
-  p.Lk1 = L;
-  if (product && o.lanes_per_row <= 0) {
-    // the same rule on A G's mean row length (at least L: the workgroup
-    // layout's passes of kBlock / L rows then hold whole passes of K1)
-    const double mean = (double)c.ag_nnz / std::max<long long>(n, 1);
-    while (p.Lk1 < 64 && p.Lk1 < mean / 8) p.Lk1 <<= 1;
-  }
+  p.gdia = gdia;
+  p.gkd = gndiag <= 4 ? 4 : 8;
   p.n = n;
   p.nblocks = nblocks;
+  p.args = c.args.p;
   p.s = s;
 
-  // upload layout, inputs, start
+  // upload the argument block (from its pinned copy: the stream's earlier
+  // work has finished with the previous contents, see the end of the solve),
+  // layout, inputs, start
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  *c.h_args = a;
+  const int fcol = m->f_col;
   CgCom com0{};
+  TBSLA_HIP_CHECK(hipMemcpyAsync(c.args.p, c.h_args, sizeof(CgArgs), hipMemcpyHostToDevice, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(c.dummy_i.p, &fcol, sizeof fcol, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.seg.p, hs.data(), sizeof(CgSeg) * nsys, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.block_seg.p, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(c.com.p, &com0, sizeof com0, hipMemcpyHostToDevice, s));
@@ -1893,15 +1951,11 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   if (pre == TBSLA_HIP_PRECOND_JACOBI)
     k_csr_dinv<<<sgrid(n), kBlock, 0, s>>>(n, m->f_row, m->f_col, m->rowptr, m->colidx, m->values,
                                           c.dinv.p);
-  const double *ax0 = nullptr;
-  if (o.use_initial_guess) {
-    TBSLA_TRY(spmv_launch(m, c.ax.p, c.x.p, 0, true, s));
-    ax0 = c.ax.p;
-  }
-  launch_residual(a, p, ax0, 0, kStart, o.use_initial_guess ? 0 : 1);
+  if (o.use_initial_guess) TBSLA_TRY(spmv_launch(m, c.ax.p, c.x.p, 0, true, s));
+  launch_residual(p, o.use_initial_guess ? 1 : 0, 0, kStart, o.use_initial_guess ? 0 : 1);
   if (pre == TBSLA_HIP_PRECOND_FACTORED) {
-    launch_gtr(a, p, 0, kStart);
-    if (!p.product) launch_gz(a, p, 0, kStart);
+    launch_gtr(p, 0, kStart);
+    launch_gz(p, 0, kStart);
   }
   // The reference clamps both divisors at an absolute 1e-10 (CG.cpp:35,63,69),
   // which stalls the iteration once r.r drops below it; it is kept for the
@@ -1910,7 +1964,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   const bool reference_setting = !o.relative && !mixed && pre == TBSLA_HIP_PRECOND_NONE;
   const double near_zero = o.near_zero > 0 ? o.near_zero
                            : (o.near_zero < 0 ? 1e-300 : (reference_setting ? kNearZero : 1e-300));
-  k_cg_start<<<nblocks, kBlock, 0, s>>>(a, o.tolerance, o.relative, near_zero);
+  k_cg_start<<<nblocks, kBlock, 0, s>>>(p.args, o.tolerance, o.relative, near_zero);
   TBSLA_HIP_LAUNCH_CHECK();
 
   // Graph batches of B iterations starting at 1 + tB (B even, so every
@@ -1929,96 +1983,64 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   }
   int it = 0;  // next global iteration to enqueue
   if (max_it > 0) {
-    TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, 0, false));
+    enqueue_iteration(p, 0, false);
     it = 1;
   }
-  GraphKey key;
-  std::memset(&key, 0, sizeof key);
-  key.args = a;
-  key.spmv_rowptr = m->rowptr;
-  key.spmv_colidx = m->colidx;
-  key.spmv_values = m->values;
-  key.spmv_split = m->split;
+  GraphKey key{};
   key.L = p.L;
-  key.Lk1 = p.Lk1;
   key.pre = pre;
   key.mixed = mixed;
   key.replace = p.replace;
   key.batch = B;
   key.nblocks = nblocks;
-  key.fused = p.fused ? 1 : 0;
-  key.product = p.product ? 1 : 0;
+  key.verify = p.verify;
   key.dia = p.dia ? 1 : 0;
-  key.spmv_lanes = pick_csr_lanes(m);
-  key.spmv_rows = pick_csr_rows(m, key.spmv_lanes);
-  key.spmv_vec = pick_csr_vec(m);
-  key.spmv_nt = pick_csr_nt(m);
-  // the graph set of this plan: an exact match is replayed as it is; one of
-  // the same shape (other pointers) is updated in place, which costs a
-  // fraction of an instantiation; else a new set is instantiated
+  key.gdia = p.gdia ? p.gkd : 0;
+  // the graph set of this plan, captured and instantiated once; the least
+  // recently used set is destroyed when kGraphSets are held (the stream is
+  // idle between solves, so no set is in flight then)
   CgCache::GraphSet *gs = nullptr;
   if (B > 0) {
-    auto shape_of = [](const GraphKey &k) {
-      return std::make_tuple(k.L, k.Lk1, k.pre, k.mixed, k.replace, k.batch, k.nblocks, k.fused,
-                             k.product, k.args.verify, k.dia);
-    };
     size_t pos = c.graphs.size();
-    for (size_t q = 0; q < c.graphs.size() && pos == c.graphs.size(); q++)
-      if (c.graphs[q].key == key) pos = q;
-    for (size_t q = 0; q < c.graphs.size() && pos == c.graphs.size(); q++)
-      if (shape_of(c.graphs[q].key) == shape_of(key)) pos = q;
+    for (size_t q = 0; q < c.graphs.size(); q++)
+      if (c.graphs[q].key == key) {
+        pos = q;
+        break;
+      }
     if (pos == c.graphs.size()) {
+      const int variants = p.replace > 0 ? 2 : 1;
+      CgCache::GraphSet fresh;
+      fresh.key = key;
+      for (int v = 0; v < variants; v++) {
+        // variant v: iterations 1..B; with v == 1 the last one replaces (its
+        // index is taken as a multiple of replace_every of the same parity)
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          for (int k = 0; k < B; k++) {
+            const bool last = k == B - 1;
+            enqueue_iteration(p, (v == 1 && last) ? B * p.replace : 1 + k, last);
+          }
+          e = hipStreamEndCapture(s, &graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&fresh.exec[v], graph, nullptr, nullptr, 0);
+        if (graph) hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          fresh.destroy();  // no partial set is ever kept
+          return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+      }
       if (c.graphs.size() == CgCache::kGraphSets) {
         c.graphs.back().destroy();
         c.graphs.pop_back();
       }
-      c.graphs.insert(c.graphs.begin(), CgCache::GraphSet{});
+      c.graphs.insert(c.graphs.begin(), fresh);
     } else if (pos != 0) {
       CgCache::GraphSet moved = c.graphs[pos];
       c.graphs.erase(c.graphs.begin() + (long)pos);
       c.graphs.insert(c.graphs.begin(), moved);
     }
     gs = &c.graphs.front();
-  }
-  if (B > 0 && !(gs->exec[0] && gs->key == key)) {
-    const int variants = p.replace > 0 ? 2 : 1;
-    for (int v = 0; v < variants; v++) {
-      // variant v: iterations 1..B; with v == 1 the last one replaces (its
-      // index is taken as a multiple of replace_every of the same parity)
-      hipGraph_t graph = nullptr;
-      TBSLA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int st = TBSLA_HIP_SUCCESS;
-      for (int k = 0; k < B && st == TBSLA_HIP_SUCCESS; k++) {
-        const bool last = k == B - 1;
-        const int i = (v == 1 && last) ? B * p.replace : 1 + k;
-        st = enqueue_iteration(m, a, p, c.ax.p, i, last);
-      }
-      hipError_t e = hipStreamEndCapture(s, &graph);
-      if (st) {
-        if (graph) hipGraphDestroy(graph);
-        return st;
-      }
-      if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-      if (gs->exec[v]) {
-        hipGraphNode_t err_node = nullptr;
-        hipGraphExecUpdateResult ur;
-        if (hipGraphExecUpdate(gs->exec[v], graph, &err_node, &ur) == hipSuccess &&
-            ur == hipGraphExecUpdateSuccess) {
-          hipGraphDestroy(graph);
-          continue;
-        }
-        (void)hipGetLastError();  // the failed update's status
-        hipGraphExecDestroy(gs->exec[v]);
-        gs->exec[v] = nullptr;
-      }
-      e = hipGraphInstantiate(&gs->exec[v], graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      if (e != hipSuccess) {
-        gs->exec[v] = nullptr;
-        return fail(TBSLA_HIP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-      }
-    }
-    gs->key = key;
   }
   const int poll_every = B > 0 ? B : 16;
 
@@ -2035,7 +2057,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
       it += B;
     } else {
       const int stop = std::min(max_it, it + poll_every);
-      for (; it < stop; it++) TBSLA_TRY(enqueue_iteration(m, a, p, c.ax.p, it, it == stop - 1));
+      for (; it < stop; it++) enqueue_iteration(p, it, it == stop - 1);
     }
     TBSLA_HIP_LAUNCH_CHECK();
     const int slot = batches & 1;
@@ -2046,8 +2068,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
       TBSLA_HIP_CHECK(hipEventSynchronize(c.ev[prev]));
       const CgCom h = c.h_com[prev];
       if (h.npending > 0 && p.verify) {
-        TBSLA_TRY(true_residual(m, a, p, c.ax.p, 0, kVerify));
-        k_cg_vcheck<<<nblocks, kBlock, 0, s>>>(a, it);
+        true_residual(p, 0, kVerify);
+        k_cg_vcheck<<<nblocks, kBlock, 0, s>>>(p.args, it);
         TBSLA_HIP_LAUNCH_CHECK();
       }
       if (h.ndone >= nactive) finished = true;
@@ -2060,10 +2082,10 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   if (h.ndone < nactive) {
     // the final test after the last iteration (CG.cpp:66-72), then the
     // true residual of systems whose recurrence converged
-    if (max_it > 0) launch_k1_any(a, p, it & 1, 1, 1);
+    if (max_it > 0) launch_k1_any(p, it & 1, 1, 1);
     if (p.verify) {
-      TBSLA_TRY(true_residual(m, a, p, c.ax.p, 0, kVerify));
-      k_cg_vcheck<<<nblocks, kBlock, 0, s>>>(a, it);
+      true_residual(p, 0, kVerify);
+      k_cg_vcheck<<<nblocks, kBlock, 0, s>>>(p.args, it);
     }
     TBSLA_HIP_LAUNCH_CHECK();
   }

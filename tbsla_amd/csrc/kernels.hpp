@@ -37,13 +37,6 @@ int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int ke
 // host, receives the sum; synchronises the stream)
 int counts_to_rowptr(const int *cnt, int *rowptr, long long n, long long *total, hipStream_t s);
 
-// spgemm.hip: C = A B (CSR blocks, B's rows = A's local columns) into new
-// arrays with local column indices, when every row of A has at most 64
-// products (*fits); otherwise nothing is allocated and *fits is false
-int csr_product_arrays(const tbsla_hip_matrix_s *A, const tbsla_hip_matrix_s *B, int **rowptr,
-                       int **colidx, double **values, long long *nnz, bool *fits,
-                       hipStream_t s);
-
 // problems.hip: CSR transpose into new arrays (rows = src's local columns)
 int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **colidx_t,
                          double **values_t, hipStream_t s);

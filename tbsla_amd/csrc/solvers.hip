@@ -472,6 +472,20 @@ int values_changed(tbsla_hip_matrix_s *m, hipStream_t s) {
 
 using namespace tbsla_hip;
 
+extern "C" int tbsla_hip_values_changed(tbsla_hip_matrix_t m, void *stream) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null handle");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  // the COO/SCOO row pointer is derived from the row array: rebuilt (and the
+  // triplets' order checked again) at the next SpMV
+  if (m->coo_rowptr) {
+    hipFree(m->coo_rowptr);
+    m->coo_rowptr = nullptr;
+  }
+  if (m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO) m->row_sorted = 1;
+  return values_changed(m, s);
+}
+
 extern "C" int tbsla_hip_get_col_sums(tbsla_hip_matrix_t m, double *d_s, void *stream) {
   if (!m || !d_s) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
   hipStream_t s = static_cast<hipStream_t>(stream);

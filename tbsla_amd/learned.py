@@ -95,36 +95,50 @@ class Solver:
         self.n = A.geometry().ln_row
         self.x = T.DeviceBuffer(self.n * 8)
         self.G, self.GT = factors if factors is not None else (T.Matrix(T.CSR), T.Matrix(T.CSR))
-        self.factor_form = T.FACTOR_AUTO
 
     def run(self, method: str, tol: float = 1e-8, max_iterations: int = 20000) -> dict:
+        if method.startswith("learned"):
+            return self.run_factored(method, None, None, tol, max_iterations)
+        mixed = method.endswith("mixed")
+        prec = T.CG_MIXED if mixed else T.CG_FP64
+        pre = T.PRECOND_JACOBI if method.startswith("jacobi") else T.PRECOND_NONE
+        t0 = time.perf_counter()
+        o = self.A.cg_options(max_iterations, tol, True, prec, pre, 50 if mixed else 0)
+        it, _ = self.A.cg_device(self.rhs, self.x, o)
+        return self._result(method, t0, it, False)
+
+    def run_factored(self, method: str, G: "T.Matrix | None", GT: "T.Matrix | None",
+                     tol: float = 1e-8, max_iterations: int = 20000) -> dict:
+        """M = G G^T PCG; G = None: the network's factor of A (inference
+        timed with the solve).  Jacobi-PCG takes over when the factored
+        iteration breaks down (r.z <= 0 or not finite, or no progress in 100
+        iterations); the true FP64 residual judges the result."""
         A = self.A
         mixed = method.endswith("mixed")
         prec = T.CG_MIXED if mixed else T.CG_FP64
         rep = 50 if mixed else 0
         fallback = False
         t0 = time.perf_counter()
-        if method.startswith("learned"):
+        if G is None:
             T.check(T.lib().tbsla_hip_precond_net_apply(self.net.h, A.h, self.G.h, self.GT.h, None))
-            o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_FACTORED, rep,
-                             factor=self.G, factor_t=self.GT, stall_window=100,
-                             factor_form=self.factor_form)
-        elif method.startswith("jacobi"):
-            o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_JACOBI, rep)
-        else:
-            o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_NONE, rep)
+            G, GT = self.G, self.GT
+        o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_FACTORED, rep,
+                         factor=G, factor_t=GT, stall_window=100)
         try:
-            it, rn = A.cg_device(self.rhs, self.x, o)
+            it, _ = A.cg_device(self.rhs, self.x, o)
         except T.TbslaHipError as e:
-            if e.status != T.ERR_BREAKDOWN or not method.startswith("learned"):
+            if e.status != T.ERR_BREAKDOWN:
                 raise
             fallback = True
             o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_JACOBI, rep)
-            it, rn = A.cg_device(self.rhs, self.x, o)
+            it, _ = A.cg_device(self.rhs, self.x, o)
+        return self._result(method, t0, it, fallback)
+
+    def _result(self, method: str, t0: float, it: int, fallback: bool) -> dict:
         ms = (time.perf_counter() - t0) * 1e3
         # accepted on the true FP64 residual
         y = T.DeviceBuffer(self.n * 8)
-        T.check(T.lib().tbsla_hip_spmv(A.h, y.ptr, self.x.ptr, 0, None))
+        T.check(T.lib().tbsla_hip_spmv(self.A.h, y.ptr, self.x.ptr, 0, None))
         b = self.rhs.to_numpy(count=self.n)
         r = b - y.to_numpy(count=self.n)
         return {"method": method, "ms": ms, "iterations": it,

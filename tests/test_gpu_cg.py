@@ -84,30 +84,6 @@ def test_factored_early_iterates_and_given_transpose(sym):
         assert np.array_equal(xg, xt)
 
 
-@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
-def test_factored_fused_and_separate_identical(sym, precision):
-    """z = G (G^T r) in the update kernel gives bitwise the iterates of the
-    separate G^T and G kernels (with residual replacement every 7 in the mixed
-    form, whose steps take the separate path), and both match the oracle."""
-    o, g = sym
-    n = o.geom["ln_row"]
-    Go = lower_factor(o)
-    G = to_gpu(Go)
-    rhs = 2 * O.unit_vector(15, n) - 1
-    rep = 7 if precision == T.CG_MIXED else 0
-    runs = [g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
-                 preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=form)
-            for form in (T.FACTOR_SEPARATE, T.FACTOR_FUSED, T.FACTOR_AUTO)]
-    for x, it, rn in runs[1:]:
-        assert it == runs[0][1] and rn == runs[0][2]
-        assert np.array_equal(x, runs[0][0])
-    xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=2, G=Go)
-    assert abs(runs[0][1] - ito) <= 2
-    assert rel_max(runs[0][0], xo) < 1e-7
-    with pytest.raises(T.TbslaHipError):
-        g.cg(rhs, preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=4)
-
-
 @pytest.fixture(scope="module")
 def poisson():
     k = O.lognormal_field(16, 16, 16, 0.0, 1.0, 4.0, 7)
@@ -116,13 +92,14 @@ def poisson():
 
 
 @pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
-def test_factored_product_form(poisson, sym, precision):
-    """The product form (A G built per call; Ap' = (A G) t + beta Ap, r.z =
-    t.t) follows the oracle's G G^T PCG: FP64 early iterates within 1e-10,
-    convergence within 3 iterations of it, true residual below the tolerance
-    (mixed: with replacements every 7, which restart Ap from A p).  A matrix
-    whose rows have more than 64 products falls back to the separate form,
-    bitwise."""
+def test_factored_diagonals_and_rows_identical(poisson, sym, precision):
+    """A stencil's lower factor (4 diagonals of offset <= 0) is applied from
+    its diagonals by default; that form sums G^T's and G's rows in the order
+    of the CSR form, so both give bitwise the same iterates (with residual
+    replacement every 7 in the mixed form), and they follow the oracle's
+    G G^T PCG (FP64 early iterates within 1e-12).  A factor off a few
+    diagonals (the symmetrised cqmat's) runs the CSR form under AUTO and is
+    refused under DIAGONALS."""
     o, g = poisson
     n = o.geom["ln_row"]
     Go = lower_factor(o)
@@ -133,36 +110,41 @@ def test_factored_product_form(poisson, sym, precision):
             xo, ito, _ = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30, kind=2, G=Go)
             xg, itg, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
                               preconditioner=T.PRECOND_FACTORED, factor=G,
-                              factor_form=T.FACTOR_PRODUCT)
+                              factor_form=T.FACTOR_DIAGONALS)
             assert itg == ito == k
-            assert rel_max(xg, xo) < 1e-10, (k, rel_max(xg, xo))
+            assert rel_max(xg, xo) < 1e-12, (k, rel_max(xg, xo))
     rep = 7 if precision == T.CG_MIXED else 0
-    x, it, rn = g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
-                     preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=T.FACTOR_PRODUCT)
+    runs = [g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+                 preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=form)
+            for form in (T.FACTOR_ROWS, T.FACTOR_DIAGONALS, T.FACTOR_AUTO)]
+    for x, it, rn in runs[1:]:
+        assert it == runs[0][1] and rn == runs[0][2]
+        assert np.array_equal(x, runs[0][0])
     xo, ito, _ = O.pcg(o, rhs, tolerance=1e-10, kind=2, G=Go)
-    assert abs(it - ito) <= 3, (it, ito)
+    assert abs(runs[0][1] - ito) <= 2
     A = scipy_of(o)
-    assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
-    x2, it2, _ = g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
-                      preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=T.FACTOR_PRODUCT)
-    assert it2 == it and np.array_equal(x2, x)
-    # rows of the symmetrised cqmat have ~25 x 13 products: the separate form runs
+    assert np.linalg.norm(rhs - A @ runs[0][0]) < 1e-10 * np.linalg.norm(rhs) * 1.0001
     so, sg = sym
     Gs = to_gpu(lower_factor(so))
     rs = 2 * O.unit_vector(20, so.geom["ln_row"]) - 1
     a = sg.cg(rs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
-              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_PRODUCT)
+              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_AUTO)
     b = sg.cg(rs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
-              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_SEPARATE)
+              preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_ROWS)
     assert a[1] == b[1] and np.array_equal(a[0], b[0])
+    with pytest.raises(T.TbslaHipError) as e:
+        sg.cg(rs, preconditioner=T.PRECOND_FACTORED, factor=Gs, factor_form=T.FACTOR_DIAGONALS)
+    assert e.value.status == T.ERR_UNSUPPORTED
+    with pytest.raises(T.TbslaHipError):
+        g.cg(rhs, preconditioner=T.PRECOND_FACTORED, factor=G, factor_form=3)
 
 
 @pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
 def test_cache_of_destroyed_handle_reused(precision):
     """A destroyed handle's CG state (stream, vectors, captured graphs) serves
-    the next handles: graphs of the same shape are updated in place with the
-    new matrix's pointers, others re-instantiated.  Every solve still follows
-    the oracle."""
+    the next handles: a graph of the same launch shape is replayed as it is
+    (its kernels read the new matrix's pointers from the argument block),
+    others are instantiated.  Every solve still follows the oracle."""
     import gc
     rep = 10 if precision == T.CG_MIXED else 0
     for side, seed in ((12, 1), (12, 2), (14, 3), (12, 4)):
@@ -320,6 +302,41 @@ def test_mixed_sees_values_changed_in_place():
     B = scipy_of(h)
     x3, _, _ = h.cg(rhs, **opts)
     assert np.linalg.norm(rhs - B @ x3) < 1e-10 * np.linalg.norm(rhs) * 1.0001
+
+
+@pytest.mark.parametrize("prec", [T.CG_FP64, T.CG_MIXED])
+def test_values_written_through_device_arrays(prec):
+    """Values rewritten in place through tbsla_hip_device_arrays and announced
+    with tbsla_hip_values_changed reach every derived copy: CG on the diagonal
+    layout (its own copy of the values, and their FP32 copy in the mixed
+    form) then solves the new matrix exactly as CG on a fresh handle imported
+    with the new values in the same layout, bit for bit."""
+    dims = (16, 14, 12)
+    g = T.Matrix(T.CSR).fill_poisson3d(*dims, kappa=T.lognormal_field(*dims, seed=11))
+    n = g.geometry().ln_row
+    rhs = T.uniform_vector(n, 12, -1.0, 1.0).to_numpy(count=n)
+    opts = dict(tolerance=1e-10, relative=True, preconditioner=T.PRECOND_JACOBI,
+                precision=prec, replace_every=20 if prec == T.CG_MIXED else 0)
+    g.set_csr_layout(T.CSR_DIAG, 65536)
+    g.cg(rhs, **opts)  # FP32 copies of the old values made and cached
+    rp, ci, va = g.arrays()
+    # scale row i by s_i and column j by s_j: stays symmetric positive definite
+    s = 1.0 + O.unit_vector(13, n)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    new = va * s[rows] * s[ci]
+    _, _, dv = g.device_arrays()
+    T.check(T.lib().tbsla_hip_memcpy_htod(dv, new.ctypes.data, new.nbytes, None))
+    g.values_changed()
+    geo = g.geometry()
+    geom = {f: getattr(geo, f) for f in T.GEOM_FIELDS}
+    geom["nnz"] = geo.nnz
+    ref = T.Matrix.from_arrays(T.CSR, geom, rp, ci, new)
+    ref.set_csr_layout(T.CSR_DIAG, 65536)
+    xg, itg, rg = g.cg(rhs, **opts)
+    xr, itr, rr = ref.cg(rhs, **opts)
+    assert itg == itr and rg == rr and np.array_equal(xg, xr)
+    A = scipy_of(ref)
+    assert np.linalg.norm(rhs - A @ xg) < 1e-10 * np.linalg.norm(rhs) * 1.0001
 
 
 @pytest.mark.parametrize("pre,prec", [(T.PRECOND_NONE, T.CG_FP64), (T.PRECOND_JACOBI, T.CG_FP64),
