@@ -280,53 +280,11 @@ class Matrix {
   // pr pc NR NC (int32), ELL max_col (int32); then each array preceded by
   // its size_t length.
   std::ostream &write(std::ostream &os) const {
-    const tbsla_hip_geometry g = geometry();
-    const int hdr1[6] = {g.n_row, g.n_col, g.ln_row, g.ln_col, g.f_row, g.f_col};
-    const long int nnz = (long int)g.nnz;
-    const int hdr2[4] = {g.pr, g.pc, g.NR, g.NC};
-    os.write(reinterpret_cast<const char *>(hdr1), sizeof(hdr1));
-    os.write(reinterpret_cast<const char *>(&nnz), sizeof(nnz));
-    os.write(reinterpret_cast<const char *>(hdr2), sizeof(hdr2));
-    std::vector<int> a, b;
-    std::vector<double> v;
-    exported(g, a, b, v);
-    auto arr = [&os](const void *p, size_t n, size_t elem) {
-      os.write(reinterpret_cast<const char *>(&n), sizeof(n));
-      if (n) os.write(reinterpret_cast<const char *>(p), (std::streamsize)(n * elem));
-    };
-    switch (g.format) {
-      case TBSLA_HIP_CSR: {
-        // values and colidx are written with the nnz field's length, as
-        // the reference does; entries beyond rowptr[ln_row] are zero here
-        v.resize((size_t)std::max<long long>(g.nnz, (long long)v.size()), 0.0);
-        b.resize(v.size(), 0);
-        arr(v.data(), (size_t)g.nnz, sizeof(double));
-        arr(a.data(), (size_t)g.ln_row + 1, sizeof(int));
-        arr(b.data(), (size_t)g.nnz, sizeof(int));
-        break;
-      }
-      case TBSLA_HIP_COO:
-      case TBSLA_HIP_SCOO:
-        arr(v.data(), (size_t)g.nnz, sizeof(double));
-        arr(a.data(), (size_t)g.nnz, sizeof(int));
-        arr(b.data(), (size_t)g.nnz, sizeof(int));
-        break;
-      case TBSLA_HIP_ELL: {
-        os.write(reinterpret_cast<const char *>(&g.max_col), sizeof(int));
-        const size_t slots = (size_t)g.ln_row * (size_t)g.max_col;
-        v.resize(slots, 0.0);
-        a.resize(slots, 0);
-        arr(v.data(), slots, sizeof(double));
-        arr(a.data(), slots, sizeof(int));
-        break;
-      }
-      case TBSLA_HIP_DENSE: {
-        const size_t cells = (size_t)g.ln_row * (size_t)g.ln_col;
-        v.resize(cells, 0.0);
-        arr(v.data(), cells, sizeof(double));
-        break;
-      }
-    }
+    size_t len = 0;
+    check(tbsla_hip_write(h_, nullptr, 0, &len, nullptr));
+    std::vector<char> img(len);
+    check(tbsla_hip_write(h_, img.data(), img.size(), &len, nullptr));
+    os.write(img.data(), (std::streamsize)len);
     return os;
   }
 

@@ -10,7 +10,15 @@ library produced it (geometry, nnz field, arrays in the reference layout), an
 input vector x (iota, so every product is an exact integer) and the
 reference's ``spmv(x)``.  These are data -- inputs and expected outputs --
 and pin the oracle wherever oracle/_ref is not built.
+
+``write_<fmt>_<case>.bin`` holds the bytes the reference's ``Matrix::write``
+produced for that block (MatrixCSR.cpp:236-261 and the per-format twins),
+listed with their cases in ``write_index.json``.  The CSR case 0 block has
+an nnz field larger than its stored entries (fill_cqmat's count pass): the
+reference writes its uninitialised array tail there, so those bytes carry no
+expected value and the index records where they start.
 """
+import json
 import os
 import sys
 
@@ -31,6 +39,8 @@ CASES = [
     (400, 400, 10, 0.1, 0, 0, 0, 1, 1),
     (1000, 1000, 16, 0.05, 0, 1, 2, 4, 4),
 ]
+# (format, case) pairs whose Matrix::write images are kept
+WRITE_CASES = [("CSR", 6), ("CSR", 0), ("ELL", 6), ("COO", 6), ("SCOO", 6), ("DENSE", 3)]
 NAMES = {O.CSR: ("rowptr", "colidx", "values"), O.ELL: ("columns", "values"),
          O.COO: ("row", "col", "values"), O.SCOO: ("row", "col", "values"),
          O.DENSE: ("values",)}
@@ -61,6 +71,22 @@ def main():
             path = os.path.join(HERE, f"cqmat_{fmt_name.lower()}_{i}.npz")
             np.savez_compressed(path, **out)
             print(path)
+    index = []
+    for fmt_name, case in WRITE_CASES:
+        fmt = O.FORMATS[fmt_name]
+        nr, nc, c, q, s, pr, pc, NR, NC = CASES[case]
+        r = O.ref_fill_cqmat(fmt, nr, nc, c, q, s, pr, pc, NR, NC)
+        img = r.write()
+        name = f"write_{fmt_name.lower()}_{case}.bin"
+        with open(os.path.join(HERE, name), "wb") as f:
+            f.write(img)
+        g = r.geom
+        stored = int(r.arrays()[0][-1]) if fmt == O.CSR else None
+        index.append(dict(file=name, format=fmt_name, case=case, nnz=int(g["nnz"]),
+                          stored=stored, bytes=len(img)))
+        print(name, len(img))
+    with open(os.path.join(HERE, "write_index.json"), "w") as f:
+        json.dump(index, f, indent=1)
 
 
 if __name__ == "__main__":
