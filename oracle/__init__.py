@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
         _lib.orc_AAxpAxpx.argtypes = [C.c_int, C.c_void_p, dp, dp, dp, C.c_int]
         _lib.orc_result_len.argtypes = [C.c_int, C.c_void_p]
         _lib.orc_csr_Ax_omp.argtypes = [C.c_void_p, dp, dp]
+        _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_get_col_sums.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_normalize_cols.argtypes = [C.c_int, C.c_void_p, dp]
         _lib.orc_make_stochastic.argtypes = [C.c_int, C.c_void_p, dp]

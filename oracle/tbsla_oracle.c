@@ -924,6 +924,9 @@ static void csr_Ax(const orc_csr *m, double *r, const double *v) {
   }
 }
 
+/* Threads of the OpenMP loops below (the CPU baseline's thread count). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 /* The same loop under `omp parallel for schedule(static)`, the CPU baseline. */
 void orc_csr_Ax_omp(const orc_csr *m, double *r, const double *v) {
   const orc_geom *g = &m->g;
@@ -1060,7 +1063,10 @@ void orc_get_col_sums(int fmt, const void *m, double *s) {
 
 /* normalize_cols: CSR (MatrixCSR.cpp:1215-1232), ELL (MatrixELL.cpp:818-833),
  * SCOO (MatrixSCOO.cpp:726-738); division only where the sum is > 0 (and,
- * for ELL, the value is > 0). */
+ * for ELL, the value is > 0).  ELL tests the value before it reads s: the
+ * reference reads s[columns - f_col] first (:828), which for a padding slot
+ * (column 0) of a block with f_col > 0 lies before s; that value is never
+ * used, so the results are the same without the out-of-bounds read. */
 void orc_normalize_cols(int fmt, void *m, const double *s) {
   if (fmt == ORC_CSR) {
     orc_csr *a = (orc_csr *)m;
@@ -1076,8 +1082,9 @@ void orc_normalize_cols(int fmt, void *m, const double *s) {
     for (int i = 0; i < a->g.ln_row; i++)
       for (int j = 0; j < a->max_col; j++) {
         size_t k = (size_t)i * a->max_col + j;
+        if (!(a->values[k] > 0)) continue;
         double sv = s[a->columns[k] - a->g.f_col];
-        if (a->values[k] > 0 && sv > 0) a->values[k] /= sv;
+        if (sv > 0) a->values[k] /= sv;
       }
   } else if (fmt == ORC_SCOO) {
     orc_coo *a = (orc_coo *)m;

@@ -116,6 +116,7 @@ void orc_spmv(int fmt, const void *m, double *r, const double *v, int vect_incr)
 void orc_a_axpx_(int fmt, const void *m, double *r, const double *v, int vect_incr);
 void orc_AAxpAx(int fmt, const void *m, double *r, const double *v, double *buf, int vect_incr);
 void orc_AAxpAxpx(int fmt, const void *m, double *r, const double *v, double *buf, int vect_incr);
+void orc_set_threads(int n);
 void orc_csr_Ax_omp(const orc_csr *m, double *r, const double *v);
 
 /* ---- column normalisation, PageRank, CG, SpMM ---------------------------- */
