@@ -45,6 +45,10 @@ CPP_PROGRAMS = [
      "$ORIGIN/.."),
     (os.path.join(PKG, "apps", "tbsla_attention_hip.cpp"),
      os.path.join(PKG, "bin", "tbsla_attention_hip"), "$ORIGIN/.."),
+    (os.path.join(PKG, "apps", "tbsla_perf_page_rank_hip.cpp"),
+     os.path.join(PKG, "bin", "tbsla_perf_page_rank_hip"), "$ORIGIN/.."),
+    (os.path.join(PKG, "apps", "tbsla_perf_cg_hip.cpp"), os.path.join(PKG, "bin", "tbsla_perf_cg_hip"),
+     "$ORIGIN/.."),
     (os.path.join(PKG, "apps", "tbsla_mm_to_bin_hip.cpp"),
      os.path.join(PKG, "bin", "tbsla_mm_to_bin_hip"), "$ORIGIN/.."),
     (os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp"),

@@ -41,3 +41,62 @@ def test_perf_app_drivers():
     assert r.returncode == 0, r.stderr
     rec = json.loads(r.stdout.strip().splitlines()[-1])
     assert int(rec["nb_iterations"]) > 10
+
+
+# keys the reference MPI apps print (tbsla_perf_page_rank_mpi.cpp:448-494,
+# tbsla_perf_cg_mpi.cpp:134-176)
+MPI_KEYS = ("test", "matrix_format", "matrix_dim", "g_row", "g_col", "nnz", "nnz_min", "nnz_max",
+            "nb_iterations", "time_app_in", "time_op", "time_gen_mat", "gnnz", "processes", "lang",
+            "matrix_type", "compiler", "compile_options", "vectorization")
+
+
+def run_app(name, *args, timeout=180):
+    exe = os.path.join(ROOT, "tbsla_amd", "bin", name)
+    assert os.path.exists(exe), f"{exe} not built"
+    return subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_page_rank_app():
+    """The PageRank app: the reference's flags and keys, 10 runs (the total
+    iteration count is 10x one run's), median time and the reference's
+    GFLOP formula; against the library's PageRank of the same matrix."""
+    import numpy as np
+    import tbsla_amd as T
+    r = run_app("tbsla_perf_page_rank_hip", "--format", "CSR", "--matrix", "cqmat",
+                "--matrix_dim", 200000, "--C", 12, "--Q", 0.1, "--S", 0, "--epsilon", 1e-10)
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    for key in MPI_KEYS + ("gflops", "cqmat_c", "cqmat_q", "cqmat_s"):
+        assert key in rec, key
+    assert rec["test"] == "page_rank" and rec["matrix_dim"] == "200000"
+    g = T.Matrix(T.CSR).fill_cqmat(200000, 200000, 12, 0.1, 0)
+    g.make_stochastic()
+    _, it, _ = g.page_rank(0.85, 1e-10)
+    assert int(rec["nb_iterations"]) == 10 * it
+    n, nnz, t = 200000, int(rec["nnz"]), float(rec["time_op"])
+    assert np.isclose(float(rec["gflops"]), ((2 * nnz + 2 * n) * it + 2 * n) / (t * 1e9), rtol=1e-3)
+    # a grid of more than one block needs more than one process
+    r = run_app("tbsla_perf_page_rank_hip", "--format", "CSR", "--matrix", "cqmat", "--GR", 2)
+    assert r.returncode == 99
+
+
+def test_cg_app(tmp_path):
+    """The CG app: the reference's keys; on an SPD Poisson problem the
+    solve converges (absolute |r| < 1e-10 as CG.cpp:34); a binary image
+    written by the library is read back through --matrix <name>."""
+    import tbsla_amd as T
+    r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "poisson", "--side", 32)
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    for key in MPI_KEYS:
+        assert key in rec, key
+    assert rec["test"] == "conjugate_gradient" and rec["status"] == "ok"
+    assert float(rec["res_norm"]) < 1e-10 and int(rec["nb_iterations"]) > 10
+    g = T.Matrix(T.CSR).fill_sym_cqmat(50000, 8, 0.1, 1)
+    T.check(T.lib().tbsla_hip_write_file(g.h, str(tmp_path / "spd.CSR").encode(), None))
+    r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "spd", "--matrix_folder",
+                tmp_path, "--precision", "mixed", "--tolerance", "1e-8", "--relative")
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    assert rec["status"] == "ok" and rec["matrix_dim"] == "50000"
+    assert run_app("tbsla_perf_cg_hip", "--format", "ELL", "--matrix", "cqmat").returncode == 1
