@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SpMV GB/s (% HBM peak); CG time-to-1e-8, learned vs Jacobi vs plain"
 HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20  # MI355X Infinity Cache: a working set below it is not an HBM figure
 N, C, Q = 4194304, 32, 0.1
 
 
@@ -84,28 +85,70 @@ def algorithmic_bytes_csr(stored, n_rows, n_cols):
     return 12 * stored + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
 
 
+def host_cpu():
+    """(model name, logical CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_threads(requested=None):
+    """Threads for the CPU baseline: the CPUs of this process's affinity,
+    capped by OMP_NUM_THREADS (the host's CPU share) when it is set."""
+    _, avail = host_cpu()
+    cap = requested or int(os.environ.get("OMP_NUM_THREADS", avail))
+    return max(1, min(avail, cap))
+
+
 def cpu_baseline(n, c, q, stored, iters, reps, threads):
     """The reference's CPU SpMV (MatrixCSR::spmv, OpenMP static rows) on this
-    host: oracle/_ref/ref_bench, built from the reference sources."""
+    host: oracle/_ref/ref_bench, built from the reference sources; where that
+    is not built, the oracle's restatement of the same loop (orc_csr_Ax_omp,
+    "port").  Median over reps of the mean of iters calls."""
     import oracle
+    model, avail = host_cpu()
     exe = oracle.REF_BENCH
-    if not os.path.exists(exe):
-        return None
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
-               OMP_PLACES="cores")
-    out = subprocess.run([exe, "CSR", str(n), str(c), str(q), "0", "0", "0", "1", "1",
-                          str(iters), str(reps)], capture_output=True, text=True, env=env,
-                         timeout=600)
-    if out.returncode != 0:
-        return None
-    rec = json.loads(out.stdout.strip().splitlines()[-1])
-    t = rec["time_op"]
-    gbs = algorithmic_bytes_csr(stored, n, n) / t / 1e9
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": rec["omp_threads"],
-            "kind": "reference",
-            "sample": f"reference MatrixCSR::spmv (OpenMP static) on the full n={n} c={c} "
-                      f"q={q} matrix, median over {reps} repeats of the mean of {iters} calls "
-                      f"({t * 1e3:.2f} ms per SpMV)"}
+    host = f"{model}, {threads} of {avail} logical CPUs in this process's affinity"
+    if os.path.exists(exe):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        out = subprocess.run([exe, "CSR", str(n), str(c), str(q), "0", "0", "0", "1", "1",
+                              str(iters), str(reps)], capture_output=True, text=True, env=env,
+                             timeout=600)
+        if out.returncode == 0:
+            rec = json.loads(out.stdout.strip().splitlines()[-1])
+            t = rec["time_op"]
+            gbs = algorithmic_bytes_csr(stored, n, n) / t / 1e9
+            return {"value": round(gbs, 2), "unit": "GB/s", "cores": rec["omp_threads"],
+                    "kind": "reference", "cpu": host,
+                    "sample": f"reference MatrixCSR::spmv (OpenMP static) on the full n={n} "
+                              f"c={c} q={q} matrix, median over {reps} repeats of the mean of "
+                              f"{iters} calls ({t * 1e3:.2f} ms per SpMV)"}
+    import numpy as np
+    oracle.lib().orc_set_threads(threads)
+    o = oracle.fill_cqmat(oracle.CSR, n, n, c, q, 0, value_seed=0)
+    x = oracle.unit_vector(1, n)
+    r = np.zeros(n)
+    o.csr_Ax_omp(r, x)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            o.csr_Ax_omp(r, x)
+        times.append((time.perf_counter() - t0) / iters)
+    t = float(np.median(times))
+    return {"value": round(algorithmic_bytes_csr(stored, n, n) / t / 1e9, 2), "unit": "GB/s",
+            "cores": threads, "kind": "port", "cpu": host,
+            "sample": f"oracle orc_csr_Ax_omp (the reference's OpenMP static row loop, "
+                      f"restated) on the full n={n} c={c} q={q} matrix, median over {reps} "
+                      f"repeats of the mean of {iters} calls ({t * 1e3:.2f} ms per SpMV)"}
 
 
 def cg_section(args):
@@ -225,11 +268,16 @@ def formats_section(iters: int = 20):
         g = m.geometry()
         if fmt == T.ELL:
             nbytes = 12 * g.ln_row * g.max_col + 8 * g.ln_col + 8 * g.ln_row
-        elif fmt == T.COO:
-            nbytes = 16 * g.stored + 8 * g.n_col + 8 * g.n_row
         else:
-            nbytes = 16 * g.stored + 8 * g.ln_col + 8 * g.ln_row
+            # the timed kernel reads the column and value arrays and a row
+            # pointer it derives once from the row array (kept with the
+            # block), then writes y over the zeroed result: 12 bytes per
+            # entry, not the format's 16
+            rows = g.n_row if fmt == T.COO else g.ln_row
+            nbytes = 12 * g.stored + 4 * (rows + 1) + 8 * g.ln_col + 8 * rows
         extra = {}
+        if fmt != T.ELL:
+            extra = {"bytes_format": 16 * g.stored + 8 * g.ln_col + 8 * g.ln_row}
         if fmt == T.ELL:
             # its column-major kernel, or the split / diagonal layout when one
             # is at least 3 % faster
@@ -248,7 +296,10 @@ def formats_section(iters: int = 20):
         nbytes = algorithmic_bytes_csr(g.stored, g.ln_row, g.ln_col)
         m.autotune(dy, dx, 5)
         out[f"CSR_grid4x4_block0{pc}"] = {"stored": g.stored, "bytes": nbytes,
-                                          "kernel": m.get_csr_kernel(), **timed(m, nbytes)}
+                                          "kernel": m.get_csr_kernel(),
+                                          "layout": m.get_csr_layout()["layout"],
+                                          "mall_resident": nbytes < MALL_BYTES,
+                                          **timed(m, nbytes)}
         del m
     # PageRank on the column-stochastic cfg2 CSR
     m = T.Matrix(T.CSR).fill_cqmat(N, N, C, Q, 0, value_seed=0)
@@ -283,19 +334,24 @@ def formats_section(iters: int = 20):
     A.autotune(y3, x3, 10)
     lay = A.get_csr_layout()
 
-    def timed3():
+    dia_bytes = 7 * 8 * n3 + 16 * n3
+
+    def timed3(own):
         A.time_spmv(y3, x3, 2)
         us = A.time_spmv(y3, x3, iters) / iters * 1e3
-        return {"us": round(us, 1), "GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
-                "frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
-    tuned = timed3()
+        return {"us": round(us, 1), "bytes": own, "GBps": round(own / (us * 1e-6) / 1e9, 1),
+                "frac": round(own / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                "csr_equivalent_GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+                "mall_resident": own < MALL_BYTES}
+    tuned = timed3(dia_bytes if lay["layout"] == T.CSR_DIAG else nbytes)
     A.set_csr_layout(T.CSR_ROWS)
     out["poisson128_spmv"] = {"stored": g.stored, "bytes_csr": nbytes,
-                              "bytes_diagonal_layout": 7 * 8 * n3 + 16 * n3,
+                              "bytes_diagonal_layout": dia_bytes,
                               "autotuned": {"layout": lay["layout"], **tuned},
-                              "row_layout": timed3(),
-                              "note": "GB/s over the CSR algorithmic bytes; layout 2 = diagonal, "
-                                      "which moves bytes_diagonal_layout"}
+                              "row_layout": timed3(nbytes),
+                              "note": "frac over the bytes the timed layout moves (the diagonal "
+                                      "layout: 7 slots of 8 bytes, x and y, per row); both fit "
+                                      "in the 256 MB MALL, so neither is an HBM figure"}
     del A, x3, y3
     # a plain device copy for reference (read + write)
     B = 1 << 30
@@ -308,6 +364,22 @@ def formats_section(iters: int = 20):
     T.check(T.lib().tbsla_hip_stream_synchronize(None))
     us = (time.perf_counter() - t0) / 10 * 1e6
     out["copy_1GiB"] = {"us": round(us, 1), "GBps_moved": round(2 * B / (us * 1e-6) / 1e9, 1)}
+    # the read-stream ceiling: one pass over 2 GiB (8x the MALL), 16-byte loads
+    del b
+    R = 2 << 30
+    big = T.DeviceBuffer(R)
+    sink = T.DeviceBuffer(8192 * 8)
+    T.check(T.lib().tbsla_hip_fill_unit_vector(big.ptr, R // 8, 3, None))
+    T.check(T.lib().tbsla_hip_read_kernel(big.ptr, R, sink.ptr, None))
+    T.check(T.lib().tbsla_hip_stream_synchronize(None))
+    t0 = time.perf_counter()
+    for _ in range(10):
+        T.check(T.lib().tbsla_hip_read_kernel(big.ptr, R, sink.ptr, None))
+    T.check(T.lib().tbsla_hip_stream_synchronize(None))
+    us = (time.perf_counter() - t0) / 10 * 1e6
+    out["read_2GiB"] = {"us": round(us, 1), "GBps_read": round(R / (us * 1e-6) / 1e9, 1),
+                        "frac": round(R / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                        "note": "attainable read stream: the ceiling for read-dominated kernels"}
     return out
 
 
@@ -364,7 +436,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 16)))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="CPU baseline threads (default: this process's CPUs, capped by "
+                         "OMP_NUM_THREADS)")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG time-to-solution section")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default CSR schedule (row layout) instead of the autotuned one")
@@ -423,7 +497,8 @@ def main():
     traffic = measured_traffic(roof["kernel"])
     if traffic is not None:
         roof["traffic"] = traffic
-    cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10, args.cpu_threads)
+    cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10,
+                                                cpu_threads(args.cpu_threads))
     sched = m.get_csr_kernel()
     del m, dx, dy
     cg = None if args.no_cg else cg_section(args)

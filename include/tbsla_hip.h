@@ -84,6 +84,10 @@ int tbsla_hip_stream_synchronize(void *stream);
 /* Device-to-device copy by a plain streaming kernel (16-byte loads and
  * stores), e.g. as a calibration dispatch for hardware counters. */
 int tbsla_hip_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* One streaming pass over bytes (a multiple of 16; d_src 16-byte aligned)
+ * with 16-byte loads, as a read-bandwidth ceiling; 8192 x 8 bytes are
+ * written to d_sink. */
+int tbsla_hip_read_kernel(const void *d_src, size_t bytes, void *d_sink, void *stream);
 int tbsla_hip_device_count(int *count);
 
 /* ---- handles ------------------------------------------------------------ */
@@ -389,8 +393,10 @@ typedef struct {
   int iterations;  /* as tbsla_hip_cg's *nb_iterations */
   int status;      /* TBSLA_HIP_CG_* */
   double res_norm; /* |r| at exit */
-  double time_us;  /* device wall clock from the first iteration to this system's end
-                      (-1 if it did not finish) */
+  double time_us;  /* device wall clock from the first iteration to the test that ended
+                      this system's iteration (the iteration it converged at; a true-residual
+                      check after it, which may resume the system, is not counted); -1 if
+                      it did not finish */
 } tbsla_hip_cg_result;
 
 /* Many independent CG solves at once (no counterpart in the reference): the

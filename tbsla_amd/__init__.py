@@ -66,6 +66,7 @@ SIGNATURES = [
     ("tbsla_hip_memcpy_dtod", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     ("tbsla_hip_stream_synchronize", C.c_int, [_vp]),
     ("tbsla_hip_copy_kernel", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("tbsla_hip_read_kernel", C.c_int, [_vp, C.c_size_t, _vp, _vp]),
     ("tbsla_hip_device_count", C.c_int, [_ip]),
     ("tbsla_hip_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
     ("tbsla_hip_destroy", C.c_int, [_vp]),

@@ -76,11 +76,50 @@ static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
 typedef double v2d_t __attribute__((ext_vector_type(2)));
 
+// 16-byte pieces, four per thread in flight: piece i + u * stride for u < 4
 __global__ __launch_bounds__(kBlock) void k_copy16(const v2d_t *__restrict__ a,
                                                    v2d_t *__restrict__ b, long long n16) {
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16;
-       i += (long long)gridDim.x * kBlock)
-    b[i] = __builtin_nontemporal_load(a + i);
+  const long long stride = (long long)gridDim.x * kBlock;
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    v2d_t t[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) t[u] = __builtin_nontemporal_load(a + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; u++) b[i + u * stride] = t[u];
+  }
+  for (; i < n16; i += stride) b[i] = __builtin_nontemporal_load(a + i);
+}
+
+// One pass over n16 16-byte pieces, four per thread in flight; each
+// workgroup writes one value (the xor of what it read) so the loads are kept.
+__global__ __launch_bounds__(kBlock) void k_read16(const v2d_t *__restrict__ a, long long n16,
+                                                   unsigned long long *__restrict__ sink) {
+  __shared__ unsigned long long lds[kBlock / kWave];
+  const long long stride = (long long)gridDim.x * kBlock;
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long h = 0;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    v2d_t t[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) t[u] = __builtin_nontemporal_load(a + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      h ^= (unsigned long long)__double_as_longlong(t[u][0]) ^
+           (unsigned long long)__double_as_longlong(t[u][1]);
+  }
+  for (; i < n16; i += stride) {
+    const v2d_t t = __builtin_nontemporal_load(a + i);
+    h ^= (unsigned long long)__double_as_longlong(t[0]) ^ (unsigned long long)__double_as_longlong(t[1]);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) h ^= __shfl_xor(h, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / kWave; w++) t ^= lds[w];
+    sink[blockIdx.x] = t;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy1(const char *__restrict__ a, char *__restrict__ b,
@@ -157,6 +196,18 @@ int tbsla_hip_copy_kernel(void *d_dst, const void *d_src, size_t bytes, void *st
                                               static_cast<char *>(d_dst) + 16 * n16, rest);
     TBSLA_HIP_LAUNCH_CHECK();
   }
+  return TBSLA_HIP_SUCCESS;
+}
+
+int tbsla_hip_read_kernel(const void *d_src, size_t bytes, void *d_sink, void *stream) {
+  if (bytes == 0) return TBSLA_HIP_SUCCESS;
+  if (!d_src || !d_sink) return fail(TBSLA_HIP_ERR_INVALID, "null pointer");
+  if ((uintptr_t)d_src % 16 != 0 || (uintptr_t)d_sink % 8 != 0)
+    return fail(TBSLA_HIP_ERR_INVALID, "d_src must be 16-byte and d_sink 8-byte aligned");
+  hipStream_t s = as_stream(stream);
+  k_read16<<<8192, kBlock, 0, s>>>(static_cast<const v2d_t *>(d_src), (long long)(bytes / 16),
+                                   static_cast<unsigned long long *>(d_sink));
+  TBSLA_HIP_LAUNCH_CHECK();
   return TBSLA_HIP_SUCCESS;
 }
 

@@ -1030,6 +1030,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_gz_gdia(CG_ARGS, int par_g, int m
   if (threadIdx.x == 0) a.prz[rsel ^ 1][b] = rz;
 }
 
+// The entries' offsets col - row of kProbeRows rows spread over the block
+// (row (k * 2654435761) mod n): out[k * 9] = the row's entry count, then up
+// to 8 offsets (a count above 8 rules the diagonal form out).
+constexpr int kProbeRows = 64;
+__global__ __launch_bounds__(kWave) void k_g_probe(long long n, int f_col,
+                                                   const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, int *__restrict__ out) {
+  const int k = threadIdx.x;
+  const long long r = (long long)(((unsigned long long)(k + 1) * 2654435761ULL) % (unsigned long long)n);
+  const int beg = rp[r], cnt = rp[r + 1] - beg;
+  out[k * 9] = cnt;
+  for (int j = 0; j < 8; j++) out[k * 9 + 1 + j] = j < cnt ? (int)((ci[beg + j] - f_col) - r) : 0;
+}
+
 // G's entries into its diagonal layout (slots zeroed first): the slot of
 // offset col - row among the offsets given; *bad set when an entry lies on
 // none of them or two entries of a row share one (G is then applied from its
@@ -1164,7 +1178,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_vcheck(CG_ARGS, int j) {
     S->rnorm = rn;
     atomicAdd(&a.com->npending, -1);
     if (rn < S->stop) {
-      S->t_done = wall_clock64();
+      // t_done keeps the time of the iteration whose test the recurrence
+      // passed (k1_test), not the time of this verification
       S->done = kConverged;
       atomicAdd(&a.com->ndone, 1);
     } else {
@@ -1576,52 +1591,42 @@ void enqueue_iteration(const Plan &p, int i, bool tail) {
 }
 
 // G's diagonals for the diagonal form of M = G G^T: the offsets of the
-// entries of three probe rows (at most 8 distinct, all <= 0: G lower
-// triangular), then every entry placed on one of them.  *ndg: the count of
+// entries of kProbeRows rows spread over the block (at most 8 distinct, all
+// <= 0: G lower triangular), then every entry placed on one of them.  *ndg: the count of
 // diagonals, or 0 (G is then applied from its CSR rows) when the probe or
 // the placement fails.
 int build_gdia(CgCache &c, const tbsla_hip_matrix_s *G, bool mixed, hipStream_t s, int *ndg_out) {
   *ndg_out = 0;
   const long long n = G->ln_row;
   if (n < 1 || G->stored < 1) return TBSLA_HIP_SUCCESS;
-  constexpr int kProbe = 3, kMaxDiag = 8;
+  constexpr int kMaxDiag = 8;
   if (!c.h_probe)
-    TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_probe, 64 * sizeof(int), hipHostMallocDefault));
+    TBSLA_HIP_CHECK(hipHostMalloc((void **)&c.h_probe, 1024 * sizeof(int), hipHostMallocDefault));
   int *h = c.h_probe;
-  const long long rows[kProbe] = {n / 4, n / 2, (3 * n) / 4};
-  for (int q = 0; q < kProbe; q++)
-    TBSLA_HIP_CHECK(hipMemcpyAsync(h + 2 * q, G->rowptr + rows[q], 2 * sizeof(int),
-                                   hipMemcpyDeviceToHost, s));
-  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-  int len[kProbe], total = 0;
-  for (int q = 0; q < kProbe; q++) {
-    len[q] = h[2 * q + 1] - h[2 * q];
-    if (len[q] < 0 || len[q] > kMaxDiag) return TBSLA_HIP_SUCCESS;
-  }
-  int beg[kProbe];
-  for (int q = 0; q < kProbe; q++) beg[q] = h[2 * q];
-  for (int q = 0; q < kProbe; q++) {
-    if (len[q] > 0)
-      TBSLA_HIP_CHECK(hipMemcpyAsync(h + 8 + total, G->colidx + beg[q], len[q] * sizeof(int),
-                                     hipMemcpyDeviceToHost, s));
-    total += len[q];
-  }
+  TBSLA_TRY(c.gbad.ensure(kProbeRows * 9 + 1));
+  k_g_probe<<<1, kProbeRows, 0, s>>>(n, G->f_col, G->rowptr, G->colidx, c.gbad.p + 1);
+  TBSLA_HIP_LAUNCH_CHECK();
+  TBSLA_HIP_CHECK(hipMemcpyAsync(h, c.gbad.p + 1, kProbeRows * 9 * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   std::vector<int> off;
-  for (int q = 0, at = 8; q < kProbe; q++)
-    for (int k = 0; k < len[q]; k++, at++) {
-      const long long o = (long long)(h[at] - G->f_col) - rows[q];
-      if (o > 0 || o < -(long long)INT32_MAX) return TBSLA_HIP_SUCCESS;
-      if (std::find(off.begin(), off.end(), (int)o) == off.end()) off.push_back((int)o);
+  for (int k = 0; k < kProbeRows; k++) {
+    const int cnt = h[k * 9];
+    if (cnt < 0 || cnt > kMaxDiag) return TBSLA_HIP_SUCCESS;
+    for (int j = 0; j < cnt; j++) {
+      const int o = h[k * 9 + 1 + j];
+      if (o > 0) return TBSLA_HIP_SUCCESS;
+      if (std::find(off.begin(), off.end(), o) == off.end()) off.push_back(o);
     }
+  }
   if (off.empty() || off.size() > (size_t)kMaxDiag) return TBSLA_HIP_SUCCESS;
   std::sort(off.begin(), off.end());
   const int ndg = (int)off.size();
   const size_t slots = (size_t)((n + kWave - 1) / kWave) * kWave * ndg;
   TBSLA_TRY(c.gdoff.ensure(kMaxDiag));
-  TBSLA_TRY(c.gbad.ensure(1));
-  for (int k = 0; k < ndg; k++) h[32 + k] = off[k];
-  TBSLA_HIP_CHECK(hipMemcpyAsync(c.gdoff.p, h + 32, ndg * sizeof(int), hipMemcpyHostToDevice, s));
+  for (int k = 0; k < ndg; k++) h[kProbeRows * 9 + k] = off[k];
+  TBSLA_HIP_CHECK(hipMemcpyAsync(c.gdoff.p, h + kProbeRows * 9, ndg * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemsetAsync(c.gbad.p, 0, sizeof(int), s));
   if (mixed) {
     TBSLA_TRY(c.gd32.ensure(slots));
@@ -1635,9 +1640,10 @@ int build_gdia(CgCache &c, const tbsla_hip_matrix_s *G, bool mixed, hipStream_t 
                                                    G->values, c.gdoff.p, ndg, c.gd64.p, c.gbad.p);
   }
   TBSLA_HIP_LAUNCH_CHECK();
-  TBSLA_HIP_CHECK(hipMemcpyAsync(h + 48, c.gbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipMemcpyAsync(h + kProbeRows * 9 + kMaxDiag, c.gbad.p, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-  if (!h[48]) *ndg_out = ndg;
+  if (!h[kProbeRows * 9 + kMaxDiag]) *ndg_out = ndg;
   return TBSLA_HIP_SUCCESS;
 }
 

@@ -140,92 +140,302 @@ __device__ inline void vset(VT &v, int q, double x) {
   else v[q] = x;
 }
 
-// One personalised-PageRank step for k = 64 columns fused with the SpMM:
+// ---- personalised PageRank step: tiles of rows, the band window of X in LDS ----
+//
+// One step for k = 64 columns fused with the SpMM:
 // Y[i,:] = A[i,:] X; X'[i,k] = (beta Y[i,k] + S_k (1-beta) [i == src_k]) / S_k;
-// per-block partials of |X' - X| and of X' per column.  The page_rank update
-// (src/mpi/matrix/Matrix.cpp:123-141) with a one-hot teleport per column.
-// Each row's entries are read kPprDepth at a time and their X rows loaded
-// before any is added; products are added in storage order with separate
-// multiply and add (the reference's arithmetic).
-__global__ __launch_bounds__(kBlock) void k_ppr_step(long long n_rows, int f_col, int ln_col,
-                                                     const int *__restrict__ rowptr,
-                                                     const int *__restrict__ colidx,
-                                                     const double *__restrict__ vals,
-                                                     const double *__restrict__ X,
-                                                     double *__restrict__ Xn,
-                                                     const int *__restrict__ src,
-                                                     const double *__restrict__ S, double beta,
-                                                     const PprState *__restrict__ st,
-                                                     double *__restrict__ p_err,
-                                                     double *__restrict__ p_sum) {
+// per-workgroup partials of |X' - X| and of X' per column -- the page_rank
+// update (src/mpi/matrix/Matrix.cpp:123-141) with a one-hot teleport per
+// column, on the SpMM of src/mpi/matrix/MatrixCSR.cpp:11-30.
+//
+// A workgroup takes tiles of kPprTile consecutive rows.  For a tile starting
+// at row r0 it stages the X rows [r0 + dlo, r0 + dlo + kPprWin) -- the band
+// the tile's near-diagonal entries read (dlo from a histogram of the
+// entries' offsets) -- in LDS, 512 bytes per row, read from HBM once and
+// then served to every entry of the band at LDS bandwidth.  A wavefront
+// takes whole rows, lane t holding column t.  A row's entries (at most 64 per
+// chunk) are loaded one per lane and broadcast by readlane; each entry's X
+// row comes from LDS when it lies in the window ("band") or from HBM
+// ("remote": rewired entries, about 10% in a cqmat with q = 0.1).  The sum
+// of a chunk is its band entries in storage order, then its remote entries
+// in storage order, each a separate multiply and add (no FMA); the remote
+// X rows of the next row are loaded while the current row's band is summed,
+// so their latency is hidden behind LDS work.
+constexpr int kPprBlock = 256;                  // threads of a tile workgroup
+constexpr int kPprWaves = kPprBlock / kWave;    // 4 wavefronts
+constexpr int kPprTile = 64;                    // rows per tile
+constexpr int kPprWin = 96;                     // X rows staged per tile (48 KB)
+constexpr int kPprReach = kPprWin - kPprTile;   // band offsets [dlo, dlo + 32] always staged
+constexpr int kPprRemote = 8;                   // remote entries per row loaded ahead
+constexpr int kPprHist = 128;                   // offsets [-64, 63] in the histogram
+
+struct PprArgs {
+  long long n;  // rows = columns of the square block
+  int f_col;
+  const int *rowptr, *colidx;
+  const double *vals;
+  const double *X;
+  double *Xn;
+  const int *src;
+  const double *S;
+  double beta;
+  const PprState *st;
+  double *p_err, *p_sum;
+  int dlo;
+  long long ntiles;
+};
+
+// lane idx's double, the same in every lane (two v_readlane_b32)
+__device__ __forceinline__ double readlane_d(double v, int idx) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), idx);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), idx);
+  return __hiloint2double(hi, lo);
+}
+
+// A row's first chunk of entries, one per lane (lane < len), and the row's
+// old X values (the lane's column).
+struct PprRow {
+  int r, beg, len;
+  int col;
+  double val, old;
+};
+
+__device__ __forceinline__ void ppr_load_row(const PprArgs &a, int r, int lane, PprRow &w) {
+  w.r = r;
+  w.beg = a.rowptr[r];
+  w.len = a.rowptr[r + 1] - w.beg;
+  const bool in = lane < w.len;
+  const int j = in ? w.beg + lane : 0;
+  w.col = in ? a.colidx[j] - a.f_col : -1;
+  w.val = in ? a.vals[j] : 0.0;
+  w.old = a.X[(size_t)r * kWave + lane];
+}
+
+// The chunk's remote entries: the first kPprRemote X rows loaded into xr
+// (their values in vr, zeros past the count); returns the mask of the ones
+// left over.
+__device__ __forceinline__ unsigned long long ppr_remote_issue(const PprArgs &a, int col, double val,
+                                                      bool in, int wlo, int whi, int lane,
+                                                      double (&xr)[kPprRemote],
+                                                      double (&vr)[kPprRemote]) {
+  unsigned long long m = __ballot(in && col >= 0 && col < a.n && (col < wlo || col >= whi));
+#pragma unroll
+  for (int k = 0; k < kPprRemote; k++) {
+    if (m) {
+      const int idx = __builtin_ctzll(m);
+      m &= m - 1;
+      const int c = __builtin_amdgcn_readlane(col, idx);
+      vr[k] = readlane_d(val, idx);
+      xr[k] = a.X[(size_t)c * kWave + lane];
+    } else {
+      vr[k] = 0.0;
+      xr[k] = 0.0;
+    }
+  }
+  return m;
+}
+
+// sum of the chunk's band entries in storage order (X rows from the window)
+__device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo, int whi, int wbase,
+                                  const double *__restrict__ win, int lane) {
   TBSLA_NO_FMA
-  if (st->done) return;
-  __shared__ double lds_e[kPprSlots][kWave];
-  __shared__ double lds_s[kPprSlots][kWave];
-  const int lane = threadIdx.x & (kPprLanes - 1);
-  const int slot = threadIdx.x / kPprLanes;
-  const int c0 = kPprCols * lane;  // this lane's columns c0..c0+kPprCols-1
-  double e[kPprCols], sm[kPprCols];
+  unsigned long long m = __ballot(in && col >= wlo && col < whi);
+  double acc = 0.0;
+  while (m) {
+    double xv[8], vv[8];
 #pragma unroll
-  for (int q = 0; q < kPprCols; q++) e[q] = sm[q] = 0.0;
-  const RowWalk wk = xcd_walk(n_rows, gridDim.x, slot);
-  for (long long row = wk.first; row < wk.end; row += wk.stride) {
-    const int r = (int)row;
-    double acc[kPprCols];
-#pragma unroll
-    for (int q = 0; q < kPprCols; q++) acc[q] = 0.0;
-    const int beg = rowptr[r], end = rowptr[r + 1];
-    for (int j = beg; j < end; j += kPprDepth) {
-      int c[kPprDepth];
-      double v[kPprDepth];
-      pprvec_t xb[kPprDepth];
-#pragma unroll
-      for (int u = 0; u < kPprDepth; u++) {
-        const int jj = min(j + u, end - 1);
-        c[u] = colidx[jj] - f_col;
-        v[u] = vals[jj];
+    for (int k = 0; k < 8; k++) {
+      if (m) {
+        const int idx = __builtin_ctzll(m);
+        m &= m - 1;
+        const int c = __builtin_amdgcn_readlane(col, idx);
+        vv[k] = readlane_d(val, idx);
+        xv[k] = win[(c - wbase) * kWave + lane];
+      } else {
+        vv[k] = 0.0;  // adds +0: acc (never -0) is unchanged
+        xv[k] = 0.0;
       }
+    }
 #pragma unroll
-      for (int u = 0; u < kPprDepth; u++) {
-        const bool ok = c[u] >= 0 && c[u] < ln_col;  // block filter (MatrixCSR.cpp:19)
-        xb[u] = *reinterpret_cast<const pprvec_t *>(X + (size_t)(ok ? c[u] : 0) * kWave + c0);
+    for (int k = 0; k < 8; k++) acc = acc + vv[k] * xv[k];
+  }
+  return acc;
+}
+
+// the remote products after the band (loaded ones, then the left-over ones)
+__device__ __forceinline__ double ppr_remote_add(const PprArgs &a, double acc, int col, double val,
+                                        unsigned long long left, const double (&xr)[kPprRemote],
+                                        const double (&vr)[kPprRemote], int lane) {
+  TBSLA_NO_FMA
+#pragma unroll
+  for (int k = 0; k < kPprRemote; k++) acc = acc + vr[k] * xr[k];
+  while (left) {
+    const int idx = __builtin_ctzll(left);
+    left &= left - 1;
+    const int c = __builtin_amdgcn_readlane(col, idx);
+    acc = acc + readlane_d(val, idx) * a.X[(size_t)c * kWave + lane];
+  }
+  return acc;
+}
+
+// entries past a row's first 64: chunk by chunk, band then remote
+__device__ __forceinline__ double ppr_long_tail(const PprArgs &a, const PprRow &w, double acc, int wlo,
+                                       int whi, int wbase, const double *__restrict__ win,
+                                       int lane) {
+  TBSLA_NO_FMA
+  for (int c0 = kWave; c0 < w.len; c0 += kWave) {
+    const bool in = c0 + lane < w.len;
+    const int j = in ? w.beg + c0 + lane : 0;
+    const int col = in ? a.colidx[j] - a.f_col : -1;
+    const double val = in ? a.vals[j] : 0.0;
+    double xr[kPprRemote], vr[kPprRemote];
+    const unsigned long long left = ppr_remote_issue(a, col, val, in, wlo, whi, lane, xr, vr);
+    acc = acc + ppr_band(col, val, in, wlo, whi, wbase, win, lane);
+    acc = ppr_remote_add(a, acc, col, val, left, xr, vr, lane);
+  }
+  return acc;
+}
+
+// a row's entries with its remote X rows in flight
+struct PprPref {
+  PprRow w;
+  double xr[kPprRemote], vr[kPprRemote];
+  unsigned long long left;
+};
+
+// Row r (in cur): load row r + kPprWaves's entries into nxt, sum r's band,
+// issue nxt's remote X rows, add r's remote products, update.  Called with
+// the two sets alternating, so no register whose load is in flight is ever
+// copied (a copy would wait for it).
+__device__ __forceinline__ void ppr_row(const PprArgs &a, PprPref &cur, PprPref &nxt, int r, long long rend,
+                               int wlo, int whi, int wbase, const double *__restrict__ win,
+                               int lane, int src_l, double S_l, double tele, double &e,
+                               double &sm) {
+  TBSLA_NO_FMA
+  const int rn = r + kPprWaves;
+  const bool has_next = rn < rend;
+  if (has_next) ppr_load_row(a, rn, lane, nxt.w);
+  double acc = ppr_band(cur.w.col, cur.w.val, lane < cur.w.len, wlo, whi, wbase, win, lane);
+  if (has_next)
+    nxt.left = ppr_remote_issue(a, nxt.w.col, nxt.w.val, lane < nxt.w.len, wlo, whi, lane, nxt.xr,
+                                nxt.vr);
+  acc = ppr_remote_add(a, acc, cur.w.col, cur.w.val, cur.left, cur.xr, cur.vr, lane);
+  if (cur.w.len > kWave) acc = ppr_long_tail(a, cur.w, acc, wlo, whi, wbase, win, lane);
+  double tv = a.beta * acc + ((r + a.f_col == src_l) ? tele : 0.0);
+  tv = tv / S_l;
+  e += fabs(tv - cur.w.old);
+  sm += tv;
+  a.Xn[(size_t)r * kWave + lane] = tv;
+}
+
+__global__ __launch_bounds__(kPprBlock, 3) void k_ppr_tile(PprArgs a) {
+  TBSLA_NO_FMA
+  if (a.st->done) return;
+  __shared__ __attribute__((aligned(16))) double win[kPprWin * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const double S_l = a.S[lane];
+  const int src_l = a.src[lane];
+  const double tele = S_l * (1 - a.beta);
+  double e = 0.0, sm = 0.0;
+  // tiles: workgroup b runs on XCD b mod 8, which owns the (b mod 8)-th
+  // contiguous eighth of the tiles; its workgroups take them in turn, so the
+  // tiles in flight on one XCD are neighbours (their windows overlap in L2)
+  const int nb = gridDim.x, b = blockIdx.x;
+  long long t0, t1, tstep;
+  if (nb % 8 == 0) {
+    const long long per = (a.ntiles + 7) / 8;
+    t0 = min(a.ntiles, (long long)(b % 8) * per) + b / 8;
+    t1 = min(a.ntiles, (long long)(b % 8 + 1) * per);
+    tstep = nb / 8;
+  } else {
+    t0 = b;
+    t1 = a.ntiles;
+    tstep = nb;
+  }
+  for (long long t = t0; t < t1; t += tstep) {
+    const long long r0 = t * kPprTile;
+    const long long rend = min(a.n, r0 + kPprTile);
+    const long long wbase = r0 + a.dlo;
+    // staged rows [wlo, whi), empty when the window lies outside the block
+    const int wlo = (int)min(max(wbase, 0LL), a.n);
+    const int whi = (int)max((long long)wlo, min(wbase + kPprWin, a.n));
+    // the wave's first row: entries and old X, then its remote X rows, all
+    // issued before the window is staged
+    PprPref A, B;
+    const int rfirst = (int)(r0 + wave);
+    const bool any = rfirst < rend;
+    A.left = 0;
+    if (any) ppr_load_row(a, rfirst, lane, A.w);
+    if (any)
+      A.left = ppr_remote_issue(a, A.w.col, A.w.val, lane < A.w.len, wlo, whi, lane, A.xr, A.vr);
+    __syncthreads();  // the previous tile's rows are done with the window
+    // stage X rows [wlo, whi): 16-byte pieces, in halves of kHalf per thread
+    // (their loads in flight together)
+    {
+      const int pieces = (whi - wlo) * (kWave / 2);
+      const double2 *srcp = reinterpret_cast<const double2 *>(a.X + (size_t)wlo * kWave);
+      const double2 *safe = reinterpret_cast<const double2 *>(a.X);
+      double2 *dst = reinterpret_cast<double2 *>(win + (size_t)(wlo - wbase) * kWave);
+      constexpr int kHalf = kPprWin * (kWave / 2) / kPprBlock / 2;  // 6 pieces
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        double2 tmp[kHalf];
+#pragma unroll
+        for (int q = 0; q < kHalf; q++) {
+          const int i = (int)threadIdx.x + (h * kHalf + q) * kPprBlock;
+          tmp[q] = *(i < pieces ? srcp + i : safe);
+        }
+#pragma unroll
+        for (int q = 0; q < kHalf; q++) {
+          const int i = (int)threadIdx.x + (h * kHalf + q) * kPprBlock;
+          if (i < pieces) dst[i] = tmp[q];
+        }
       }
-#pragma unroll
-      for (int u = 0; u < kPprDepth; u++)
-        if (j + u < end && c[u] >= 0 && c[u] < ln_col)
-#pragma unroll
-          for (int q = 0; q < kPprCols; q++) acc[q] = acc[q] + v[u] * vget(xb[u], q);
     }
-    const size_t o = (size_t)r * kWave + c0;
-    const pprvec_t old = *reinterpret_cast<const pprvec_t *>(X + o);
-    pprvec_t nv;
-#pragma unroll
-    for (int q = 0; q < kPprCols; q++) {
-      const double Sk = S[c0 + q];
-      double t = beta * acc[q] + ((r + f_col == src[c0 + q]) ? Sk * (1 - beta) : 0.0);
-      t = t / Sk;
-      e[q] += fabs(t - vget(old, q));
-      sm[q] += t;
-      vset(nv, q, t);
+    __syncthreads();
+    for (int r = rfirst; r < rend; r += 2 * kPprWaves) {
+      ppr_row(a, A, B, r, rend, wlo, whi, (int)wbase, win, lane, src_l, S_l, tele, e, sm);
+      if (r + kPprWaves < rend)
+        ppr_row(a, B, A, r + kPprWaves, rend, wlo, whi, (int)wbase, win, lane, src_l, S_l, tele, e,
+                sm);
     }
-    *reinterpret_cast<pprvec_t *>(Xn + o) = nv;
   }
-#pragma unroll
-  for (int q = 0; q < kPprCols; q++) {
-    lds_e[slot][c0 + q] = e[q];
-    lds_s[slot][c0 + q] = sm[q];
-  }
+  // per-column partials of the workgroup: waves in order
+  __syncthreads();
+  win[wave * kWave + lane] = e;
+  win[(kPprWaves + wave) * kWave + lane] = sm;
   __syncthreads();
   if (threadIdx.x < kWave) {
-    const int col = threadIdx.x;
     double te = 0.0, ts = 0.0;
-    for (int w = 0; w < kPprSlots; w++) {
-      te += lds_e[w][col];
-      ts += lds_s[w][col];
+    for (int w = 0; w < kPprWaves; w++) {
+      te += win[w * kWave + lane];
+      ts += win[(kPprWaves + w) * kWave + lane];
     }
-    p_err[(size_t)blockIdx.x * kWave + col] = te;
-    p_sum[(size_t)blockIdx.x * kWave + col] = ts;
+    a.p_err[(size_t)b * kWave + lane] = te;
+    a.p_sum[(size_t)b * kWave + lane] = ts;
   }
+}
+
+// Histogram of the entries' offsets col - row in [-64, 63] over every 16th
+// row (the window's placement): per-workgroup bins in LDS, then added to the
+// global bins.
+__global__ __launch_bounds__(kBlock) void k_ppr_offsets(long long n, int f_col,
+                                                        const int *__restrict__ rowptr,
+                                                        const int *__restrict__ colidx,
+                                                        unsigned int *__restrict__ bins) {
+  __shared__ unsigned int h[kPprHist];
+  if (threadIdx.x < kPprHist) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (long long i = ((long long)blockIdx.x * kBlock + threadIdx.x) * 16; i < n;
+       i += (long long)gridDim.x * kBlock * 16) {
+    for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {
+      const long long o = (long long)(colidx[j] - f_col) - i;
+      if (o >= -kPprHist / 2 && o < kPprHist / 2) atomicAdd(&h[o + kPprHist / 2], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kPprHist && h[threadIdx.x]) atomicAdd(&bins[threadIdx.x], h[threadIdx.x]);
 }
 
 // Sum over nb per-block rows of 64 partials, in block order within each of
@@ -360,13 +570,21 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     return fail(TBSLA_HIP_ERR_INVALID, "personalised PageRank needs a square diagonal block");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long n = m->ln_row;
-  // kPprGrid workgroups (a multiple of 8: one contiguous eighth of the rows
-  // per XCD) unless the block has fewer rows than slots
-  const unsigned nb = n >= (long long)kPprSlots * kPprGrid
-                          ? (unsigned)kPprGrid
-                          : (unsigned)std::max<long long>(1, (n + kPprSlots - 1) / kPprSlots);
+  if (n < 1) return fail(TBSLA_HIP_ERR_INVALID, "empty block");
+  // tile workgroups: three per CU (48 KB of LDS each, 12 wavefronts) on 256 CUs, a
+  // multiple of 8 (one contiguous eighth of the tiles per XCD), fewer when
+  // the block has fewer tiles
+  const long long ntiles = (n + kPprTile - 1) / kPprTile;
+  const unsigned nb = (unsigned)std::min<long long>(768, ntiles);
+  static_assert(kPprWin * (kWave / 2) % (2 * kPprBlock) == 0, "window staging pieces");
+  // partials of the column sums at the end: the row-slot grid of k_col_partial
+  const unsigned nbc = n >= (long long)kPprSlots * kPprGrid
+                           ? (unsigned)kPprGrid
+                           : (unsigned)std::max<long long>(1, (n + kPprSlots - 1) / kPprSlots);
+  const unsigned nparts = std::max(nb, nbc);
   double *buf = nullptr, *part = nullptr, *S = nullptr;
   int *src = nullptr;
+  unsigned int *bins = nullptr;
   PprState *st = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(s);
@@ -375,14 +593,35 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     hipFree(S);
     hipFree(src);
     hipFree(st);
+    hipFree(bins);
   };
   int rc;
   if ((rc = device_alloc((void **)&buf, (size_t)n * kWave * 8)) ||
-      (rc = device_alloc((void **)&part, (size_t)2 * nb * kWave * 8)) ||
+      (rc = device_alloc((void **)&part, (size_t)2 * nparts * kWave * 8)) ||
       (rc = device_alloc((void **)&S, kWave * 8)) || (rc = device_alloc((void **)&src, kWave * 4)) ||
-      (rc = device_alloc((void **)&st, sizeof(PprState)))) {
+      (rc = device_alloc((void **)&st, sizeof(PprState))) ||
+      (rc = device_alloc((void **)&bins, kPprHist * sizeof(unsigned int)))) {
     cleanup();
     return rc;
+  }
+  // where the band lies: the offsets col - row of every 16th row; the window
+  // covers the kPprReach + 1 consecutive offsets holding the most entries
+  std::vector<unsigned int> hb(kPprHist, 0);
+  hipMemsetAsync(bins, 0, kPprHist * sizeof(unsigned int), s);
+  if (m->stored > 0)
+    k_ppr_offsets<<<grid_for((n + 15) / 16), kBlock, 0, s>>>(n, m->f_col, m->rowptr, m->colidx,
+                                                             bins);
+  hipMemcpyAsync(hb.data(), bins, kPprHist * sizeof(unsigned int), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  int dlo = 0;
+  unsigned long long best = 0;
+  for (int lo = 0; lo + kPprReach < kPprHist; lo++) {
+    unsigned long long c = 0;
+    for (int o = lo; o <= lo + kPprReach; o++) c += hb[o];
+    if (c > best) {
+      best = c;
+      dlo = lo - kPprHist / 2;
+    }
   }
   std::vector<double> hS(kWave, (double)n);  // X starts at 1: column sums n
   PprState hst{0, 0, 0.0};
@@ -391,13 +630,27 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   hipMemcpyAsync(st, &hst, sizeof(hst), hipMemcpyHostToDevice, s);
   k_fill_d<<<stride_grid(n * kWave), kBlock, 0, s>>>(d_X, n * kWave, 1.0);
   double *Xb[2] = {d_X, buf};
+  PprArgs pa{};
+  pa.n = n;
+  pa.f_col = m->f_col;
+  pa.rowptr = m->rowptr;
+  pa.colidx = m->colidx;
+  pa.vals = m->values;
+  pa.src = src;
+  pa.S = S;
+  pa.beta = beta;
+  pa.st = st;
+  pa.p_err = part;
+  pa.p_sum = part + (size_t)nb * kWave;
+  pa.dlo = dlo;
+  pa.ntiles = ntiles;
   int launched = 0;
   while (launched < max_iterations) {
     const int batch = std::min(16, max_iterations - launched);
     for (int i = 0; i < batch; i++, launched++) {
-      k_ppr_step<<<nb, kBlock, 0, s>>>(n, m->f_col, m->ln_col, m->rowptr, m->colidx, m->values,
-                                       Xb[launched % 2], Xb[(launched + 1) % 2], src, S, beta, st,
-                                       part, part + (size_t)nb * kWave);
+      pa.X = Xb[launched % 2];
+      pa.Xn = Xb[(launched + 1) % 2];
+      k_ppr_tile<<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     hipError_t e = hipGetLastError();
@@ -410,8 +663,8 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     if (hst.done) break;
   }
   double *X = Xb[hst.iters % 2];
-  k_col_partial<<<nb, kBlock, 0, s>>>(X, n, part);
-  k_col_total<<<1, kBlock, 0, s>>>(part, (int)nb, S);
+  k_col_partial<<<nbc, kBlock, 0, s>>>(X, n, part);
+  k_col_total<<<1, kBlock, 0, s>>>(part, (int)nbc, S);
   k_col_divide<<<stride_grid(n * kWave), kBlock, 0, s>>>(X, n * kWave, S);
   if (X != d_X) hipMemcpyAsync(d_X, X, (size_t)n * kWave * 8, hipMemcpyDeviceToDevice, s);
   hipError_t e = hipGetLastError();

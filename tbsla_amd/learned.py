@@ -120,8 +120,10 @@ class Solver:
         fallback = False
         t0 = time.perf_counter()
         if G is None:
-            T.check(T.lib().tbsla_hip_precond_net_apply(self.net.h, A.h, self.G.h, self.GT.h, None))
-            G, GT = self.G, self.GT
+            # G only: on the stencil's pattern the CG engine applies G and G^T
+            # from G's diagonals (a factor off them gets G^T built per call)
+            T.check(T.lib().tbsla_hip_precond_net_apply(self.net.h, A.h, self.G.h, None, None))
+            G, GT = self.G, None
         o = A.cg_options(max_iterations, tol, True, prec, T.PRECOND_FACTORED, rep,
                          factor=G, factor_t=GT, stall_window=100)
         try:

@@ -198,34 +198,37 @@ def test_cfg3_poisson32_direct():
 
 def test_cfg4_batched_cg_512():
     """512 SPD systems (n = 2^U(12,18), c = 8..32, q = 0.1) in one batched
-    mixed CG: every system converged with a true FP64 relative residual
-    <= 1e-8 (A x computed by the library's FP64 SpMV of the whole block),
-    and 8 sampled systems bit-identical to solving each alone."""
+    CG.  Mixed precision (the benchmark's form): every system converged with
+    a true FP64 relative residual <= 1e-8 (A x by the library's FP64 SpMV of
+    the whole block).  FP64: 8 sampled systems bit-identical to solving each
+    alone (mixed solves verify on a schedule that depends on the other
+    systems, so only their converged residuals are compared)."""
     say("cfg4 batched CG: building 512 systems")
     bd, offs, rhs, n, c = batched.build(512)
     lanes = 8
-    opt = batched.options(bd, lanes)
-    x = T.DeviceBuffer(int(offs[-1]) * 8)
-    res = batched.run(bd, offs, rhs, opt, x)
+    nrows = int(offs[-1])
+    x = T.DeviceBuffer(nrows * 8)
+    _, res, x = batched.run(bd, offs, rhs, batched.options(bd, lanes), x)
     st = np.array([r.status for r in res])
     assert np.all(st == T.CG_CONVERGED), np.unique(st, return_counts=True)
-    y = T.DeviceBuffer(int(offs[-1]) * 8)
+    y = T.DeviceBuffer(nrows * 8)
     T.check(T.lib().tbsla_hip_spmv(bd.h, y.ptr, x.ptr, 0, None))
-    b, ax = rhs.to_numpy(count=int(offs[-1])), y.to_numpy(count=int(offs[-1]))
-    xs = x.to_numpy(count=int(offs[-1]))
-    worst = 0.0
-    for k in range(512):
-        s = slice(offs[k], offs[k + 1])
-        worst = max(worst, float(np.linalg.norm(b[s] - ax[s]) / np.linalg.norm(b[s])))
-    say(f"  worst true rel res {worst:.3e}")
+    b, ax = rhs.to_numpy(count=nrows), y.to_numpy(count=nrows)
+    worst = max(float(np.linalg.norm(b[offs[k]:offs[k + 1]] - ax[offs[k]:offs[k + 1]]) /
+                      np.linalg.norm(b[offs[k]:offs[k + 1]])) for k in range(512))
+    say(f"  mixed: worst true rel res {worst:.3e}, iterations {min(r.iterations for r in res)}"
+        f"..{max(r.iterations for r in res)}")
     assert worst <= 1e-8
+    opt64 = bd.cg_options(5000, 1e-8, True, T.CG_FP64, T.PRECOND_NONE, 0, lanes_per_row=lanes)
+    x64 = T.DeviceBuffer(nrows * 8)
+    _, res64, x64 = batched.run(bd, offs, rhs, opt64, x64)
+    xs = x64.to_numpy(count=nrows)
     rng = np.random.default_rng(5)
     for k in sorted(rng.choice(512, 8, replace=False)):
         p = T.Matrix(T.CSR).fill_sym_cqmat(int(n[k]), int(c[k]), 0.1, 1, value_seed=int(k))
         xk, itk, rk = p.cg(b[offs[k]:offs[k + 1]], tolerance=1e-8, relative=True,
-                           precision=T.CG_MIXED, replace_every=50, lanes_per_row=lanes,
-                           max_iterations=5000)
-        assert itk == res[k].iterations and rk == res[k].res_norm
+                           lanes_per_row=lanes, max_iterations=5000)
+        assert itk == res64[k].iterations and rk == res64[k].res_norm
         assert np.array_equal(xk, xs[offs[k]:offs[k + 1]])
 
 
