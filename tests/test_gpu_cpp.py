@@ -81,17 +81,27 @@ def test_page_rank_app():
 
 
 def test_cg_app(tmp_path):
-    """The CG app: the reference's keys; on an SPD Poisson problem the
-    solve converges (absolute |r| < 1e-10 as CG.cpp:34); a binary image
-    written by the library is read back through --matrix <name>."""
+    """The CG app: the reference's keys.  In the reference's own setting
+    (absolute |r| < 1e-10, CG.cpp:34, with its 1e-10 clamp of the divisors,
+    :35) the iteration stalls once r.r falls below the clamp, so it runs to
+    --max-iterations; with --relative on an SPD Poisson problem it converges.
+    A binary image written by the library is read back through --matrix
+    <name>."""
+    import numpy as np
     import tbsla_amd as T
-    r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "poisson", "--side", 32)
+    r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "poisson", "--side", 16,
+                "--max-iterations", 300)
     assert r.returncode == 0, r.stderr
     rec = json.loads(r.stdout.strip().splitlines()[-1])
     for key in MPI_KEYS:
         assert key in rec, key
-    assert rec["test"] == "conjugate_gradient" and rec["status"] == "ok"
-    assert float(rec["res_norm"]) < 1e-10 and int(rec["nb_iterations"]) > 10
+    assert rec["test"] == "conjugate_gradient" and int(rec["nb_iterations"]) <= 300
+    r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "poisson", "--side", 32,
+                "--tolerance", "1e-8", "--relative")
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    assert rec["status"] == "ok" and int(rec["nb_iterations"]) > 10
+    assert float(rec["res_norm"]) < 1e-8 * np.sqrt(32 ** 3)  # |b| = sqrt(n): b is all ones
     g = T.Matrix(T.CSR).fill_sym_cqmat(50000, 8, 0.1, 1)
     T.check(T.lib().tbsla_hip_write_file(g.h, str(tmp_path / "spd.CSR").encode(), None))
     r = run_app("tbsla_perf_cg_hip", "--format", "CSR", "--matrix", "spd", "--matrix_folder",
@@ -100,3 +110,18 @@ def test_cg_app(tmp_path):
     rec = json.loads(r.stdout.strip().splitlines()[-1])
     assert rec["status"] == "ok" and rec["matrix_dim"] == "50000"
     assert run_app("tbsla_perf_cg_hip", "--format", "ELL", "--matrix", "cqmat").returncode == 1
+
+
+def test_cpp_subclass_of_reference_matrix():
+    """tbsla::hip::CppMatrix (include/tbsla/hip/CppMatrix.hpp) is a
+    tbsla::cpp::Matrix: GPU and reference CPU blocks driven through the
+    reference's base class -- its virtual fills, spmv, Ax, write/read and
+    its own compiled a_axpx_ / AAxpAxpx / make_stochastic (Matrix.cpp:5-38)
+    -- give equal results for all five formats.  The program is built by
+    oracle/Makefile where the reference's sources are present."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "test_cpp_subclass")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/test_cpp_subclass not built (needs the reference's sources)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-2000:])
+    assert "test_cpp_subclass ok" in r.stdout
