@@ -9,8 +9,10 @@
 // the reference's own non-virtual a_axpx_ / AAxpAx / AAxpAxpx /
 // make_stochastic (src/cpp/matrix/Matrix.cpp:5-38, which call the
 // virtuals), the base class's getters, write and read.  Values are 1.0 and
-// x holds small integers, so every product and sum is exact: results must
-// be equal.  Prints "test_cpp_subclass ok".
+// x holds small integers, so every product and sum is exact and results
+// must be equal -- until make_stochastic divides by column sums; from there
+// on a SpMV is compared to 1e-12 relative (the GPU sums a row in another
+// order).  Prints "test_cpp_subclass ok".
 #include <tbsla/cpp/MatrixCOO.hpp>
 #include <tbsla/cpp/MatrixCSR.hpp>
 #include <tbsla/cpp/MatrixDENSE.hpp>
@@ -47,6 +49,15 @@ void destroy(tbsla::cpp::Matrix *m) {
   else if (auto *p = dynamic_cast<tbsla::cpp::MatrixDENSE *>(m)) delete p;
 }
 typedef std::unique_ptr<tbsla::cpp::Matrix, void (*)(tbsla::cpp::Matrix *)> RefPtr;
+
+// |a[i] - b[i]| <= 1e-12 |b[i]| for every i (NaN fails)
+bool close12(const double *a, const double *b, int n) {
+  for (int i = 0; i < n; i++) {
+    const double d = a[i] - b[i], t = 1e-12 * (b[i] < 0 ? -b[i] : b[i]) + 1e-300;
+    if (!(d <= t && -d <= t)) return false;
+  }
+  return true;
+}
 
 RefPtr reference(int fmt) {
   switch (fmt) {
@@ -96,13 +107,7 @@ void drive(int fmt, tbsla::cpp::Matrix &G, tbsla::cpp::Matrix &R, const std::str
   expect(same(s1.data(), s2.data(), R.get_ln_col()), name + ": make_stochastic column sums");
   {
     std::unique_ptr<double[]> a(G.spmv(x.data())), b(R.spmv(x.data()));
-    bool ok = true;
-    for (int i = 0; i < nout; i++) {
-      const double d = a[i] - b[i];
-      if (!(d <= 1e-12 * (b[i] < 0 ? -b[i] : b[i]) + 1e-300 && -d <= 1e-12 * (b[i] < 0 ? -b[i] : b[i]) + 1e-300))
-        ok = false;
-    }
-    expect(ok, name + ": spmv after make_stochastic (1e-12)");
+    expect(close12(a.get(), b.get(), nout), name + ": spmv after make_stochastic (1e-12)");
   }
   // the reference's image read by the GPU block gives the same SpMV
   std::ostringstream os;
@@ -114,10 +119,7 @@ void drive(int fmt, tbsla::cpp::Matrix &G, tbsla::cpp::Matrix &R, const std::str
   expect(Hb.get_nnz() == R.get_nnz() && Hb.get_ln_row() == R.get_ln_row(), name + ": read getters");
   {
     std::unique_ptr<double[]> a(Hb.spmv(x.data())), b(R.spmv(x.data()));
-    bool ok = true;
-    for (int i = 0; i < nout; i++)
-      if (a[i] != b[i]) ok = false;
-    expect(ok, name + ": spmv of the reference's image read on the GPU");
+    expect(close12(a.get(), b.get(), nout), name + ": spmv of the reference's image read on the GPU (1e-12)");
   }
 }
 
