@@ -216,6 +216,17 @@ int tbsla_hip_values_changed(tbsla_hip_matrix_t m, void *stream);
  * d_v has ln_col entries (n_col for COO); d_r has ln_row (n_row for COO). */
 int tbsla_hip_Ax(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
                  void *stream);
+/* Matrix::Ax without device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under hipGraph capture, serving
+ * loops, allocation-free harnesses).  Same result as tbsla_hip_Ax bit for
+ * bit (unsorted COO triplets, summed with atomics in both, agree within
+ * rounding).  The one path of tbsla_hip_Ax that allocates -- the segmented
+ * accumulation of row-sorted COO/SCOO triplets, which keeps per-chunk
+ * partials -- uses a fixed 16 MiB array per GPU instead: at most about
+ * 358 M triplets (more gives TBSLA_HIP_ERR_RANGE).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU race. */
+int tbsla_hip_Ax_noalloc(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
+                         void *stream);
 /* Matrix::spmv (Matrix.hpp:14): r = 0, then Ax. */
 int tbsla_hip_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int vect_incr,
                    void *stream);

@@ -95,6 +95,7 @@ SIGNATURES = [
     ("tbsla_hip_write_file", C.c_int, [_vp, C.c_char_p, _vp]),
     ("tbsla_hip_read_file", C.c_int, [_vp, C.c_char_p, C.c_longlong, _vp]),
     ("tbsla_hip_Ax", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    ("tbsla_hip_Ax_noalloc", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_a_axpx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     ("tbsla_hip_AAxpAx", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),

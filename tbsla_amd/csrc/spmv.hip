@@ -22,6 +22,9 @@
 #include <algorithm>
 #include <climits>
 
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 #include "csr_rows.hpp"
 #include "kernels.hpp"
@@ -345,6 +348,33 @@ int pick_csr_rows(const tbsla_hip_matrix_s *m, int L) {
 int pick_csr_vec(const tbsla_hip_matrix_s *m) { return m->csr_vec > 0 ? m->csr_vec : 1; }
 int pick_csr_nt(const tbsla_hip_matrix_s *m) { return m->csr_nt >= 0 ? m->csr_nt : 1; }
 
+// Scratch of the segmented COO/SCOO accumulation of n row-sorted triplets:
+// each chunk's head and tail keys (int) and values (double).
+size_t coo_seg_scratch_bytes(long long n) {
+  const long long nchunks = (n + kSegChunk - 1) / kSegChunk;
+  return (size_t)nchunks * 24 + 256;
+}
+
+// k_coo_seg + k_coo_fixup with their per-chunk heads and tails in sc (every
+// chunk writes both, so sc needs no clearing)
+int coo_seg_launch(const tbsla_hip_matrix_s *m, long long n, int row_off, int col_off,
+                   const double *d_v, double *d_r, char *sc, hipStream_t s) {
+  const long long nchunks = (n + kSegChunk - 1) / kSegChunk;
+  int *head_key = reinterpret_cast<int *>(sc);
+  int *tail_key = head_key + nchunks;
+  const size_t off = ((size_t)nchunks * 8 + 15) & ~(size_t)15;
+  double *head_val = reinterpret_cast<double *>(sc + off);
+  double *tail_val = head_val + nchunks;
+  k_coo_seg<<<grid_for(nchunks * kWave), kBlock, 0, s>>>(n, m->rowidx, m->colidx, m->values, d_v,
+                                                         row_off, col_off, d_r, head_key, head_val,
+                                                         tail_key, tail_val);
+  TBSLA_HIP_LAUNCH_CHECK();
+  k_coo_fixup<<<grid_for(nchunks), kBlock, 0, s>>>(nchunks, head_key, head_val, tail_key, tail_val,
+                                                   row_off, d_r);
+  TBSLA_HIP_LAUNCH_CHECK();
+  return TBSLA_HIP_SUCCESS;
+}
+
 int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_incr,
                 bool zero_first, hipStream_t s, const int *d_done, const int *seg_offs, int nseg,
                 const int *seg_done) {
@@ -411,22 +441,8 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
         TBSLA_HIP_LAUNCH_CHECK();
         return TBSLA_HIP_SUCCESS;
       }
-      const long long nchunks = (n + kSegChunk - 1) / kSegChunk;
-      TBSLA_TRY(ensure_scratch(m, (size_t)nchunks * 24 + 256));
-      char *sc = static_cast<char *>(m->scratch);
-      int *head_key = reinterpret_cast<int *>(sc);
-      int *tail_key = head_key + nchunks;
-      size_t off = ((size_t)nchunks * 8 + 15) & ~(size_t)15;
-      double *head_val = reinterpret_cast<double *>(sc + off);
-      double *tail_val = head_val + nchunks;
-      k_coo_seg<<<grid_for(nchunks * kWave), kBlock, 0, s>>>(n, m->rowidx, m->colidx, m->values,
-                                                             d_v, row_off, col_off, d_r, head_key,
-                                                             head_val, tail_key, tail_val);
-      TBSLA_HIP_LAUNCH_CHECK();
-      k_coo_fixup<<<grid_for(nchunks), kBlock, 0, s>>>(nchunks, head_key, head_val, tail_key,
-                                                       tail_val, row_off, d_r);
-      TBSLA_HIP_LAUNCH_CHECK();
-      return TBSLA_HIP_SUCCESS;
+      TBSLA_TRY(ensure_scratch(m, coo_seg_scratch_bytes(n)));
+      return coo_seg_launch(m, n, row_off, col_off, d_v, d_r, static_cast<char *>(m->scratch), s);
     }
     case TBSLA_HIP_DENSE: {
       if (zero_first) TBSLA_TRY(launch_fill(d_r, m->ln_row, 0.0, s));
@@ -442,4 +458,66 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
   return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
 }
 
+// ---- Ax without allocation -----------------------------------------------
+//
+// One scratch array per GPU for the segmented COO/SCOO accumulation:
+// kNoAllocScratch bytes, i.e. up to (kNoAllocScratch - 256) / 24 chunks of
+// 512 triplets (about 358 M).  Each GPU the library runs on holds it until
+// the process ends.  Host code uses the device address resolved once per
+// device by hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocScratch = 16u << 20;
+__device__ __align__(256) char scratch[kNoAllocScratch];
+
+namespace {
+
+std::mutex &scratch_mutex() {
+  static std::mutex *mu = new std::mutex;
+  return *mu;
+}
+
+// the current device's copy of `scratch`, resolved on the first call on it
+int scratch_address(char **out) {
+  int dev = 0;
+  TBSLA_HIP_CHECK(hipGetDevice(&dev));
+  static std::vector<char *> *ptrs = new std::vector<char *>;
+  std::lock_guard<std::mutex> g(scratch_mutex());
+  if ((int)ptrs->size() <= dev) ptrs->resize((size_t)dev + 1, nullptr);
+  if (!(*ptrs)[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TBSLA_HIP_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    (*ptrs)[dev] = static_cast<char *>(p);
+  }
+  *out = (*ptrs)[dev];
+  return TBSLA_HIP_SUCCESS;
+}
+
+}  // namespace
+
 }  // namespace tbsla_hip
+
+using namespace tbsla_hip;
+
+extern "C" int tbsla_hip_Ax_noalloc(tbsla_hip_matrix_t m, double *d_r, const double *d_v,
+                                    int vect_incr, void *stream) {
+  if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null matrix handle");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool coo = m->format == TBSLA_HIP_COO, scoo = m->format == TBSLA_HIP_SCOO;
+  // CSR, ELL and DENSE, and unsorted triplets (atomics), allocate nothing
+  // in tbsla_hip_Ax: the same launch
+  if (!(coo || scoo) || !m->row_sorted) return spmv_launch(m, d_r, d_v, vect_incr, false, s);
+  const long long n = (!coo && m->nnz == 0) ? 0 : m->stored;
+  if (n <= 0 || !m->has_arrays) return TBSLA_HIP_SUCCESS;
+  const size_t need = coo_seg_scratch_bytes(n);
+  if (need > kNoAllocScratch)
+    return fail(TBSLA_HIP_ERR_RANGE, "tbsla_hip_Ax_noalloc: " + std::to_string(n) +
+                                         " triplets need " + std::to_string(need) +
+                                         " bytes of scratch, more than its " +
+                                         std::to_string(kNoAllocScratch));
+  char *sc = nullptr;
+  TBSLA_TRY(scratch_address(&sc));
+  const int row_off = coo ? vect_incr : vect_incr - m->f_row;
+  const int col_off = coo ? 0 : -m->f_col;
+  return coo_seg_launch(m, n, row_off, col_off, d_v, d_r, sc, s);
+}
