@@ -448,8 +448,9 @@ def main():
                     help="skip the other formats, grid blocks, PageRank, attention and copy "
                          "figures")
     ap.add_argument("--no-batched", action="store_true",
-                    help="skip the batched CG, batched PageRank and personalised-PageRank "
-                         "sections")
+                    help="skip the batched CG and batched PageRank sections")
+    ap.add_argument("--no-ppr", action="store_true",
+                    help="skip the personalised-PageRank (SpMM) section")
     ap.add_argument("--batched-systems", type=int, default=512)
     ap.add_argument("--cg-problems", type=int, default=8,
                     help="held-out 64^3 problems in the CG section (128^3: a quarter)")
@@ -510,6 +511,7 @@ def main():
         from tbsla_amd import batched
         extra["batched_cg"] = batched.bench(args.batched_systems)
         extra["batched_page_rank"] = batched.page_rank_bench(args.batched_systems)
+    if not args.no_ppr:
         extra["ppr_spmm"] = ppr_section()
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),

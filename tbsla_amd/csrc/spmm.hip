@@ -158,15 +158,18 @@ __device__ inline void vset(VT &v, int q, double x) {
 // row comes from LDS when it lies in the window ("band") or from HBM
 // ("remote": rewired entries, about 10% in a cqmat with q = 0.1).  The sum
 // of a chunk is its band entries in storage order, then its remote entries
-// in storage order, each a separate multiply and add (no FMA); the remote
-// X rows of the next row are loaded while the current row's band is summed,
-// so their latency is hidden behind LDS work.
+// in storage order, each a separate multiply and add (no FMA).  A
+// wavefront runs its rows through a software pipeline (row pointers, then
+// entries, then the remote X rows, then the sum), so each row's loads are in
+// flight while earlier rows are summed from LDS; the band sum walks the
+// entries by lane index, with no scalar bookkeeping per entry.
 constexpr int kPprBlock = 256;                  // threads of a tile workgroup
 constexpr int kPprWaves = kPprBlock / kWave;    // 4 wavefronts
 constexpr int kPprTile = 64;                    // rows per tile
+constexpr int kPprRowsPerWave = kPprTile / kPprWaves;
 constexpr int kPprWin = 96;                     // X rows staged per tile (48 KB)
 constexpr int kPprReach = kPprWin - kPprTile;   // band offsets [dlo, dlo + 32] always staged
-constexpr int kPprRemote = 8;                   // remote entries per row loaded ahead
+constexpr int kPprRemote = 8;                   // remote X rows in flight together (long rows)
 constexpr int kPprHist = 128;                   // offsets [-64, 63] in the histogram
 
 struct PprArgs {
@@ -183,6 +186,7 @@ struct PprArgs {
   double *p_err, *p_sum;
   int dlo;
   long long ntiles;
+  int last;  // index of the last stored entry (a dummy entry when the block has none)
 };
 
 // lane idx's double, the same in every lane (two v_readlane_b32)
@@ -190,25 +194,6 @@ __device__ __forceinline__ double readlane_d(double v, int idx) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), idx);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), idx);
   return __hiloint2double(hi, lo);
-}
-
-// A row's first chunk of entries, one per lane (lane < len), and the row's
-// old X values (the lane's column).
-struct PprRow {
-  int r, beg, len;
-  int col;
-  double val, old;
-};
-
-__device__ __forceinline__ void ppr_load_row(const PprArgs &a, int r, int lane, PprRow &w) {
-  w.r = r;
-  w.beg = a.rowptr[r];
-  w.len = a.rowptr[r + 1] - w.beg;
-  const bool in = lane < w.len;
-  const int j = in ? w.beg + lane : 0;
-  w.col = in ? a.colidx[j] - a.f_col : -1;
-  w.val = in ? a.vals[j] : 0.0;
-  w.old = a.X[(size_t)r * kWave + lane];
 }
 
 // The chunk's remote entries: the first kPprRemote X rows loaded into xr
@@ -262,6 +247,36 @@ __device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo
   return acc;
 }
 
+// The same sum for a row's first chunk (len entries, at most 64), walking
+// the entries in storage order by lane index: a lane's entry outside the
+// window points at the window's zero row with value 0 (adds +0, which
+// leaves acc unchanged), so the walk needs no mask and no scalar
+// bookkeeping -- three readlanes, an LDS read, a multiply and an add per
+// entry.
+__device__ __forceinline__ double ppr_band_walk(int col, double val, int len, int wlo, int whi,
+                                                int wbase, const double *__restrict__ win,
+                                                int lane) {
+  TBSLA_NO_FMA
+  const bool band = lane < len && col >= wlo && col < whi;
+  const int off = band ? (col - wbase) * kWave : kPprWin * kWave;  // zero row
+  const double v = band ? val : 0.0;
+  double acc = 0.0;
+#pragma unroll
+  for (int ch = 0; ch < kWave / 8; ch++) {
+    if (ch * 8 >= len) break;
+    double xv[8], vv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int o = __builtin_amdgcn_readlane(off, ch * 8 + k);
+      vv[k] = readlane_d(v, ch * 8 + k);
+      xv[k] = win[o + lane];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc = acc + vv[k] * xv[k];
+  }
+  return acc;
+}
+
 // the remote products after the band (loaded ones, then the left-over ones)
 __device__ __forceinline__ double ppr_remote_add(const PprArgs &a, double acc, int col, double val,
                                         unsigned long long left, const double (&xr)[kPprRemote],
@@ -279,13 +294,13 @@ __device__ __forceinline__ double ppr_remote_add(const PprArgs &a, double acc, i
 }
 
 // entries past a row's first 64: chunk by chunk, band then remote
-__device__ __forceinline__ double ppr_long_tail(const PprArgs &a, const PprRow &w, double acc, int wlo,
-                                       int whi, int wbase, const double *__restrict__ win,
-                                       int lane) {
+__device__ __forceinline__ double ppr_long_tail(const PprArgs &a, int beg, int len, double acc,
+                                                int wlo, int whi, int wbase,
+                                                const double *__restrict__ win, int lane) {
   TBSLA_NO_FMA
-  for (int c0 = kWave; c0 < w.len; c0 += kWave) {
-    const bool in = c0 + lane < w.len;
-    const int j = in ? w.beg + c0 + lane : 0;
+  for (int c0 = kWave; c0 < len; c0 += kWave) {
+    const bool in = c0 + lane < len;
+    const int j = in ? beg + c0 + lane : 0;
     const int col = in ? a.colidx[j] - a.f_col : -1;
     const double val = in ? a.vals[j] : 0.0;
     double xr[kPprRemote], vr[kPprRemote];
@@ -296,110 +311,235 @@ __device__ __forceinline__ double ppr_long_tail(const PprArgs &a, const PprRow &
   return acc;
 }
 
-// a row's entries with its remote X rows in flight
-struct PprPref {
-  PprRow w;
-  double xr[kPprRemote], vr[kPprRemote];
-  unsigned long long left;
+// a row in the pipeline (row q of the wavefront's sequence): its row-pointer
+// pair, its first chunk of entries, one per lane (lane < len), the lane's old
+// X value, and the X rows of its first S remote entries
+template <int S>
+struct PprSlot {
+  int rp;                   // lanes 0, 1: rowptr[r], rowptr[r + 1]
+  int beg, len;
+  int col;                  // lane's entry: column in the block, -1 past the chunk
+  double val, old;
+  double xr[S];             // X rows of the first S remote entries
 };
 
-// Row r (in cur): load row r + kPprWaves's entries into nxt, sum r's band,
-// issue nxt's remote X rows, add r's remote products, update.  Called with
-// the two sets alternating, so no register whose load is in flight is ever
-// copied (a copy would wait for it).
-__device__ __forceinline__ void ppr_row(const PprArgs &a, PprPref &cur, PprPref &nxt, int r, long long rend,
-                               int wlo, int whi, int wbase, const double *__restrict__ win,
-                               int lane, int src_l, double S_l, double tele, double &e,
-                               double &sm) {
+// the staged window of row r's tile: X rows [wlo, whi), window row 0 = wbase
+struct PprWindow {
+  int wlo, whi, wbase;
+};
+__device__ __forceinline__ PprWindow ppr_window(const PprArgs &a, long long r) {
+  const long long wbase = (r / kPprTile) * kPprTile + a.dlo;
+  PprWindow w;
+  w.wlo = (int)min(max(wbase, 0LL), a.n);
+  w.whi = (int)max((long long)w.wlo, min(wbase + kPprWin, a.n));
+  w.wbase = (int)wbase;
+  return w;
+}
+
+// The rows of one wavefront: workgroup b runs on XCD b mod 8, which owns the
+// (b mod 8)-th contiguous eighth of the tiles; its workgroups take them in
+// turn (the tiles in flight on one XCD are neighbours, their windows overlap
+// in L2).  Row q of the wavefront is row wave + kPprWaves (q mod
+// kPprRowsPerWave) of its (q / kPprRowsPerWave)-th tile.
+struct PprWalk {
+  long long t0, t1, tstep;
+  int wave;
+};
+__device__ __forceinline__ int ppr_row_at(const PprArgs &a, const PprWalk &wk, int q) {
+  const long long t = wk.t0 + (long long)(q / kPprRowsPerWave) * wk.tstep;
+  const long long r = t * kPprTile + wk.wave + kPprWaves * (q % kPprRowsPerWave);
+  return (t < wk.t1 && r < a.n) ? (int)r : -1;
+}
+
+// The stages issue the same loads whatever the row (a missing row or an
+// unused slot reads row 0 or the row's own X row, both valid), so the
+// compiler counts the loads in flight exactly and waits only for the ones a
+// stage consumes.
+//
+// stage P: the row-pointer pair
+template <int S>
+__device__ __forceinline__ void ppr_sp(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
+  s.rp = a.rowptr[max(r, 0) + (lane & 1)];
+}
+
+// stage E: the first chunk of entries and the old X values
+template <int S>
+__device__ __forceinline__ void ppr_se(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
+  const int beg = __builtin_amdgcn_readlane(s.rp, 0);
+  const int len = r < 0 ? 0 : __builtin_amdgcn_readlane(s.rp, 1) - beg;
+  const bool in = lane < len;
+  const int j = min(beg + lane, a.last);
+  const int c = a.colidx[j];
+  const double v = a.vals[j];
+  s.old = a.X[(size_t)max(r, 0) * kWave + lane];
+  s.beg = beg;
+  s.len = len;
+  s.col = in ? c - a.f_col : -1;
+  s.val = in ? v : 0.0;
+}
+
+// the chunk's entries outside the row's window (summed from HBM, not LDS)
+__device__ __forceinline__ unsigned long long ppr_remote_mask(const PprArgs &a, int col,
+                                                              const PprWindow &w) {
+  return __ballot(col >= 0 && col < a.n && (col < w.wlo || col >= w.whi));
+}
+
+// stage R: the X rows of the first S remote entries
+template <int S>
+__device__ __forceinline__ void ppr_sr(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
+  const int rr = max(r, 0);
+  unsigned long long m = ppr_remote_mask(a, s.col, ppr_window(a, rr));
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    const int idx = m ? __builtin_ctzll(m) : 0;
+    const int c = m ? __builtin_amdgcn_readlane(s.col, idx) : rr;
+    m &= m - 1;
+    s.xr[k] = a.X[(size_t)c * kWave + lane];
+  }
+}
+
+// stage C: the row's sum -- band entries (LDS) in storage order, then remote
+// entries in storage order, then entries past the first 64 chunk by chunk,
+// each a separate multiply and add -- and its update
+template <int S>
+__device__ __forceinline__ void ppr_sc(const PprArgs &a, int r, const PprSlot<S> &s,
+                                       const double *__restrict__ win, int lane, int src_l,
+                                       double S_l, double tele, double &e, double &sm) {
   TBSLA_NO_FMA
-  const int rn = r + kPprWaves;
-  const bool has_next = rn < rend;
-  if (has_next) ppr_load_row(a, rn, lane, nxt.w);
-  double acc = ppr_band(cur.w.col, cur.w.val, lane < cur.w.len, wlo, whi, wbase, win, lane);
-  if (has_next)
-    nxt.left = ppr_remote_issue(a, nxt.w.col, nxt.w.val, lane < nxt.w.len, wlo, whi, lane, nxt.xr,
-                                nxt.vr);
-  acc = ppr_remote_add(a, acc, cur.w.col, cur.w.val, cur.left, cur.xr, cur.vr, lane);
-  if (cur.w.len > kWave) acc = ppr_long_tail(a, cur.w, acc, wlo, whi, wbase, win, lane);
+  if (r < 0) return;
+  const PprWindow w = ppr_window(a, r);
+  double acc = ppr_band_walk(s.col, s.val, s.len, w.wlo, w.whi, w.wbase, win, lane);
+  unsigned long long m = ppr_remote_mask(a, s.col, w);
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    if (m) {
+      const int idx = __builtin_ctzll(m);
+      m &= m - 1;
+      acc = acc + readlane_d(s.val, idx) * s.xr[k];
+    }
+  }
+  while (m) {  // remote entries past the first S (rare)
+    const int idx = __builtin_ctzll(m);
+    m &= m - 1;
+    const int c = __builtin_amdgcn_readlane(s.col, idx);
+    acc = acc + readlane_d(s.val, idx) * a.X[(size_t)c * kWave + lane];
+  }
+  if (s.len > kWave) acc = ppr_long_tail(a, s.beg, s.len, acc, w.wlo, w.whi, w.wbase, win, lane);
   double tv = a.beta * acc + ((r + a.f_col == src_l) ? tele : 0.0);
   tv = tv / S_l;
-  e += fabs(tv - cur.w.old);
+  e += fabs(tv - s.old);
   sm += tv;
   a.Xn[(size_t)r * kWave + lane] = tv;
 }
 
-__global__ __launch_bounds__(kPprBlock, 3) void k_ppr_tile(PprArgs a) {
+// Distances of the stages ahead of the row being summed: P at D0 = NS - 1,
+// E at DE, R at DR.
+template <int NS, int DE, int DR>
+struct PprPipe {
+  static constexpr int D0 = NS - 1;
+  static_assert(D0 > DE && DE > DR && DR >= 1, "each stage after the one it consumes");
+};
+
+// One group of NS rows: for row q0 + K, stage P of row q0 + K + D0, E of
+// q0 + K + DE, R of q0 + K + DR, then C of the row itself.  The slot index
+// is a constant in every call, so the slots live in registers.
+template <int NS, int DE, int DR, int S, int K>
+__device__ __forceinline__ void ppr_group_step(const PprArgs &a, const PprWalk &wk, int q0,
+                                               PprSlot<S> (&st)[NS],
+                                               const double *__restrict__ win, int lane,
+                                               int src_l, double S_l, double tele, double &e,
+                                               double &sm) {
+  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
+  const int q = q0 + K;
+  ppr_sp(a, ppr_row_at(a, wk, q + D0), lane, st[(K + D0) % NS]);
+  ppr_se(a, ppr_row_at(a, wk, q + DE), lane, st[(K + DE) % NS]);
+  ppr_sr(a, ppr_row_at(a, wk, q + DR), lane, st[(K + DR) % NS]);
+  ppr_sc(a, ppr_row_at(a, wk, q), st[K], win, lane, src_l, S_l, tele, e, sm);
+  if constexpr (K + 1 < NS)
+    ppr_group_step<NS, DE, DR, S, K + 1>(a, wk, q0, st, win, lane, src_l, S_l, tele, e, sm);
+}
+
+// before the first group: P of rows 0 .. D0 - 1, E of rows 0 .. DE - 1, R of
+// rows 0 .. DR - 1
+template <int NS, int DE, int DR, int S, int K>
+__device__ __forceinline__ void ppr_prologue(const PprArgs &a, const PprWalk &wk,
+                                             PprSlot<S> (&st)[NS], int lane) {
+  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
+  if constexpr (K < D0) {
+    ppr_sp(a, ppr_row_at(a, wk, K), lane, st[K]);
+  } else if constexpr (K < D0 + DE) {
+    ppr_se(a, ppr_row_at(a, wk, K - D0), lane, st[K - D0]);
+  } else if constexpr (K < D0 + DE + DR) {
+    ppr_sr(a, ppr_row_at(a, wk, K - D0 - DE), lane, st[K - D0 - DE]);
+  }
+  if constexpr (K + 1 < D0 + DE + DR) ppr_prologue<NS, DE, DR, S, K + 1>(a, wk, st, lane);
+}
+
+// A workgroup takes tiles of kPprTile rows (kPprRowsPerWave per wavefront);
+// a wavefront runs its rows through the four stages above, so a row's
+// row-pointer, entry and remote X loads are in flight while the rows before
+// it are summed from LDS.  NS divides kPprRowsPerWave, so a tile starts a
+// group.
+template <int NS, int DE, int DR, int S>
+__global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   TBSLA_NO_FMA
+  static_assert(kPprRowsPerWave % NS == 0, "a tile starts a group");
   if (a.st->done) return;
-  __shared__ __attribute__((aligned(16))) double win[kPprWin * kWave];
+  // the window, then one row of zeros (read for entries outside it)
+  __shared__ __attribute__((aligned(16))) double win[(kPprWin + 1) * kWave];
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  if (threadIdx.x < kWave) win[kPprWin * kWave + threadIdx.x] = 0.0;  // before the first barrier
   const double S_l = a.S[lane];
   const int src_l = a.src[lane];
   const double tele = S_l * (1 - a.beta);
   double e = 0.0, sm = 0.0;
-  // tiles: workgroup b runs on XCD b mod 8, which owns the (b mod 8)-th
-  // contiguous eighth of the tiles; its workgroups take them in turn, so the
-  // tiles in flight on one XCD are neighbours (their windows overlap in L2)
   const int nb = gridDim.x, b = blockIdx.x;
-  long long t0, t1, tstep;
+  PprWalk wk;
+  wk.wave = wave;
   if (nb % 8 == 0) {
     const long long per = (a.ntiles + 7) / 8;
-    t0 = min(a.ntiles, (long long)(b % 8) * per) + b / 8;
-    t1 = min(a.ntiles, (long long)(b % 8 + 1) * per);
-    tstep = nb / 8;
+    wk.t0 = min(a.ntiles, (long long)(b % 8) * per) + b / 8;
+    wk.t1 = min(a.ntiles, (long long)(b % 8 + 1) * per);
+    wk.tstep = nb / 8;
   } else {
-    t0 = b;
-    t1 = a.ntiles;
-    tstep = nb;
+    wk.t0 = b;
+    wk.t1 = a.ntiles;
+    wk.tstep = nb;
   }
-  for (long long t = t0; t < t1; t += tstep) {
-    const long long r0 = t * kPprTile;
-    const long long rend = min(a.n, r0 + kPprTile);
-    const long long wbase = r0 + a.dlo;
-    // staged rows [wlo, whi), empty when the window lies outside the block
-    const int wlo = (int)min(max(wbase, 0LL), a.n);
-    const int whi = (int)max((long long)wlo, min(wbase + kPprWin, a.n));
-    // the wave's first row: entries and old X, then its remote X rows, all
-    // issued before the window is staged
-    PprPref A, B;
-    const int rfirst = (int)(r0 + wave);
-    const bool any = rfirst < rend;
-    A.left = 0;
-    if (any) ppr_load_row(a, rfirst, lane, A.w);
-    if (any)
-      A.left = ppr_remote_issue(a, A.w.col, A.w.val, lane < A.w.len, wlo, whi, lane, A.xr, A.vr);
-    __syncthreads();  // the previous tile's rows are done with the window
-    // stage X rows [wlo, whi): 16-byte pieces, in halves of kHalf per thread
-    // (their loads in flight together)
-    {
-      const int pieces = (whi - wlo) * (kWave / 2);
-      const double2 *srcp = reinterpret_cast<const double2 *>(a.X + (size_t)wlo * kWave);
+  const long long my_tiles = wk.t0 < wk.t1 ? (wk.t1 - wk.t0 + wk.tstep - 1) / wk.tstep : 0;
+  const int nq = (int)(my_tiles * kPprRowsPerWave);
+  PprSlot<S> st[NS];
+  ppr_prologue<NS, DE, DR, S, 0>(a, wk, st, lane);
+  for (int q0 = 0; q0 < nq; q0 += NS) {
+    if (q0 % kPprRowsPerWave == 0) {
+      const long long t = wk.t0 + (long long)(q0 / kPprRowsPerWave) * wk.tstep;
+      const PprWindow w = ppr_window(a, t * kPprTile);
+      __syncthreads();  // the previous tile's rows are done with the window
+      // stage X rows [wlo, whi): 16-byte pieces, in halves of kHalf per
+      // thread (their loads in flight together)
+      const int pieces = (w.whi - w.wlo) * (kWave / 2);
+      const double2 *srcp = reinterpret_cast<const double2 *>(a.X + (size_t)w.wlo * kWave);
       const double2 *safe = reinterpret_cast<const double2 *>(a.X);
-      double2 *dst = reinterpret_cast<double2 *>(win + (size_t)(wlo - wbase) * kWave);
+      double2 *dst = reinterpret_cast<double2 *>(win + (size_t)(w.wlo - w.wbase) * kWave);
       constexpr int kHalf = kPprWin * (kWave / 2) / kPprBlock / 2;  // 6 pieces
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         double2 tmp[kHalf];
 #pragma unroll
-        for (int q = 0; q < kHalf; q++) {
-          const int i = (int)threadIdx.x + (h * kHalf + q) * kPprBlock;
-          tmp[q] = *(i < pieces ? srcp + i : safe);
+        for (int p = 0; p < kHalf; p++) {
+          const int i = (int)threadIdx.x + (h * kHalf + p) * kPprBlock;
+          tmp[p] = *(i < pieces ? srcp + i : safe);
         }
 #pragma unroll
-        for (int q = 0; q < kHalf; q++) {
-          const int i = (int)threadIdx.x + (h * kHalf + q) * kPprBlock;
-          if (i < pieces) dst[i] = tmp[q];
+        for (int p = 0; p < kHalf; p++) {
+          const int i = (int)threadIdx.x + (h * kHalf + p) * kPprBlock;
+          if (i < pieces) dst[i] = tmp[p];
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int r = rfirst; r < rend; r += 2 * kPprWaves) {
-      ppr_row(a, A, B, r, rend, wlo, whi, (int)wbase, win, lane, src_l, S_l, tele, e, sm);
-      if (r + kPprWaves < rend)
-        ppr_row(a, B, A, r + kPprWaves, rend, wlo, whi, (int)wbase, win, lane, src_l, S_l, tele, e,
-                sm);
-    }
+    ppr_group_step<NS, DE, DR, S, 0>(a, wk, q0, st, win, lane, src_l, S_l, tele, e, sm);
   }
   // per-column partials of the workgroup: waves in order
   __syncthreads();
@@ -408,9 +548,9 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_tile(PprArgs a) {
   __syncthreads();
   if (threadIdx.x < kWave) {
     double te = 0.0, ts = 0.0;
-    for (int w = 0; w < kPprWaves; w++) {
-      te += win[w * kWave + lane];
-      ts += win[(kPprWaves + w) * kWave + lane];
+    for (int v = 0; v < kPprWaves; v++) {
+      te += win[v * kWave + lane];
+      ts += win[(kPprWaves + v) * kWave + lane];
     }
     a.p_err[(size_t)b * kWave + lane] = te;
     a.p_sum[(size_t)b * kWave + lane] = ts;
@@ -571,6 +711,8 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   hipStream_t s = static_cast<hipStream_t>(stream);
   const long long n = m->ln_row;
   if (n < 1) return fail(TBSLA_HIP_ERR_INVALID, "empty block");
+  if (!m->rowptr) return fail(TBSLA_HIP_ERR_INVALID, "the block has no row pointer");
+  if (m->stored > INT_MAX) return fail(TBSLA_HIP_ERR_RANGE, "more than 2^31 - 1 entries");
   // tile workgroups: three per CU (48 KB of LDS each, 12 wavefronts) on 256 CUs, a
   // multiple of 8 (one contiguous eighth of the tiles per XCD), fewer when
   // the block has fewer tiles
@@ -634,8 +776,12 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   pa.n = n;
   pa.f_col = m->f_col;
   pa.rowptr = m->rowptr;
-  pa.colidx = m->colidx;
-  pa.vals = m->values;
+  // the kernel reads entry min(j, last) whatever the row's length: with no
+  // entries stored it reads a dummy one (src[0], S[0]; never summed)
+  const bool any = m->stored > 0 && m->colidx && m->values;
+  pa.colidx = any ? m->colidx : src;
+  pa.vals = any ? m->values : S;
+  pa.last = any ? (int)(m->stored - 1) : 0;
   pa.src = src;
   pa.S = S;
   pa.beta = beta;
@@ -650,7 +796,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     for (int i = 0; i < batch; i++, launched++) {
       pa.X = Xb[launched % 2];
       pa.Xn = Xb[(launched + 1) % 2];
-      k_ppr_tile<<<nb, kPprBlock, 0, s>>>(pa);
+      k_ppr_pipe<4, 2, 1, 6><<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     hipError_t e = hipGetLastError();
