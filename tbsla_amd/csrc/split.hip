@@ -155,11 +155,14 @@ __global__ __launch_bounds__(kBlock) void k_diag_hist(long long n, long long ste
 
 // y[r] = sum over the stored diagonals k (ascending offset) of
 // dv[k n + r] x[r + doff[k]] (zero slots skipped) + R[r] (when add is not
-// null).  The diagonals go in chunks of eight whose loads are issued
-// together: a last chunk's missing diagonals reread the last one and add
-// nothing.  x is read at the clamped column for every slot (in bounds), and
-// a zero slot's product is not added, so an Inf or NaN in x reaches only
-// the rows that hold an entry on its column.
+// null).  A wavefront takes two tiles of kWave rows, each lane two adjacent
+// rows of one of them, so every load of a diagonal's values is 16 bytes per
+// lane (one 512-byte piece per half-wavefront).  The diagonals go in chunks
+// of eight whose loads are issued together: a last chunk's missing
+// diagonals reread the last one and add nothing.  x is read at the clamped
+// column for every slot (in bounds), and a zero slot's product is not
+// added, so an Inf or NaN in x reaches only the rows that hold an entry on
+// its column.
 __global__ __launch_bounds__(kBlock) void k_dia_rows(long long n, long long ncol, int ndiag,
                                                      const int *__restrict__ doff,
                                                      const double *__restrict__ dv,
@@ -169,22 +172,44 @@ __global__ __launch_bounds__(kBlock) void k_dia_rows(long long n, long long ncol
                                                      const int *__restrict__ done) {
   if (done && *done) return;
   constexpr int U = 8;
-  for (long long r = (long long)blockIdx.x * kBlock + threadIdx.x; r < n;
-       r += (long long)gridDim.x * kBlock) {
-    double sum = 0.0;
+  // lane k holds diagonal k's offset (ndiag <= kWave): read by readlane,
+  // so no scalar load waits inside the loop
+  const int lane = (int)threadIdx.x & (kWave - 1);
+  const int od = lane < ndiag ? doff[lane] : 0;
+  const long long last = ncol - 1;
+  const long long ntiles = (n + kWave - 1) / kWave;
+  const long long npairs = (ntiles + 1) / 2;
+  const long long wstep = (long long)gridDim.x * (kBlock / kWave);
+  for (long long pr = (long long)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; pr < npairs;
+       pr += wstep) {
+    const long long t = 2 * pr + lane / (kWave / 2);  // this lane's tile
+    const bool tile_ok = t < ntiles;
+    const long long r = t * kWave + 2 * (lane % (kWave / 2));  // rows r, r + 1
+    // rows r, r + 1 on diagonal k: 16 bytes at tile + k kWave (a missing
+    // second tile reads tile 0 and stores nothing)
+    const double *tile = dv + (tile_ok ? t : 0) * kWave * (long long)ndiag + 2 * (lane % (kWave / 2));
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    double s0 = 0.0, s1 = 0.0;
     for (int k = 0; k < ndiag; k += U) {
-      double v[U], xv[U];
+      v2 v[U];
+      double x0[U], x1[U];
 #pragma unroll
       for (int i = 0; i < U; i++) {
         const int kk = min(k + i, ndiag - 1);
-        v[i] = __builtin_nontemporal_load(dv + dia_at(r, kk, ndiag));
-        xv[i] = x[min(max(r + doff[kk], 0LL), ncol - 1)];
+        const int o = __builtin_amdgcn_readlane(od, kk);
+        v[i] = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(tile + kk * kWave));
+        x0[i] = x[min(max(r + o, 0LL), last)];
+        x1[i] = x[min(max(r + 1 + o, 0LL), last)];
       }
 #pragma unroll
-      for (int i = 0; i < U; i++)
-        sum = (k + i < ndiag && v[i] != 0.0) ? fma(v[i], xv[i], sum) : sum;
+      for (int i = 0; i < U; i++) {
+        const bool on = k + i < ndiag;
+        s0 = (on && v[i].x != 0.0) ? fma(v[i].x, x0[i], s0) : s0;
+        s1 = (on && v[i].y != 0.0) ? fma(v[i].y, x1[i], s1) : s1;
+      }
     }
-    y[r] = add ? sum + add[r] : sum;
+    if (tile_ok && r < n) y[r] = add ? s0 + add[r] : s0;
+    if (tile_ok && r + 1 < n) y[r + 1] = add ? s1 + add[r + 1] : s1;
   }
 }
 
@@ -389,7 +414,7 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
     if (sp->nr > 0)
       k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(m->ln_row, sp->slices, sp->nblk,
                                                              sp->boff, sp->sr, sp->P, sp->R, done);
-    k_dia_rows<<<sgrid(m->ln_row), kBlock, 0, s>>>(m->ln_row, std::max(m->ln_col, 1), sp->ndiag,
+    k_dia_rows<<<sgrid(m->ln_row / 2), kBlock, 0, s>>>(m->ln_row, std::max(m->ln_col, 1), sp->ndiag,
                                                    sp->doff, sp->dv, x,
                                                    sp->nr > 0 ? sp->R : nullptr, y, done);
     TBSLA_HIP_LAUNCH_CHECK();
