@@ -67,6 +67,11 @@ def measured_traffic(kernel: str):
     return None
 
 
+def step_done(what: str) -> None:
+    """One progress line on stderr as each section ends (stdout keeps the JSON line)."""
+    print(f"[bench] {what} done", file=sys.stderr, flush=True)
+
+
 def csr_kernel_name(k: dict, layout: int) -> str:
     """The kernels of one SpMV as rocprofv3 names them: the row kernel with
     its template arguments (k_csr_rows<vec, lanes, rows, nontemporal,
@@ -444,6 +449,9 @@ def main():
                     help="keep the default CSR schedule (row layout) instead of the autotuned one")
     ap.add_argument("--csr-kernel", default=None, metavar="VEC,LANES,ROWS,NT",
                     help="use this CSR row kernel (row layout) instead of the autotuned one")
+    ap.add_argument("--csr-layout", default=None, metavar="split|diag,SLICES",
+                    help="a fixed derived layout instead of the autotuner's choice (with "
+                         "--csr-kernel, that row schedule)")
     ap.add_argument("--no-formats", action="store_true",
                     help="skip the other formats, grid blocks, PageRank, attention and copy "
                          "figures")
@@ -471,9 +479,14 @@ def main():
     dy = T.DeviceBuffer(N * 8)
     # schedule and layout from the autotuner (untimed)
     t0 = time.time()
-    if args.csr_kernel:
-        vec, lanes, rows, nt = (int(t) for t in args.csr_kernel.split(","))
-        m.set_csr_kernel(lanes, rows, vec, nt)
+    if args.csr_kernel or args.csr_layout:
+        if args.csr_kernel:
+            vec, lanes, rows, nt = (int(t) for t in args.csr_kernel.split(","))
+            m.set_csr_kernel(lanes, rows, vec, nt)
+        if args.csr_layout:
+            name, slices = args.csr_layout.split(",")
+            kind = {"split": T.CSR_SPLIT, "diag": T.CSR_DIAG}[name]
+            m.set_csr_layout(kind, 65536 if kind == T.CSR_DIAG else 64, int(slices))
     elif not args.no_autotune:
         m.autotune(dy, dx, 5)
     t_tune = time.time() - t0
@@ -498,21 +511,31 @@ def main():
     traffic = measured_traffic(roof["kernel"])
     if traffic is not None:
         roof["traffic"] = traffic
+    step_done("csr spmv (%.1f GB/s)" % gbs)
     cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10,
                                                 cpu_threads(args.cpu_threads))
+    if cpu is not None:
+        step_done("cpu baseline")
     sched = m.get_csr_kernel()
     del m, dx, dy
     cg = None if args.no_cg else cg_section(args)
+    if cg is not None:
+        step_done("cg")
     extra = {}
     if not args.no_formats:
         extra["cfg2_other"] = formats_section()
+        step_done("formats")
         extra["attention"] = attention_section()
+        step_done("attention")
     if not args.no_batched:
         from tbsla_amd import batched
         extra["batched_cg"] = batched.bench(args.batched_systems)
+        step_done("batched cg")
         extra["batched_page_rank"] = batched.page_rank_bench(args.batched_systems)
+        step_done("batched page rank")
     if not args.no_ppr:
         extra["ppr_spmm"] = ppr_section()
+        step_done("personalised page rank")
     line = {"metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
