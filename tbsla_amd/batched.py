@@ -141,7 +141,9 @@ def bench(count: int = 512, seed: int = 0, tol: float = 1e-8, log=None) -> dict:
            "batch_ms": round(ms, 3), "systems_per_s": round(count / (ms * 1e-3), 1),
            "per_system_us_p50": round(float(np.percentile(t, 50)), 1),
            "per_system_us_p99": round(float(np.percentile(t, 99)), 1),
-           "iterations_p50": float(np.median(it)), "iterations_max": int(it.max()),
+           "per_system_us_max": round(float(t.max()), 1),
+           "iterations_p50": float(np.median(it)),
+           "iterations_p99": float(np.percentile(it, 99)), "iterations_max": int(it.max()),
            "converged": int(conv.sum()), "max_true_rel_res": float(rel.max())}
     if log:
         log(json.dumps(out))
