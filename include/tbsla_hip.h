@@ -497,6 +497,11 @@ int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *v
  * of the rows (up to 64, the fullest) are stored densely, tiled by 64 rows
  * (8 bytes a slot, no column indices); every other entry -- off those
  * diagonals, explicit zeros, a second entry on a row's slot -- is remote.
+ * The diagonal layout multiplies its remote entries through LDS: column
+ * slices of 16384 (x's piece of 128 KB held by one workgroup), products
+ * summed per row in CSR order inside the diagonal kernel, whenever every
+ * block of 64..4096 rows holds at most 14336 remote entries (slices is then
+ * unused); otherwise through L2 slices as the split layout.
  * Inf and NaN in x reach the same rows as in the row-major sum.  Both copy
  * the entries once, are rebuilt by normalize_cols and dropped when the
  * arrays change; a block without columns cannot take the diagonal layout. */
@@ -508,7 +513,8 @@ int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int
 /* Autotuner (no reference counterpart): times every compiled CSR row
  * kernel (tbsla_hip_set_csr_kernel: lanes x rows x vec x nontemporal) on the
  * row layout and on the split layout (window 64, 8 and 16 slices), and the
- * diagonal layout (window 65536, 8 and 16 slices), on this block with d_v /
+ * diagonal layout (window 65536; 8 and 16 slices unless its LDS form is
+ * taken), on this block with d_v /
  * d_r (`iters` launches each; the three fastest row kernels are timed again
  * with 3 x `iters` launches before the choice) and keeps the fastest; a
  * split or diagonal layout is kept only when it is at least 3 % faster than

@@ -557,7 +557,7 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
     const float ell_ms = ms;
     int best_slices = 0, best_layout = 0, best_window = 0;
     for (int layout : {TBSLA_HIP_CSR_SPLIT, TBSLA_HIP_CSR_DIAG})
-      for (int slices : {8, 16}) {
+      for (int slices : {16, 8}) {
         if (layout == TBSLA_HIP_CSR_DIAG && m->ln_col == 0) break;
         const int window = layout == TBSLA_HIP_CSR_DIAG ? kDiagWindow : 64;
         TBSLA_TRY(split_build(m, window, slices, s, layout));
@@ -569,6 +569,8 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
           best_layout = layout;
           best_window = window;
         }
+        // the LDS-slice form does not depend on the slice count
+        if (layout == TBSLA_HIP_CSR_DIAG && split_lds_block_rows(m) > 0) break;
       }
     split_release(m);
     if (best_slices) TBSLA_TRY(split_build(m, best_window, best_slices, s, best_layout));
@@ -643,14 +645,16 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
     if (remote == 0) break;
     TBSLA_TRY(sweep(TBSLA_HIP_CSR_SPLIT, kWindow, slices));
   }
-  // the diagonal layout: its own kernel, one timing per slice count
-  for (int slices : {8, 16}) {
+  // the diagonal layout: its own kernels, one timing per slice count (one
+  // only for the LDS-slice form, which does not depend on it)
+  for (int slices : {16, 8}) {
     if (!m->has_arrays || m->ln_row == 0 || m->ln_col == 0) break;
     TBSLA_TRY(split_build(m, kDiagWindow, slices, s, TBSLA_HIP_CSR_DIAG));
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
     if (derived_wins(ms))
       best = Best{TBSLA_HIP_CSR_DIAG, kDiagWindow, slices, best.L, best.R, best.V, best.NT, ms};
+    if (split_lds_block_rows(m) > 0) break;
   }
   split_release(m);
   if (best.layout == TBSLA_HIP_CSR_SPLIT || best.layout == TBSLA_HIP_CSR_DIAG)

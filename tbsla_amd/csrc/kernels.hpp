@@ -47,6 +47,9 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
 int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s,
                 int layout = TBSLA_HIP_CSR_SPLIT);
 int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *layout);
+// rows per block of the diagonal layout's LDS-slice remote form, or 0 (no
+// layout, no remote entries, or the L2-slice form, whose slice count matters)
+int split_lds_block_rows(const tbsla_hip_matrix_s *m);
 // The diagonal layout's arrays when it holds every entry (no remote part):
 // offsets, tiled values (dia_at), count, values' length, and the build's
 // id (unique in the process: a copy made from one build is stale for any

@@ -5,15 +5,34 @@
 // rows reuse, but entries far from it (e.g. cqmat's rewired columns,
 // MatrixCSR.cpp:373-486 with q > 0) gather uniformly over all of x, which is
 // several times one XCD's 4 MB L2; every such gather then pulls a whole
-// 128-byte line from the MALL or HBM for 8 useful bytes.  Here the remote
-// entries are sorted by (column slice, row) and multiplied by a kernel whose
-// workgroups for slice s all run on XCD s mod 8 (workgroups are dispatched
-// to the eight XCDs round-robin), so each XCD keeps its slice of x in its
-// own L2.  The products go to an array P in that order.  A combine kernel
-// then gives each block of kCombineRows rows one workgroup, which reads that
-// block's contiguous run of P in every slice (slice by slice, in order) and
-// writes the row totals R[row]; the row kernel streams the near-diagonal CSR
-// and stores y[row] = its sum + R[row].  Every read of P is sequential.
+// 128-byte line from the MALL or HBM for 8 useful bytes.
+//
+// Two forms of the remote part:
+//
+// * LDS slices (the diagonal layout, whenever every block of rows fits):
+//   the remote entries are grouped by column slice of kSliceCols columns
+//   (128 KB of x, held in one workgroup's LDS) and, within a slice, by
+//   block of H rows.  k_slice_products gives each slice a workgroup that
+//   loads its piece of x into LDS once and multiplies the slice's entries
+//   (2-byte column offsets and row-block indices), gathering x from LDS; it
+//   writes the products
+//   to P ordered by (row block, slice, row), so each row block's products
+//   are one contiguous run.  k_dia_rows_remote gives each row block one
+//   workgroup, which scatters the block's products into LDS in row-major
+//   order (each carries its 2-byte position), sums each row's products in
+//   its CSR order and adds the total to the row's diagonal sum.  No x
+//   gather leaves LDS, and no row total goes through HBM.
+// * L2 slices (the split layout, and the diagonal layout's fallback): the
+//   remote entries are sorted by (column slice, row) and multiplied by a
+//   kernel whose workgroups for slice s all run on XCD s mod 8 (workgroups
+//   are dispatched to the eight XCDs round-robin), so each XCD keeps its
+//   slice of x in its own L2.  The products go to an array P in that
+//   order.  A combine kernel then gives each block of kCombineRows rows one
+//   workgroup, which reads that block's contiguous run of P in every slice
+//   (slice by slice, in order) and writes the row totals R[row]; the row
+//   kernel streams the near-diagonal entries and stores y[row] = its sum +
+//   R[row].
+//
 // Every sum is taken in a fixed order, so results are deterministic; they
 // differ from the row-major CSR sum only by rounding.
 //
@@ -24,6 +43,14 @@
 //   P[nr], R[ln_row]                   products and row totals, rewritten
 //                                      every SpMV
 //   doff[ndiag], dv[tiles]             diagonal layout: stored diagonals
+//   c16[nr], q16[nr], sv[nr]           LDS slices: column offset in the slice,
+//                                      row block and value, (slice, row) order
+//   sst[S R + 1], pst[S R]             LDS slices: run (slice f, block q) at
+//                                      sst[f R + q] (entries), pst[f R + q] (P)
+//   d16[nr]                            LDS slices, P order: position of the
+//                                      product in its block's row-major order
+//   pb[R + 1], rs16[R (H + 1)]         LDS slices: first product of each row
+//                                      block; row starts within the block
 #include <algorithm>
 #include <atomic>
 #include <vector>
@@ -36,6 +63,9 @@ namespace tbsla_hip {
 
 constexpr int kCombineRows = 1024;  // rows per combine workgroup (LDS: 8 KB)
 constexpr int kMaxDiagWindow = 1 << 20;  // diagonal layout: offsets searched
+constexpr int kSliceCols = 16384;  // LDS slices: columns per slice (128 KB of x)
+constexpr int kBlockCap = 14336;   // LDS slices: products per row block (112 KB)
+constexpr int kBigBlock = 1024;    // threads of the LDS-slice kernels
 
 struct SplitCsr {
   int layout = 0, window = 0, slices = 0;
@@ -47,8 +77,20 @@ struct SplitCsr {
   double *lv = nullptr, *sv = nullptr, *P = nullptr, *R = nullptr;
   double *dv = nullptr;  // dv[k n + r]: row r's entry on diagonal k, or 0
   std::vector<long long> soff;  // slice boundaries (host)
+  // LDS slices (lds_h > 0): rows per block H, slices S, blocks R, workgroups
+  // per slice G
+  int lds_h = 0, lds_s = 0, lds_r = 0, lds_g = 0;
+  unsigned short *c16 = nullptr, *q16 = nullptr, *d16 = nullptr, *rs16 = nullptr;
+  int *sst = nullptr, *pst = nullptr, *pb = nullptr;
   ~SplitCsr() {
     hipDeviceSynchronize();
+    hipFree(c16);
+    hipFree(q16);
+    hipFree(d16);
+    hipFree(rs16);
+    hipFree(sst);
+    hipFree(pst);
+    hipFree(pb);
     hipFree(lrp);
     hipFree(lci);
     hipFree(lv);
@@ -214,9 +256,11 @@ __global__ __launch_bounds__(kBlock) void k_dia_rows(long long n, long long ncol
 }
 
 // local entries in row order; remote entries in row-major order with their
-// (slice, row) sort key
+// (slice, row) sort key: slice c / slice_cols when slice_cols > 0 (LDS
+// slices), else c slices / n_col
 __global__ __launch_bounds__(kBlock) void k_split_fill(
-    long long n, long long n_col, int f_col, int window, int slices, int rb, RowSrc src,
+    long long n, long long n_col, int f_col, int window, int slices, int slice_cols, int rb,
+    RowSrc src,
     const int *__restrict__ ci, const double *__restrict__ v, const int *__restrict__ dmap,
     int ndiag, const int *__restrict__ lrp, const int *__restrict__ rptr, int *__restrict__ lci,
     double *__restrict__ lv, double *__restrict__ dv, int *__restrict__ rc,
@@ -240,7 +284,8 @@ __global__ __launch_bounds__(kBlock) void k_split_fill(
         lv[pl] = val;
         pl++;
       } else {
-        const unsigned long long s = (unsigned long long)(c * slices / n_col);
+        const unsigned long long s =
+            (unsigned long long)(slice_cols > 0 ? c / slice_cols : c * slices / n_col);
         rc[pr] = (int)c;
         rv[pr] = val;
         keys[pr] = (s << rb) | (unsigned long long)r;
@@ -353,6 +398,248 @@ __global__ __launch_bounds__(kBlock) void k_remote_products(const long long *__r
   }
 }
 
+// ---- LDS slices ------------------------------------------------------------
+
+// out[i] = rp[min(i h, n)] for i in [0, nb]: the first remote entry of each
+// block of h rows (rp: row-major remote offsets, n + 1 values)
+__global__ __launch_bounds__(kBlock) void k_block_starts(long long n, int h, long long nb,
+                                                         const int *__restrict__ rp,
+                                                         int *__restrict__ out) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i <= nb;
+       i += (long long)gridDim.x * kBlock)
+    out[i] = rp[min(i * h, n)];
+}
+
+// (slice, row) order: the column's offset in its slice, the row's block of
+// h rows and the value
+__global__ __launch_bounds__(kBlock) void k_slice_place(long long nr,
+                                                        const unsigned long long *__restrict__ keys,
+                                                        int rb, int h, const int *__restrict__ payload,
+                                                        const int *__restrict__ rc,
+                                                        const double *__restrict__ rv,
+                                                        unsigned short *__restrict__ c16,
+                                                        unsigned short *__restrict__ q16,
+                                                        double *__restrict__ sv) {
+  const unsigned long long mask = (1ULL << rb) - 1;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < nr;
+       e += (long long)gridDim.x * kBlock) {
+    const int j = payload[e];
+    const unsigned long long k = keys[e];
+    const long long f = (long long)(k >> rb);
+    c16[e] = (unsigned short)(rc[j] - f * kSliceCols);
+    q16[e] = (unsigned short)((long long)(k & mask) / h);
+    sv[e] = rv[j];
+  }
+}
+
+// sst[f R + q] = the first entry (in (slice, row) order) of slice f whose row
+// is at least q h; sst[S R] = nr
+__global__ __launch_bounds__(kBlock) void k_slice_starts(long long nr, int rb, int S, int R, int h,
+                                                         const unsigned long long *__restrict__ keys,
+                                                         int *__restrict__ sst) {
+  const long long total = (long long)S * R;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i <= total;
+       i += (long long)gridDim.x * kBlock) {
+    if (i == total) {
+      sst[i] = (int)nr;
+      continue;
+    }
+    const unsigned long long want =
+        ((unsigned long long)(i / R) << rb) | (unsigned long long)((i % R) * (long long)h);
+    long long lo = 0, hi = nr;
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (keys[mid] < want) lo = mid + 1;
+      else hi = mid;
+    }
+    sst[i] = (int)lo;
+  }
+}
+
+// pst[f R + q]: where run (slice f, block q) starts in P, whose order is
+// (block, slice, row): block q's runs follow each other in slice order from
+// pb[q]
+__global__ __launch_bounds__(kBlock) void k_slice_pstarts(int S, int R, const int *__restrict__ sst,
+                                                          const int *__restrict__ pb,
+                                                          int *__restrict__ pst) {
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < R; q += gridDim.x * kBlock) {
+    int acc = pb[q];
+    for (int f = 0; f < S; f++) {
+      const long long run = (long long)f * R + q;
+      pst[run] = acc;
+      acc += sst[run + 1] - sst[run];
+    }
+  }
+}
+
+// d16, in P order: the product's position in its block's row-major order
+// (payload = the entry's row-major remote index)
+__global__ __launch_bounds__(kBlock) void k_slice_dest(long long nr, int rb, int R, int h,
+                                                       const unsigned long long *__restrict__ keys,
+                                                       const int *__restrict__ payload,
+                                                       const int *__restrict__ sst,
+                                                       const int *__restrict__ pst,
+                                                       const int *__restrict__ pb,
+                                                       unsigned short *__restrict__ d16) {
+  const unsigned long long mask = (1ULL << rb) - 1;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < nr;
+       e += (long long)gridDim.x * kBlock) {
+    const unsigned long long k = keys[e];
+    const long long f = (long long)(k >> rb);
+    const int q = (int)((long long)(k & mask) / h);
+    const long long run = f * R + q;
+    const long long p = pst[run] + (e - sst[run]);
+    d16[p] = (unsigned short)(payload[e] - pb[q]);
+  }
+}
+
+// rs16[q (h + 1) + i] = the first product of row q h + i within block q
+// (rows past n: the block's end)
+__global__ __launch_bounds__(kBlock) void k_row_starts16(long long n, int h, int R,
+                                                         const int *__restrict__ rp,
+                                                         const int *__restrict__ pb,
+                                                         unsigned short *__restrict__ rs16) {
+  const long long total = (long long)R * (h + 1);
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kBlock) {
+    const long long q = i / (h + 1), j = i % (h + 1);
+    rs16[i] = (unsigned short)(rp[min(q * h + j, n)] - pb[q]);
+  }
+}
+
+// P[pst[run] + j] = sv[sst[run] + j] x[f kSliceCols + c16[sst[run] + j]] for
+// the runs (f, q) of slice f = blockIdx / G over the blocks q of group
+// blockIdx % G (at most kRunsCap).  Those runs are one contiguous range of
+// entries, which the workgroup streams with every load independent of the
+// run tables (kEntriesInFlight entries a lane, a wavefront's loads
+// contiguous); each entry carries its block q (q16), whose P offset the
+// workgroup holds in LDS with the slice's piece of x.
+constexpr int kRunsCap = 2048;
+constexpr int kEntriesInFlight = 8;
+__global__ __launch_bounds__(kBigBlock) void k_slice_products(
+    long long ncol, int S, int R, int G, const int *__restrict__ sst, const int *__restrict__ pst,
+    const unsigned short *__restrict__ c16, const unsigned short *__restrict__ q16,
+    const double *__restrict__ sv, const double *__restrict__ x, double *__restrict__ P,
+    const int *__restrict__ done) {
+  __shared__ double xs[kSliceCols];
+  __shared__ int dl[kRunsCap];  // P position - entry position, per run
+  if (done && *done) return;
+  const int f = (int)(blockIdx.x / G), g = (int)(blockIdx.x % G);
+  if (f >= S) return;
+  const int q0 = (int)((long long)R * g / G), q1 = (int)((long long)R * (g + 1) / G);
+  const int nq = q1 - q0;  // <= kRunsCap (G is chosen so)
+  const long long run0 = (long long)f * R + q0;
+  const long long c0 = (long long)f * kSliceCols;
+  constexpr int UX = kSliceCols / kBigBlock;
+  double t[UX];
+#pragma unroll
+  for (int i = 0; i < UX; i++) {
+    const long long c = c0 + threadIdx.x + (long long)i * kBigBlock;
+    t[i] = c < ncol ? x[c] : 0.0;
+  }
+  for (int i = (int)threadIdx.x; i < nq; i += kBigBlock) dl[i] = pst[run0 + i] - sst[run0 + i];
+#pragma unroll
+  for (int i = 0; i < UX; i++) xs[threadIdx.x + i * kBigBlock] = t[i];
+  const int e0 = sst[run0], e1 = sst[run0 + nq];
+  __syncthreads();
+  constexpr int K = kEntriesInFlight;
+  for (int base = e0 + (int)threadIdx.x; base < e1; base += kBigBlock * K) {
+    int c[K], q[K];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = base + k * kBigBlock;
+      const bool in = e < e1;
+      c[k] = in ? (int)c16[e] : 0;
+      q[k] = in ? (int)q16[e] - q0 : 0;
+      v[k] = in ? sv[e] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = base + k * kBigBlock;
+      if (e < e1) P[e + dl[q[k]]] = v[k] * xs[c[k]];
+    }
+  }
+}
+
+// y[r] = the diagonal sum of row r (as k_dia_rows) + the sum of its remote
+// products, for the h rows of block blockIdx.  The block's products
+// P[pb[q] .. pb[q + 1]) are first scattered into LDS in row-major order
+// (d16); each row then sums its own run in CSR order.  Sixteen wavefronts
+// take 2048 rows per pass, two tiles of kWave rows each, a lane two
+// adjacent rows.
+__global__ __launch_bounds__(kBigBlock) void k_dia_rows_remote(
+    long long n, long long ncol, int ndiag, int h, const int *__restrict__ doff,
+    const double *__restrict__ dv, const double *__restrict__ x, const int *__restrict__ pb,
+    const unsigned short *__restrict__ d16, const unsigned short *__restrict__ rs16,
+    const double *__restrict__ P, double *__restrict__ y, const int *__restrict__ done) {
+  __shared__ double buf[kBlockCap];
+  if (done && *done) return;
+  const long long q = blockIdx.x;
+  const long long r0 = q * h;
+  {
+    const int p0 = pb[q], cnt = pb[q + 1] - p0;
+    constexpr int K = (kBlockCap + kBigBlock - 1) / kBigBlock;
+    double pv[K];
+    int dd[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int i = (int)threadIdx.x + k * kBigBlock;
+      const bool ok = i < cnt;
+      pv[k] = ok ? P[p0 + i] : 0.0;
+      dd[k] = ok ? (int)d16[p0 + i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if ((int)threadIdx.x + k * kBigBlock < cnt) buf[dd[k]] = pv[k];
+  }
+  __syncthreads();
+  constexpr int U = 8;
+  const int lane = (int)threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  const int od = lane < ndiag ? doff[lane] : 0;
+  const long long last = ncol - 1;
+  const long long ntiles = (n + kWave - 1) / kWave;
+  const unsigned short *rs = rs16 + q * (h + 1);
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  for (int pass = 0; pass * 2 * kBigBlock < h; pass++) {
+    // local row (even) of this lane's pair
+    const int lr = pass * 2 * kBigBlock + wave * 2 * kWave + (lane / (kWave / 2)) * kWave +
+                   2 * (lane % (kWave / 2));
+    const long long r = r0 + lr;
+    const long long t = r / kWave;
+    const bool tile_ok = lr < h && t < ntiles;
+    const double *tile = dv + (tile_ok ? t : 0) * kWave * (long long)ndiag + 2 * (lane % (kWave / 2));
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < ndiag; k += U) {
+      v2 v[U];
+      double x0[U], x1[U];
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        const int kk = min(k + i, ndiag - 1);
+        const int o = __builtin_amdgcn_readlane(od, kk);
+        v[i] = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(tile + kk * kWave));
+        x0[i] = x[min(max(r + o, 0LL), last)];
+        x1[i] = x[min(max(r + 1 + o, 0LL), last)];
+      }
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        const bool on = k + i < ndiag;
+        s0 = (on && v[i].x != 0.0) ? fma(v[i].x, x0[i], s0) : s0;
+        s1 = (on && v[i].y != 0.0) ? fma(v[i].y, x1[i], s1) : s1;
+      }
+    }
+    if (tile_ok) {
+      const int a = rs[lr], b = rs[lr + 1], c = rs[lr + 2];
+      double t0 = 0.0, t1 = 0.0;
+      for (int i = a; i < b; i++) t0 += buf[i];
+      for (int i = b; i < c; i++) t1 += buf[i];
+      if (r < n) y[r] = s0 + t0;
+      if (r + 1 < n) y[r + 1] = s1 + t1;
+    }
+  }
+}
+
 unsigned sgrid(long long n) {
   long long g = (n + kBlock - 1) / kBlock;
   return (unsigned)std::max<long long>(1, std::min<long long>(g, 65536));
@@ -373,6 +660,11 @@ int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *lay
   *slices = sp->slices;
   if (layout) *layout = sp->layout;
   return 1;
+}
+
+int split_lds_block_rows(const tbsla_hip_matrix_s *m) {
+  const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  return sp ? sp->lds_h : 0;
 }
 
 int split_diag_only(const tbsla_hip_matrix_s *m, const int **doff, const double **dv, int *ndiag,
@@ -398,6 +690,16 @@ void split_release(tbsla_hip_matrix_s *m) {
 int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
                  hipStream_t s) {
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  if (sp->lds_h > 0) {
+    k_slice_products<<<(unsigned)(sp->lds_s * sp->lds_g), kBigBlock, 0, s>>>(
+        m->ln_col, sp->lds_s, sp->lds_r, sp->lds_g, sp->sst, sp->pst, sp->c16, sp->q16, sp->sv, x,
+        sp->P, done);
+    k_dia_rows_remote<<<(unsigned)sp->lds_r, kBigBlock, 0, s>>>(
+        m->ln_row, std::max(m->ln_col, 1), sp->ndiag, sp->lds_h, sp->doff, sp->dv, x, sp->pb,
+        sp->d16, sp->rs16, sp->P, y, done);
+    TBSLA_HIP_LAUNCH_CHECK();
+    return TBSLA_HIP_SUCCESS;
+  }
   const long long *d_soff = reinterpret_cast<const long long *>(sp->P + std::max(sp->nr, 1LL));
   if (sp->nr > 0) {
     for (int pass = 0; pass * 8 < sp->slices; pass++) {
@@ -430,10 +732,38 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
   return TBSLA_HIP_SUCCESS;
 }
 
+// Rows per block for the LDS-slice form: the largest power of two from 4096
+// down to 64 whose every block of rows holds at most kBlockCap remote
+// entries, or 0 (rp: row-major remote offsets, n + 1 values, on the device)
+static int pick_block_rows(const int *rp, long long n, hipStream_t s, int *h_out) {
+  *h_out = 0;
+  const long long nb = (n + kWave - 1) / kWave;
+  int *d = nullptr;
+  int st;
+  if ((st = device_alloc((void **)&d, (size_t)(nb + 1) * 4))) return st;
+  std::vector<int> hb(nb + 1);
+  k_block_starts<<<sgrid(nb + 1), kBlock, 0, s>>>(n, kWave, nb, rp, d);
+  const bool ok = hipGetLastError() == hipSuccess &&
+                  hipMemcpyAsync(hb.data(), d, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s) ==
+                      hipSuccess &&
+                  hipStreamSynchronize(s) == hipSuccess;
+  hipFree(d);
+  if (!ok) return fail(TBSLA_HIP_ERR_HIP, "diagonal layout: block sizes");
+  for (int h = 4096; h >= kWave && *h_out == 0; h /= 2) {
+    const long long step = h / kWave;
+    bool fits = true;
+    for (long long b = 0; b * step < nb && fits; b++)
+      fits = hb[std::min((b + 1) * step, nb)] - hb[b * step] <= kBlockCap;
+    if (fits) *h_out = h;
+  }
+  return TBSLA_HIP_SUCCESS;
+}
+
 int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, int layout) {
   split_release(m);
   const long long n = m->ln_row, nc = std::max(m->ln_col, 1);
   const bool diag = layout == TBSLA_HIP_CSR_DIAG;
+  const int s_lds = (int)((nc + kSliceCols - 1) / kSliceCols);  // LDS slices of x
   static std::atomic<unsigned long long> builds{0};
   auto *sp = new SplitCsr();
   sp->build_id = ++builds;
@@ -447,6 +777,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
   double *rv = nullptr;
   unsigned long long *keys = nullptr;
   void *ws = nullptr;
+  int tot32[2] = {0, 0};  // rptr[n] and lrp[n], copied from here before the stream syncs
   auto cleanup = [&]() {
     hipStreamSynchronize(s);
     hipFree(lcnt);
@@ -471,9 +802,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
       (st = device_alloc((void **)&rcnt, (size_t)(n + 1) * 4)) ||
       (st = device_alloc((void **)&sp->lrp, (size_t)(n + 1) * 4)) ||
       (st = device_alloc((void **)&rptr, (size_t)(n + 1) * 4)) ||
-      (st = device_alloc((void **)&scount, (size_t)(slices + 1) * 4)) ||
-      (st = device_alloc((void **)&sp->R, (size_t)n * 8)) ||
-      (st = device_alloc((void **)&sp->boff, (size_t)slices * (size_t)(sp->nblk + 1) * 4)) ||
+      (st = device_alloc((void **)&scount, (size_t)(std::max(slices, s_lds) + 1) * 4)) ||
       (st = device_alloc(&ws, scan_workspace_bytes(n + 1) + 64)))
     return failed(st);
   long long *d_total = reinterpret_cast<long long *>(static_cast<char *>(ws) + scan_workspace_bytes(n + 1));
@@ -525,29 +854,73 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     if (hipMemcpyAsync(&tot[k], d_total, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: scan"));
-    const int t = (int)tot[k];
-    hipMemcpyAsync(ptrs[k] + n, &t, 4, hipMemcpyHostToDevice, s);
+    tot32[k] = (int)tot[k];
+    hipMemcpyAsync(ptrs[k] + n, &tot32[k], 4, hipMemcpyHostToDevice, s);
   }
   sp->nl = tot[0];
   sp->nr = tot[1];
+  const long long nr1 = std::max(sp->nr, 1LL);
+  // the diagonal layout's remote entries through LDS slices when every
+  // block of at most 4096 rows fits and the run tables stay small
+  int H = 0;
+  if (diag && sp->nr > 0) {
+    if ((st = pick_block_rows(rptr, n, s, &H))) return failed(st);
+    // block indices are 2-byte
+    const long long nblocks = H > 0 ? (n + H - 1) / H : 0;
+    if (H > 0 && (nblocks > 65536 || (long long)s_lds * nblocks > (1LL << 26))) H = 0;
+  }
   const long long nl_store = diag ? 1 : std::max(sp->nl, 1LL);  // DIA entries live in dv
   if ((st = device_alloc((void **)&sp->lci, (size_t)nl_store * 4)) ||
       (st = device_alloc((void **)&sp->lv, (size_t)nl_store * 8)) ||
-      (st = device_alloc((void **)&sp->sr, (size_t)std::max(sp->nr, 1LL) * 4)) ||
-      (st = device_alloc((void **)&sp->sc, (size_t)std::max(sp->nr, 1LL) * 4)) ||
-      (st = device_alloc((void **)&sp->sv, (size_t)std::max(sp->nr, 1LL) * 8)) ||
-      // P, followed by the device copy of the slice offsets
-      (st = device_alloc((void **)&sp->P, (size_t)std::max(sp->nr, 1LL) * 8 + (size_t)(slices + 1) * 8)) ||
-      (st = device_alloc((void **)&rc, (size_t)std::max(sp->nr, 1LL) * 4)) ||
-      (st = device_alloc((void **)&rv, (size_t)std::max(sp->nr, 1LL) * 8)) ||
-      (st = device_alloc((void **)&keys, (size_t)std::max(sp->nr, 1LL) * 8)) ||
-      (st = device_alloc((void **)&pay, (size_t)std::max(sp->nr, 1LL) * 4)))
+      (st = device_alloc((void **)&sp->sv, (size_t)nr1 * 8)) ||
+      (st = device_alloc((void **)&rc, (size_t)nr1 * 4)) ||
+      (st = device_alloc((void **)&rv, (size_t)nr1 * 8)) ||
+      (st = device_alloc((void **)&keys, (size_t)nr1 * 8)) ||
+      (st = device_alloc((void **)&pay, (size_t)nr1 * 4)))
     return failed(st);
-  hipMemsetAsync(scount, 0, (size_t)(slices + 1) * 4, s);
+  hipMemsetAsync(scount, 0, (size_t)(std::max(slices, s_lds) + 1) * 4, s);
   const int rb = bits_of(std::max(n, 1LL));
-  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, rb, src, m->colidx,
-                                           m->values, dmap, sp->ndiag, sp->lrp, rptr, sp->lci, sp->lv, sp->dv,
-                                           rc, rv, keys, pay, scount);
+  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, H > 0 ? kSliceCols : 0,
+                                           rb, src, m->colidx, m->values, dmap, sp->ndiag, sp->lrp,
+                                           rptr, sp->lci, sp->lv, sp->dv, rc, rv, keys, pay, scount);
+  if (H > 0) {
+    const int S = s_lds, R = (int)((n + H - 1) / H);
+    sp->lds_h = H;
+    sp->lds_s = S;
+    sp->lds_r = R;
+    // at least 256 workgroups, at most kRunsCap blocks each
+    sp->lds_g = std::max(1, std::min(R, std::max((256 + S - 1) / S, (R + kRunsCap - 1) / kRunsCap)));
+    const long long runs = (long long)S * R;
+    if ((st = device_alloc((void **)&sp->c16, (size_t)nr1 * 2)) ||
+        (st = device_alloc((void **)&sp->q16, (size_t)nr1 * 2)) ||
+        (st = device_alloc((void **)&sp->d16, (size_t)nr1 * 2)) ||
+        (st = device_alloc((void **)&sp->P, (size_t)nr1 * 8)) ||
+        (st = device_alloc((void **)&sp->sst, (size_t)(runs + 1) * 4)) ||
+        (st = device_alloc((void **)&sp->pst, (size_t)runs * 4)) ||
+        (st = device_alloc((void **)&sp->pb, (size_t)(R + 1) * 4)) ||
+        (st = device_alloc((void **)&sp->rs16, (size_t)R * (size_t)(H + 1) * 2)))
+      return failed(st);
+    if (sp->nr > 1 && (st = radix_sort_pairs(keys, pay, sp->nr, rb + bits_of(S), s))) return failed(st);
+    k_slice_place<<<sgrid(sp->nr), kBlock, 0, s>>>(sp->nr, keys, rb, H, pay, rc, rv, sp->c16,
+                                                   sp->q16, sp->sv);
+    k_block_starts<<<sgrid(R + 1), kBlock, 0, s>>>(n, H, R, rptr, sp->pb);
+    k_slice_starts<<<sgrid(runs + 1), kBlock, 0, s>>>(sp->nr, rb, S, R, H, keys, sp->sst);
+    k_slice_pstarts<<<sgrid(R), kBlock, 0, s>>>(S, R, sp->sst, sp->pb, sp->pst);
+    k_slice_dest<<<sgrid(sp->nr), kBlock, 0, s>>>(sp->nr, rb, R, H, keys, pay, sp->sst, sp->pst,
+                                                  sp->pb, sp->d16);
+    k_row_starts16<<<sgrid((long long)R * (H + 1)), kBlock, 0, s>>>(n, H, R, rptr, sp->pb, sp->rs16);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return failed(fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e)));
+    cleanup();
+    return TBSLA_HIP_SUCCESS;
+  }
+  if ((st = device_alloc((void **)&sp->R, (size_t)n * 8)) ||
+      (st = device_alloc((void **)&sp->boff, (size_t)slices * (size_t)(sp->nblk + 1) * 4)) ||
+      (st = device_alloc((void **)&sp->sr, (size_t)nr1 * 4)) ||
+      (st = device_alloc((void **)&sp->sc, (size_t)nr1 * 4)) ||
+      // P, followed by the device copy of the slice offsets
+      (st = device_alloc((void **)&sp->P, (size_t)nr1 * 8 + (size_t)(slices + 1) * 8)))
+    return failed(st);
   if (sp->nr > 1 && (st = radix_sort_pairs(keys, pay, sp->nr, rb + bits_of(slices), s))) return failed(st);
   if (sp->nr > 0)
     k_split_place<<<sgrid(sp->nr), kBlock, 0, s>>>(sp->nr, keys, rb, pay, rc, rv, sp->sr, sp->sc,
@@ -558,7 +931,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: slice counts"));
   sp->soff.assign(slices + 1, 0);
   for (int k = 0; k < slices; k++) sp->soff[k + 1] = sp->soff[k] + hc[k];
-  long long *d_soff = reinterpret_cast<long long *>(sp->P + std::max(sp->nr, 1LL));
+  long long *d_soff = reinterpret_cast<long long *>(sp->P + nr1);
   if (hipMemcpyAsync(d_soff, sp->soff.data(), (size_t)(slices + 1) * 8, hipMemcpyHostToDevice, s) != hipSuccess)
     return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: offsets"));
   k_block_offsets<<<sgrid((long long)slices * (sp->nblk + 1)), kBlock, 0, s>>>(

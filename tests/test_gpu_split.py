@@ -68,6 +68,30 @@ def test_diag_spmv_matches_oracle(n, c, q, grid, window, slices):
     assert np.array_equal(g.spmv(x), rows)
 
 
+# The diagonal layout's remote form: LDS slices with 4096-row blocks (n =
+# 100000: 7 column slices, 25 blocks, a short last slice and block); 128-row
+# blocks (64 remote entries a row: 256 rows would hold 16384 > 14336); no
+# block size fits (300 remote entries a row: 64 rows hold 19200), so the
+# L2-slice form; the same matrix as a block of a 3 x 3 grid (f_col > 0).
+@pytest.mark.parametrize("n,c,q,grid", [
+    (100000, 16, 0.1, (0, 0, 1, 1)),
+    (20000, 64, 1.0, (0, 0, 1, 1)),
+    (20000, 300, 1.0, (0, 0, 1, 1)),
+    (100000, 16, 0.1, (1, 2, 3, 3)),
+])
+def test_diag_remote_forms_match_oracle(n, c, q, grid):
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, *grid, value_seed=11)
+    g = T.Matrix(T.CSR).fill_cqmat(n, n, c, q, 0, *grid, value_seed=11)
+    x = O.unit_vector(5, o.geom["ln_col"]) - 0.5
+    g.set_csr_layout(T.CSR_DIAG, 65536, 16)
+    y1, y2 = g.spmv(x), g.spmv(x)
+    assert rel_max(y1, o.spmv(x)) < 1e-12
+    assert np.array_equal(y1, y2)
+    # Ax (MatrixCSR::Ax overwrites its result)
+    r = O.unit_vector(9, o.geom["ln_row"])
+    assert rel_max(g.Ax(r.copy(), x), o.Ax(r.copy(), x)) < 1e-12
+
+
 def test_diag_remote_count_zeros_duplicates_and_nonfinite_x():
     """Rows r of a 3000 x 3000 block: (r, r) = 1 + r/7; (r, r+1) = 0.0 for
     odd r, 2.5 otherwise; a second (r, r) = -0.5 for r % 3 == 0; (r, r+500)

@@ -277,6 +277,24 @@ def test_spmv_config2_sampled_rows(fmt):
         close(y[geo["f_row"]:geo["f_row"] + geo["ln_row"]], blk.spmv(x))
 
 
+@pytest.mark.parametrize("fmt", [T.CSR, T.ELL])
+def test_spmv_config2_diag_layout_sampled_rows(fmt):
+    """The same matrix in the diagonal layout (32 stored diagonals, the
+    13.4 M rewired entries multiplied through LDS column slices): the rows of
+    three oracle row blocks, and the same bits on a second SpMV."""
+    n, c, q = 4194304, 32, 0.1
+    g = T.Matrix(fmt).fill_cqmat(n, n, c, q, 0, value_seed=0)
+    g.set_csr_layout(T.CSR_DIAG, 65536, 16)
+    x = O.unit_vector(1, n)
+    dx = T.DeviceBuffer.from_numpy(x)
+    y = g.spmv(dx)
+    assert np.array_equal(y, g.spmv(dx))
+    for pr in (0, 37, 255):
+        blk = O.fill_cqmat(O.CSR, n, n, c, q, 0, pr, 0, 256, 1, value_seed=0)
+        geo = blk.geom
+        close(y[geo["f_row"]:geo["f_row"] + geo["ln_row"]], blk.spmv(x))
+
+
 def test_dense_slice_of_config2():
     """DENSE block (0,0) of a 256 x 256 grid of the n = 2^22 cqmat:
     16384 x 16384."""
