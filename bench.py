@@ -72,13 +72,19 @@ def step_done(what: str) -> None:
     print(f"[bench] {what} done", file=sys.stderr, flush=True)
 
 
-def csr_kernel_name(k: dict, layout: int) -> str:
+def csr_kernel_name(k: dict, lay: dict) -> str:
     """The kernels of one SpMV as rocprofv3 names them: the row kernel with
     its template arguments (k_csr_rows<vec, lanes, rows, nontemporal,
     split>), or the diagonal kernel, plus the remote kernels of the split
-    and diagonal layouts."""
+    and diagonal layouts (the diagonal layout's LDS-slice form: the slice
+    products and the diagonal kernel that sums them per row)."""
     import tbsla_amd as T
+    layout = lay["layout"]
     if layout == T.CSR_DIAG:
+        if lay.get("lds_block_rows", 0) > 0:
+            return "k_slice_products + k_dia_rows_remote"
+        if lay["remote"] == 0:
+            return "k_dia_rows"
         return "k_dia_rows + k_remote_products + k_remote_combine"
     split = layout == T.CSR_SPLIT
     name = "k_csr_rows<%d, %d, %d, %d, %s>" % (k["vec"], k["lanes"], k["rows"], k["nontemporal"],
@@ -499,7 +505,7 @@ def main():
     gbs = nbytes / (per * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": csr_kernel_name(m.get_csr_kernel(), layout["layout"]),
+            "kernel": csr_kernel_name(m.get_csr_kernel(), layout),
             "bytes_per_launch": nbytes,
             "avg_launch_us": round(per * 1e3, 2)}
     # calibration copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch

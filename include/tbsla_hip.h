@@ -510,6 +510,12 @@ int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int s
                              void *stream);
 int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *window, int *slices,
                              long long *remote_entries);
+/* The remote form of the current layout (no reference counterpart):
+ * *block_rows = the rows per block of the diagonal layout's LDS-slice form
+ * (kernels k_slice_products + k_dia_rows_remote), or 0 when the remote
+ * entries go through L2 slices (k_remote_products + k_remote_combine), the
+ * layout has no remote entries, or there is no derived layout. */
+int tbsla_hip_get_csr_remote_form(tbsla_hip_matrix_t m, int *block_rows);
 /* Autotuner (no reference counterpart): times every compiled CSR row
  * kernel (tbsla_hip_set_csr_kernel: lanes x rows x vec x nontemporal) on the
  * row layout and on the split layout (window 64, 8 and 16 slices), and the

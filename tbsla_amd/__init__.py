@@ -132,6 +132,7 @@ SIGNATURES = [
                                                    C.c_float, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_set_csr_layout", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
     ("tbsla_hip_get_csr_layout", C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_longlong)]),
+    ("tbsla_hip_get_csr_remote_form", C.c_int, [_vp, _ip]),
     ("tbsla_hip_cg_batched", C.c_int, [_vp, C.c_int, _ip, _ip, _vp, _vp, C.c_void_p, C.c_void_p,
                                        _vp]),
     ("tbsla_hip_spmm", C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
@@ -586,7 +587,10 @@ class Matrix:
         lay, w, s, r = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
         check(lib().tbsla_hip_get_csr_layout(self.h, C.byref(lay), C.byref(w), C.byref(s),
                                              C.byref(r)))
-        return {"layout": lay.value, "window": w.value, "slices": s.value, "remote": r.value}
+        h = C.c_int()
+        check(lib().tbsla_hip_get_csr_remote_form(self.h, C.byref(h)))
+        return {"layout": lay.value, "window": w.value, "slices": s.value, "remote": r.value,
+                "lds_block_rows": h.value}
 
     def cg_options(self, max_iterations=10000, tolerance=1e-10, relative=False,
                    precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0,

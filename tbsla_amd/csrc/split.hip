@@ -978,3 +978,9 @@ extern "C" int tbsla_hip_get_csr_layout(tbsla_hip_matrix_t m, int *layout, int *
   if (remote_entries) *remote_entries = sp ? sp->nr : 0;
   return TBSLA_HIP_SUCCESS;
 }
+
+extern "C" int tbsla_hip_get_csr_remote_form(tbsla_hip_matrix_t m, int *block_rows) {
+  if (!m || !block_rows) return fail(TBSLA_HIP_ERR_INVALID, "null handle or output");
+  *block_rows = split_lds_block_rows(m);
+  return TBSLA_HIP_SUCCESS;
+}
