@@ -374,6 +374,13 @@ typedef struct {
                             _DIAGONALS: the diagonals or TBSLA_HIP_ERR_UNSUPPORTED.  Both
                             forms sum G^T's and G's rows in ascending column, so they give
                             bitwise the same iterates */
+  int max_blocks_per_system; /* workgroups of the iteration's kernels per system, at most
+                            (1..1024); 0 picks 1024 for one system and 256 when
+                            tbsla_hip_cg_batched solves more than one.  Every workgroup reads all of its system's
+                            dot-product partials, so a batch, whose systems fill the GPU
+                            together, takes fewer; a system solved alone and in a batch
+                            gives bitwise the same iterates when both use the same value
+                            (and the same lanes_per_row) */
 } tbsla_hip_cg_options;
 enum {
   TBSLA_HIP_FACTOR_AUTO = 0,

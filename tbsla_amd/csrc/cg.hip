@@ -43,6 +43,7 @@ namespace {
 
 constexpr double kNearZero = 1.0e-10;  // CG.cpp:35
 constexpr int kMaxBlocksPerSystem = 1024;
+constexpr int kBatchBlocksPerSystem = 256;  // default for a batch of systems
 
 enum SegDone { kRunning = 0, kConverged = 1, kPending = 2, kBreakdown = 3, kInactive = 5 };
 
@@ -1698,6 +1699,14 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   const bool mixed = o.precision == TBSLA_HIP_CG_MIXED;
   const int max_it = o.max_iterations;
   if (max_it < 0) return fail(TBSLA_HIP_ERR_INVALID, "max_iterations < 0");
+  if (o.max_blocks_per_system < 0 || o.max_blocks_per_system > kMaxBlocksPerSystem)
+    return fail(TBSLA_HIP_ERR_INVALID, "max_blocks_per_system must be 0..1024");
+  // workgroups per system: the dot-product partials every workgroup of a
+  // system reads grow with their square, which a lone system's parallelism
+  // pays for and a batch's does not
+  const long long cap = o.max_blocks_per_system > 0 ? o.max_blocks_per_system
+                        : nsys > 1                  ? kBatchBlocksPerSystem
+                                                    : kMaxBlocksPerSystem;
 
   if (!m->cg_cache) m->cg_cache = take_cache();
   CgCache &c = *static_cast<CgCache *>(m->cg_cache);
@@ -1722,7 +1731,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   const long long pass_rows = kBlock / L;
   for (int k = 0; k < nsys; k++) {
     const long long rows = h_rows[k + 1] - h_rows[k];
-    long long rpb = std::max<long long>(pass_rows, (rows + kMaxBlocksPerSystem - 1) / kMaxBlocksPerSystem);
+    long long rpb = std::max<long long>(pass_rows, (rows + cap - 1) / cap);
     rpb = (rpb + pass_rows - 1) / pass_rows * pass_rows;
     const int nb = (int)((rows + rpb - 1) / rpb);
     CgSeg &S = hs[k];

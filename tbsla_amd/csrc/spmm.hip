@@ -148,17 +148,23 @@ __device__ inline void vset(VT &v, int q, double x) {
 // update (src/mpi/matrix/Matrix.cpp:123-141) with a one-hot teleport per
 // column, on the SpMM of src/mpi/matrix/MatrixCSR.cpp:11-30.
 //
-// A workgroup takes tiles of kPprTile consecutive rows.  For a tile starting
-// at row r0 it stages the X rows [r0 + dlo, r0 + dlo + kPprWin) -- the band
-// the tile's near-diagonal entries read (dlo from a histogram of the
-// entries' offsets) -- in LDS, 512 bytes per row, read from HBM once and
-// then served to every entry of the band at LDS bandwidth.  A wavefront
+// A workgroup takes a contiguous run of tiles of kPprTile consecutive rows.
+// For a tile starting at row r0 the X rows [r0 + dlo, r0 + dlo + kPprWin) --
+// the band the tile's near-diagonal entries read (dlo from a histogram of
+// the entries' offsets) -- are in LDS, 512 bytes per row, served to every
+// entry of the band at LDS bandwidth.  The LDS holds them as a ring of
+// kPprWin rows (X row j in slot j mod kPprWin): the next tile's window is
+// this one shifted by kPprTile rows, so only its last kPprTile rows are new;
+// they are loaded into registers while this tile is summed and written over
+// the departing rows at the tile boundary.  Each X row is read from HBM once
+// per step (plus the first window of each workgroup), and a row's old value
+// X[r, :] comes from the ring too.  A wavefront
 // takes whole rows, lane t holding column t.  A row's entries (at most 64 per
 // chunk) are loaded one per lane and broadcast by readlane; each entry's X
 // row comes from LDS when it lies in the window ("band") or from HBM
 // ("remote": rewired entries, about 10% in a cqmat with q = 0.1).  The sum
 // of a chunk is its band entries in storage order, then its remote entries
-// in storage order, each a separate multiply and add (no FMA).  A
+// in storage order, each a fused multiply-add.  A
 // wavefront runs its rows through a software pipeline (row pointers, then
 // entries, then the remote X rows, then the sum), so each row's loads are in
 // flight while earlier rows are summed from LDS; the band sum walks the
@@ -175,6 +181,7 @@ constexpr int kPprHist = 128;                   // offsets [-64, 63] in the hist
 struct PprArgs {
   long long n;  // rows = columns of the square block
   int f_col;
+  int old_lds;  // every tile's own rows lie in its window: old X values from the ring
   const int *rowptr, *colidx;
   const double *vals;
   const double *X;
@@ -195,6 +202,14 @@ __device__ __forceinline__ double readlane_d(double v, int idx) {
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), idx);
   return __hiloint2double(hi, lo);
 }
+
+// one product added to a row sum (a fused multiply-add: one VALU
+// instruction per entry and column instead of two; the step is within
+// rounding of the unfused sum)
+__device__ __forceinline__ double ppr_madd(double acc, double v, double x) { return fma(v, x, acc); }
+
+// the ring slot of X row j (j >= -kPprWin)
+__device__ __forceinline__ int ring_slot(int j) { return (int)((unsigned)(j + kPprWin) % kPprWin); }
 
 // The chunk's remote entries: the first kPprRemote X rows loaded into xr
 // (their values in vr, zeros past the count); returns the mask of the ones
@@ -221,7 +236,7 @@ __device__ __forceinline__ unsigned long long ppr_remote_issue(const PprArgs &a,
 }
 
 // sum of the chunk's band entries in storage order (X rows from the window)
-__device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo, int whi, int wbase,
+__device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo, int whi,
                                   const double *__restrict__ win, int lane) {
   TBSLA_NO_FMA
   unsigned long long m = __ballot(in && col >= wlo && col < whi);
@@ -235,14 +250,14 @@ __device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo
         m &= m - 1;
         const int c = __builtin_amdgcn_readlane(col, idx);
         vv[k] = readlane_d(val, idx);
-        xv[k] = win[(c - wbase) * kWave + lane];
+        xv[k] = win[ring_slot(c) * kWave + lane];
       } else {
         vv[k] = 0.0;  // adds +0: acc (never -0) is unchanged
         xv[k] = 0.0;
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc = acc + vv[k] * xv[k];
+    for (int k = 0; k < 8; k++) acc = ppr_madd(acc, vv[k], xv[k]);
   }
   return acc;
 }
@@ -254,11 +269,11 @@ __device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo
 // bookkeeping -- three readlanes, an LDS read, a multiply and an add per
 // entry.
 __device__ __forceinline__ double ppr_band_walk(int col, double val, int len, int wlo, int whi,
-                                                int wbase, const double *__restrict__ win,
+                                                const double *__restrict__ win,
                                                 int lane) {
   TBSLA_NO_FMA
   const bool band = lane < len && col >= wlo && col < whi;
-  const int off = band ? (col - wbase) * kWave : kPprWin * kWave;  // zero row
+  const int off = band ? ring_slot(col) * kWave : kPprWin * kWave;  // zero row
   const double v = band ? val : 0.0;
   double acc = 0.0;
 #pragma unroll
@@ -272,7 +287,7 @@ __device__ __forceinline__ double ppr_band_walk(int col, double val, int len, in
       xv[k] = win[o + lane];
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc = acc + vv[k] * xv[k];
+    for (int k = 0; k < 8; k++) acc = ppr_madd(acc, vv[k], xv[k]);
   }
   return acc;
 }
@@ -283,19 +298,19 @@ __device__ __forceinline__ double ppr_remote_add(const PprArgs &a, double acc, i
                                         const double (&vr)[kPprRemote], int lane) {
   TBSLA_NO_FMA
 #pragma unroll
-  for (int k = 0; k < kPprRemote; k++) acc = acc + vr[k] * xr[k];
+  for (int k = 0; k < kPprRemote; k++) acc = ppr_madd(acc, vr[k], xr[k]);
   while (left) {
     const int idx = __builtin_ctzll(left);
     left &= left - 1;
     const int c = __builtin_amdgcn_readlane(col, idx);
-    acc = acc + readlane_d(val, idx) * a.X[(size_t)c * kWave + lane];
+    acc = ppr_madd(acc, readlane_d(val, idx), a.X[(size_t)c * kWave + lane]);
   }
   return acc;
 }
 
 // entries past a row's first 64: chunk by chunk, band then remote
 __device__ __forceinline__ double ppr_long_tail(const PprArgs &a, int beg, int len, double acc,
-                                                int wlo, int whi, int wbase,
+                                                int wlo, int whi,
                                                 const double *__restrict__ win, int lane) {
   TBSLA_NO_FMA
   for (int c0 = kWave; c0 < len; c0 += kWave) {
@@ -305,7 +320,7 @@ __device__ __forceinline__ double ppr_long_tail(const PprArgs &a, int beg, int l
     const double val = in ? a.vals[j] : 0.0;
     double xr[kPprRemote], vr[kPprRemote];
     const unsigned long long left = ppr_remote_issue(a, col, val, in, wlo, whi, lane, xr, vr);
-    acc = acc + ppr_band(col, val, in, wlo, whi, wbase, win, lane);
+    acc = acc + ppr_band(col, val, in, wlo, whi, win, lane);
     acc = ppr_remote_add(a, acc, col, val, left, xr, vr, lane);
   }
   return acc;
@@ -323,24 +338,21 @@ struct PprSlot {
   double xr[S];             // X rows of the first S remote entries
 };
 
-// the staged window of row r's tile: X rows [wlo, whi), window row 0 = wbase
+// the window of row r's tile in the ring: X rows [wlo, whi)
 struct PprWindow {
-  int wlo, whi, wbase;
+  int wlo, whi;
 };
 __device__ __forceinline__ PprWindow ppr_window(const PprArgs &a, long long r) {
   const long long wbase = (r / kPprTile) * kPprTile + a.dlo;
   PprWindow w;
   w.wlo = (int)min(max(wbase, 0LL), a.n);
   w.whi = (int)max((long long)w.wlo, min(wbase + kPprWin, a.n));
-  w.wbase = (int)wbase;
   return w;
 }
 
-// The rows of one wavefront: workgroup b runs on XCD b mod 8, which owns the
-// (b mod 8)-th contiguous eighth of the tiles; its workgroups take them in
-// turn (the tiles in flight on one XCD are neighbours, their windows overlap
-// in L2).  Row q of the wavefront is row wave + kPprWaves (q mod
-// kPprRowsPerWave) of its (q / kPprRowsPerWave)-th tile.
+// The rows of one wavefront: its workgroup's tiles t0, t0 + tstep, ... below
+// t1; row q of the wavefront is row wave + kPprWaves (q mod kPprRowsPerWave)
+// of its (q / kPprRowsPerWave)-th tile.
 struct PprWalk {
   long long t0, t1, tstep;
   int wave;
@@ -371,7 +383,7 @@ __device__ __forceinline__ void ppr_se(const PprArgs &a, int r, int lane, PprSlo
   const int j = min(beg + lane, a.last);
   const int c = a.colidx[j];
   const double v = a.vals[j];
-  s.old = a.X[(size_t)max(r, 0) * kWave + lane];
+  s.old = a.old_lds ? 0.0 : a.X[(size_t)max(r, 0) * kWave + lane];
   s.beg = beg;
   s.len = len;
   s.col = in ? c - a.f_col : -1;
@@ -408,26 +420,27 @@ __device__ __forceinline__ void ppr_sc(const PprArgs &a, int r, const PprSlot<S>
   TBSLA_NO_FMA
   if (r < 0) return;
   const PprWindow w = ppr_window(a, r);
-  double acc = ppr_band_walk(s.col, s.val, s.len, w.wlo, w.whi, w.wbase, win, lane);
+  double acc = ppr_band_walk(s.col, s.val, s.len, w.wlo, w.whi, win, lane);
   unsigned long long m = ppr_remote_mask(a, s.col, w);
 #pragma unroll
   for (int k = 0; k < S; k++) {
     if (m) {
       const int idx = __builtin_ctzll(m);
       m &= m - 1;
-      acc = acc + readlane_d(s.val, idx) * s.xr[k];
+      acc = ppr_madd(acc, readlane_d(s.val, idx), s.xr[k]);
     }
   }
   while (m) {  // remote entries past the first S (rare)
     const int idx = __builtin_ctzll(m);
     m &= m - 1;
     const int c = __builtin_amdgcn_readlane(s.col, idx);
-    acc = acc + readlane_d(s.val, idx) * a.X[(size_t)c * kWave + lane];
+    acc = ppr_madd(acc, readlane_d(s.val, idx), a.X[(size_t)c * kWave + lane]);
   }
-  if (s.len > kWave) acc = ppr_long_tail(a, s.beg, s.len, acc, w.wlo, w.whi, w.wbase, win, lane);
+  if (s.len > kWave) acc = ppr_long_tail(a, s.beg, s.len, acc, w.wlo, w.whi, win, lane);
   double tv = a.beta * acc + ((r + a.f_col == src_l) ? tele : 0.0);
   tv = tv / S_l;
-  e += fabs(tv - s.old);
+  const double old = a.old_lds ? win[ring_slot(r) * kWave + lane] : s.old;
+  e += fabs(tv - old);
   sm += tv;
   a.Xn[(size_t)r * kWave + lane] = tv;
 }
@@ -475,17 +488,56 @@ __device__ __forceinline__ void ppr_prologue(const PprArgs &a, const PprWalk &wk
   if constexpr (K + 1 < D0 + DE + DR) ppr_prologue<NS, DE, DR, S, K + 1>(a, wk, st, lane);
 }
 
-// A workgroup takes tiles of kPprTile rows (kPprRowsPerWave per wavefront);
-// a wavefront runs its rows through the four stages above, so a row's
-// row-pointer, entry and remote X loads are in flight while the rows before
-// it are summed from LDS.  NS divides kPprRowsPerWave, so a tile starts a
-// group.
+// The ring's loads: kPprRingLoads 16-byte pieces per thread cover the
+// kPprTile rows a tile adds to the ring.  Rows [j0, j0 + kPprTile) of tile t
+// are loaded and stored when t < t1 and the row lies in [0, min(n, jhi)); a
+// missing row reads X's first piece and is not stored.
+constexpr int kPprPieces = kWave / 2;  // 16-byte pieces per X row
+constexpr int kPprRingLoads = kPprTile * kPprPieces / kPprBlock;
+static_assert(kPprTile * kPprPieces % kPprBlock == 0, "whole pieces per thread");
+
+// one 16-byte piece of an X row (a vector type, so the prefetched pieces
+// stay in registers)
+typedef double pprpiece_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bool ring_row(const PprArgs &a, bool tile_ok, long long j, long long jhi) {
+  return tile_ok && j >= 0 && j < a.n && j < jhi;
+}
+__device__ __forceinline__ void ring_load(const PprArgs &a, bool tile_ok, long long j0, long long jhi,
+                                          pprpiece_t (&pre)[kPprRingLoads]) {
+  const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
+#pragma unroll
+  for (int k = 0; k < kPprRingLoads; k++) {
+    const int p = (int)threadIdx.x + k * kPprBlock;
+    const long long j = j0 + p / kPprPieces;
+    pre[k] = X2[ring_row(a, tile_ok, j, jhi) ? j * kPprPieces + p % kPprPieces : 0];
+  }
+}
+__device__ __forceinline__ void ring_store(const PprArgs &a, bool tile_ok, long long j0, long long jhi,
+                                           const pprpiece_t (&pre)[kPprRingLoads], double *win) {
+  pprpiece_t *w2 = reinterpret_cast<pprpiece_t *>(win);
+#pragma unroll
+  for (int k = 0; k < kPprRingLoads; k++) {
+    const int p = (int)threadIdx.x + k * kPprBlock;
+    const long long j = j0 + p / kPprPieces;
+    if (ring_row(a, tile_ok, j, jhi)) w2[ring_slot((int)j) * kPprPieces + p % kPprPieces] = pre[k];
+  }
+}
+
+// A workgroup takes a contiguous run of tiles of kPprTile rows
+// (kPprRowsPerWave per wavefront); a wavefront runs its rows through the four
+// stages above, so a row's row-pointer, entry and remote X loads are in
+// flight while the rows before it are summed from LDS.  NS divides
+// kPprRowsPerWave, so a tile starts a group.  At a tile boundary the
+// workgroup waits for every row of the tile, writes the prefetched rows over
+// the departing ones, waits again, and issues the loads of the next tile's
+// new rows.
 template <int NS, int DE, int DR, int S>
 __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   TBSLA_NO_FMA
   static_assert(kPprRowsPerWave % NS == 0, "a tile starts a group");
+  static_assert(kPprWin > kPprTile && kPprWin <= 2 * kPprTile, "the first window is two loads");
   if (a.st->done) return;
-  // the window, then one row of zeros (read for entries outside it)
+  // the ring, then one row of zeros (read for entries outside the window)
   __shared__ __attribute__((aligned(16))) double win[(kPprWin + 1) * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
@@ -494,50 +546,35 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   const int src_l = a.src[lane];
   const double tele = S_l * (1 - a.beta);
   double e = 0.0, sm = 0.0;
-  const int nb = gridDim.x, b = blockIdx.x;
+  const long long nb = gridDim.x, b = blockIdx.x;
+  const long long per = (a.ntiles + nb - 1) / nb;
   PprWalk wk;
   wk.wave = wave;
-  if (nb % 8 == 0) {
-    const long long per = (a.ntiles + 7) / 8;
-    wk.t0 = min(a.ntiles, (long long)(b % 8) * per) + b / 8;
-    wk.t1 = min(a.ntiles, (long long)(b % 8 + 1) * per);
-    wk.tstep = nb / 8;
-  } else {
-    wk.t0 = b;
-    wk.t1 = a.ntiles;
-    wk.tstep = nb;
-  }
-  const long long my_tiles = wk.t0 < wk.t1 ? (wk.t1 - wk.t0 + wk.tstep - 1) / wk.tstep : 0;
-  const int nq = (int)(my_tiles * kPprRowsPerWave);
+  wk.t0 = min(a.ntiles, b * per);
+  wk.t1 = min(a.ntiles, wk.t0 + per);
+  wk.tstep = 1;
+  const int nq = (int)((wk.t1 - wk.t0) * kPprRowsPerWave);
+  // the first tile's window [base0, base0 + kPprWin) in two loads of
+  // kPprTile rows, then the next tile's new rows into registers
+  const long long base0 = wk.t0 * kPprTile + a.dlo;
+  const bool any = wk.t0 < wk.t1;
+  pprpiece_t pre[kPprRingLoads];
+  ring_load(a, any, base0, base0 + kPprWin, pre);
+  ring_store(a, any, base0, base0 + kPprWin, pre, win);
+  ring_load(a, any, base0 + kPprTile, base0 + kPprWin, pre);
+  ring_store(a, any, base0 + kPprTile, base0 + kPprWin, pre, win);
+  ring_load(a, wk.t0 + 1 < wk.t1, base0 + kPprWin, base0 + kPprWin + kPprTile, pre);
+  __syncthreads();
   PprSlot<S> st[NS];
   ppr_prologue<NS, DE, DR, S, 0>(a, wk, st, lane);
   for (int q0 = 0; q0 < nq; q0 += NS) {
-    if (q0 % kPprRowsPerWave == 0) {
-      const long long t = wk.t0 + (long long)(q0 / kPprRowsPerWave) * wk.tstep;
-      const PprWindow w = ppr_window(a, t * kPprTile);
-      __syncthreads();  // the previous tile's rows are done with the window
-      // stage X rows [wlo, whi): 16-byte pieces, in halves of kHalf per
-      // thread (their loads in flight together)
-      const int pieces = (w.whi - w.wlo) * (kWave / 2);
-      const double2 *srcp = reinterpret_cast<const double2 *>(a.X + (size_t)w.wlo * kWave);
-      const double2 *safe = reinterpret_cast<const double2 *>(a.X);
-      double2 *dst = reinterpret_cast<double2 *>(win + (size_t)(w.wlo - w.wbase) * kWave);
-      constexpr int kHalf = kPprWin * (kWave / 2) / kPprBlock / 2;  // 6 pieces
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        double2 tmp[kHalf];
-#pragma unroll
-        for (int p = 0; p < kHalf; p++) {
-          const int i = (int)threadIdx.x + (h * kHalf + p) * kPprBlock;
-          tmp[p] = *(i < pieces ? srcp + i : safe);
-        }
-#pragma unroll
-        for (int p = 0; p < kHalf; p++) {
-          const int i = (int)threadIdx.x + (h * kHalf + p) * kPprBlock;
-          if (i < pieces) dst[i] = tmp[p];
-        }
-      }
+    if (q0 > 0 && q0 % kPprRowsPerWave == 0) {
+      const long long t = wk.t0 + q0 / kPprRowsPerWave;
+      const long long jn = t * kPprTile + a.dlo + (kPprWin - kPprTile);  // tile t's new rows
+      __syncthreads();  // tile t - 1's rows are done with the departing rows
+      ring_store(a, true, jn, jn + kPprTile, pre, win);
       __syncthreads();
+      ring_load(a, t + 1 < wk.t1, jn + kPprTile, jn + 2 * kPprTile, pre);
     }
     ppr_group_step<NS, DE, DR, S, 0>(a, wk, q0, st, win, lane, src_l, S_l, tele, e, sm);
   }
@@ -713,12 +750,11 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   if (n < 1) return fail(TBSLA_HIP_ERR_INVALID, "empty block");
   if (!m->rowptr) return fail(TBSLA_HIP_ERR_INVALID, "the block has no row pointer");
   if (m->stored > INT_MAX) return fail(TBSLA_HIP_ERR_RANGE, "more than 2^31 - 1 entries");
-  // tile workgroups: three per CU (48 KB of LDS each, 12 wavefronts) on 256 CUs, a
-  // multiple of 8 (one contiguous eighth of the tiles per XCD), fewer when
-  // the block has fewer tiles
+  // tile workgroups: three per CU (48 KB of LDS each, 12 wavefronts) on 256
+  // CUs, each taking a contiguous run of tiles; fewer when the block has
+  // fewer tiles
   const long long ntiles = (n + kPprTile - 1) / kPprTile;
   const unsigned nb = (unsigned)std::min<long long>(768, ntiles);
-  static_assert(kPprWin * (kWave / 2) % (2 * kPprBlock) == 0, "window staging pieces");
   // partials of the column sums at the end: the row-slot grid of k_col_partial
   const unsigned nbc = n >= (long long)kPprSlots * kPprGrid
                            ? (unsigned)kPprGrid
@@ -789,6 +825,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   pa.p_err = part;
   pa.p_sum = part + (size_t)nb * kWave;
   pa.dlo = dlo;
+  pa.old_lds = dlo <= 0 && dlo + kPprWin >= kPprTile;
   pa.ntiles = ntiles;
   int launched = 0;
   while (launched < max_iterations) {
@@ -796,7 +833,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     for (int i = 0; i < batch; i++, launched++) {
       pa.X = Xb[launched % 2];
       pa.Xn = Xb[(launched + 1) % 2];
-      k_ppr_pipe<4, 2, 1, 6><<<nb, kPprBlock, 0, s>>>(pa);
+      k_ppr_pipe<4, 2, 1, 4><<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     hipError_t e = hipGetLastError();

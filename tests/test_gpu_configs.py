@@ -227,7 +227,8 @@ def test_cfg4_batched_cg_512():
     for k in sorted(rng.choice(512, 8, replace=False)):
         p = T.Matrix(T.CSR).fill_sym_cqmat(int(n[k]), int(c[k]), 0.1, 1, value_seed=int(k))
         xk, itk, rk = p.cg(b[offs[k]:offs[k + 1]], tolerance=1e-8, relative=True,
-                           lanes_per_row=lanes, max_iterations=5000)
+                           lanes_per_row=lanes, max_iterations=5000,
+                           max_blocks_per_system=256)
         assert itk == res64[k].iterations and rk == res64[k].res_norm
         assert np.array_equal(xk, xs[offs[k]:offs[k + 1]])
 
