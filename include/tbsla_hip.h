@@ -381,6 +381,12 @@ typedef struct {
                             together, takes fewer; a system solved alone and in a batch
                             gives bitwise the same iterates when both use the same value
                             (and the same lanes_per_row) */
+  int spmv_passes;       /* K1 (p = z + beta p, then A p) on A's CSR rows: 1 forms p at every
+                            gathered entry from z and p (one kernel); 2 writes p first and
+                            then gathers it alone (two kernels, 24 more bytes per row, half
+                            the gathers: for matrices whose gathers miss the caches); 0 picks
+                            2 from 2^24 stored entries.  Both give bitwise the same iterates;
+                            A in the diagonal layout always takes its own one-kernel form */
 } tbsla_hip_cg_options;
 enum {
   TBSLA_HIP_FACTOR_AUTO = 0,

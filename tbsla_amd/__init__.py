@@ -33,7 +33,7 @@ class CGOptions(C.Structure):
                 ("use_initial_guess", C.c_int), ("near_zero", C.c_double),
                 ("factor", C.c_void_p), ("factor_t", C.c_void_p), ("graph_batch", C.c_int),
                 ("lanes_per_row", C.c_int), ("stall_window", C.c_int), ("factor_form", C.c_int),
-                ("max_blocks_per_system", C.c_int)]
+                ("max_blocks_per_system", C.c_int), ("spmv_passes", C.c_int)]
 
 
 class CGResult(C.Structure):
@@ -597,17 +597,20 @@ class Matrix:
                    precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0,
                    use_initial_guess=False, near_zero=0.0, factor=None, factor_t=None,
                    graph_batch=0, lanes_per_row=0, stall_window=0,
-                   factor_form=FACTOR_AUTO, max_blocks_per_system=0) -> CGOptions:
+                   factor_form=FACTOR_AUTO, max_blocks_per_system=0,
+                   spmv_passes=0) -> CGOptions:
         return CGOptions(max_iterations, tolerance, int(relative), precision, preconditioner,
                          replace_every, int(use_initial_guess), near_zero,
                          factor.h if factor is not None else None,
                          factor_t.h if factor_t is not None else None, graph_batch,
-                         lanes_per_row, stall_window, factor_form, max_blocks_per_system)
+                         lanes_per_row, stall_window, factor_form, max_blocks_per_system,
+                         spmv_passes)
 
     def cg(self, rhs, max_iterations=10000, tolerance=1e-10, relative=False,
            precision=CG_FP64, preconditioner=PRECOND_NONE, replace_every=0, x0=None,
            near_zero=0.0, factor=None, factor_t=None, graph_batch=0, lanes_per_row=0,
-           stall_window=0, factor_form=0, max_blocks_per_system=0, stream=None):
+           stall_window=0, factor_form=0, max_blocks_per_system=0, spmv_passes=0,
+           stream=None):
         """Corrected tbsla::mpi::Matrix::CG; returns (x, iterations, |r|).
         rhs / x0: numpy or DeviceBuffer; M = G G^T with factor=G (a CSR Matrix)."""
         drhs = _dev(rhs)
@@ -616,7 +619,7 @@ class Matrix:
         o = self.cg_options(max_iterations, tolerance, relative, precision, preconditioner,
                             replace_every, x0 is not None, near_zero, factor, factor_t,
                             graph_batch, lanes_per_row, stall_window, factor_form,
-                            max_blocks_per_system)
+                            max_blocks_per_system, spmv_passes)
         it, rn = C.c_int(), C.c_double()
         check(lib().tbsla_hip_cg(self.h, drhs.ptr, dx.ptr, C.byref(o), C.byref(it),
                                  C.byref(rn), stream))

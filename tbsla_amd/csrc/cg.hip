@@ -44,6 +44,7 @@ namespace {
 constexpr double kNearZero = 1.0e-10;  // CG.cpp:35
 constexpr int kMaxBlocksPerSystem = 1024;
 constexpr int kBatchBlocksPerSystem = 256;  // default for a batch of systems
+constexpr long long kTwoPassEntries = 1LL << 24;  // K1 in two passes from this many entries
 
 enum SegDone { kRunning = 0, kConverged = 1, kPending = 2, kBreakdown = 3, kInactive = 5 };
 
@@ -437,6 +438,123 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_p(CG_ARGS, int par_g, int ch
         pn[rw[r]] = own;
         a.ap[rw[r]] = (double)acc;
         pap += own * (double)acc;
+      }
+    }
+  }
+  pap = wg_sum(pap, lds);
+  if (threadIdx.x == 0) a.ppap[b] = pap;
+}
+
+// K1 in two passes (for matrices whose gathers miss the caches: each
+// gathered entry then pulls one line of p' instead of one of z and one of
+// p).  First pass: the partials, the convergence test and beta exactly as
+// k_cg_spmv_p, then p' = z + beta p for the workgroup's rows.
+template <bool ZBUF>
+__global__ __launch_bounds__(kBlock) void k_cg_pnew(CG_ARGS, int par_g, int check,
+                                                    int check_only) {
+  TBSLA_NO_FMA
+  CG_BIND;
+  constexpr int NP = ZBUF ? 3 : 2;
+  __shared__ SegShared sh;
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  const int par = (par_g ^ S->shift) & 1;
+  const int b0 = S->blk0, b1 = S->blk1;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const double *zv = ZBUF ? a.z : a.r[par];
+  const double *po = a.p[par];
+  const int flag = __atomic_load_n(&S->done, __ATOMIC_RELAXED);
+  SegLoad sl;
+  seg_issue(a.prz[par ^ 1], a.prz[par], a.prr[par ^ 1], NP, b0, b1, sl);
+  double t[3];
+  if (seg_finish(flag, sl, NP, b0, b1, sh, t) != kRunning) return;
+  const double rz_new = t[0], rz_old = t[1];
+  const double rr = ZBUF ? t[2] : rz_new;
+  if (check) {
+    const int fin = k1_test(a, S, b, b0, par, rz_new, rr, check_only);
+    if (fin != kRunning || check_only) return;
+  }
+  const double beta = rz_new / fmax(rz_old, S->near_zero);
+  double *pn = a.p[par ^ 1];
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += 2 * kBlock) {
+    const int row2 = row + kBlock;
+    const double z0 = zv[row], p0 = po[row];
+    const double z1 = row2 < r1 ? zv[row2] : 0.0, p1 = row2 < r1 ? po[row2] : 0.0;
+    pn[row] = z0 + beta * p0;
+    if (row2 < r1) pn[row2] = z1 + beta * p1;
+  }
+}
+
+// Second pass: Ap' and the partials of p'.Ap' for the rows of systems still
+// running (the first pass made this iteration's test), the row schedule of
+// k_cg_spmv_p gathering p' alone.  Every value is the one k_cg_spmv_p
+// computes: p' is formed by the same expression.
+template <int L, bool MIX>
+__global__ __launch_bounds__(kBlock) void k_cg_ap(CG_ARGS, int par_g) {
+  TBSLA_NO_FMA  // p'.Ap' partials added as k_cg_spmv_p adds them
+  CG_BIND;
+  typedef typename std::conditional<MIX, float, double>::type V;
+  constexpr int P = L == 1 ? 8 : 4;
+  __shared__ double lds[kBlock / kWave];
+  const int b = blockIdx.x;
+  CgSeg *S = a.seg + a.block_seg[b];
+  if (__atomic_load_n(&S->done, __ATOMIC_RELAXED) != kRunning) return;
+  const int par = (par_g ^ S->shift) & 1;
+  const int b0 = S->blk0;
+  const int r0 = S->row0 + (b - b0) * S->rpb;
+  const int r1 = min(S->row1, r0 + S->rpb);
+  const int lane = threadIdx.x & (L - 1);
+  const double *pn = a.p[par ^ 1];
+  const V *val = MIX ? (const V *)a.val32 : (const V *)a.val64;
+  constexpr int PASS = kBlock / L;
+  constexpr int R2 = 2;
+  double pap = 0.0;
+  // rows two at a time: both rows' extents, then entries, then gathers are
+  // issued before either is summed
+  for (int row = r0 + (int)threadIdx.x / L; row < r1; row += R2 * PASS) {
+    int beg[R2], end[R2], rw[R2], col[R2][P];
+    V v[R2][P];
+    double g[R2][P], own[R2];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      rw[r] = min(row + r * PASS, r1 - 1);
+      beg[r] = a.rowptr[rw[r]];
+      end[r] = a.rowptr[rw[r] + 1];
+      own[r] = pn[rw[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        const int j = min(beg[r] + lane + k * L, a.a_nnz - 1);
+        col[r][k] = a.colidx[j];
+        v[r][k] = val[j];
+      }
+#pragma unroll
+    for (int r = 0; r < R2; r++)
+#pragma unroll
+      for (int k = 0; k < P; k++) g[r][k] = pn[col[r][k] - a.f_col];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+      if (row + r * PASS >= r1) break;
+      V acc = 0;
+#pragma unroll
+      for (int k = 0; k < P; k++) {
+        if (beg[r] + lane + k * L < end[r]) {
+          if (MIX) acc = fmaf((float)v[r][k], (float)g[r][k], (float)acc);
+          else acc = fma((double)v[r][k], g[r][k], (double)acc);
+        }
+      }
+      for (int j = beg[r] + lane + P * L; j < end[r]; j += L) {
+        const double x = pn[a.colidx[j] - a.f_col];
+        if (MIX) acc = fmaf((float)val[j], (float)x, (float)acc);
+        else acc = fma((double)val[j], x, (double)acc);
+      }
+      acc = lane_sum<L>(acc);
+      if (lane == 0) {
+        a.ap[rw[r]] = (double)acc;
+        pap += own[r] * (double)acc;
       }
     }
   }
@@ -1279,11 +1397,11 @@ struct Buf {
 // launch plan (kernel variants, grid, iteration indices).  Pointers are not
 // part of it -- every kernel reads them from the argument block at run time.
 struct GraphKey {
-  int L, pre, mixed, replace, batch, nblocks, verify, dia, gdia;
+  int L, pre, mixed, replace, batch, nblocks, verify, dia, gdia, two;
   bool operator==(const GraphKey &o) const {
     return L == o.L && pre == o.pre && mixed == o.mixed && replace == o.replace &&
            batch == o.batch && nblocks == o.nblocks && verify == o.verify && dia == o.dia &&
-           gdia == o.gdia;
+           gdia == o.gdia && two == o.two;
   }
 };
 
@@ -1448,6 +1566,7 @@ namespace {
 struct Plan {
   int L, pre, mixed, replace, verify, stall;
   bool dia;   // K1 reads A's diagonal layout
+  bool two;   // K1 in two passes (k_cg_pnew, then k_cg_ap) on A's CSR rows
   bool gdia;  // M = G G^T applied from G's diagonals
   int gkd;    // their kernels' capacity: 4 or 8 diagonals
   long long n;
@@ -1458,7 +1577,12 @@ struct Plan {
 
 template <int L, bool MIX, bool ZBUF>
 void launch_k1(const Plan &p, int par, int check, int check_only) {
-  k_cg_spmv_p<L, MIX, ZBUF><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
+  if (!p.two) {
+    k_cg_spmv_p<L, MIX, ZBUF><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
+    return;
+  }
+  k_cg_pnew<ZBUF><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par, check, check_only);
+  if (!check_only) k_cg_ap<L, MIX><<<p.nblocks, kBlock, 0, p.s>>>(p.args, par);
 }
 
 template <int L>
@@ -1699,6 +1823,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   const bool mixed = o.precision == TBSLA_HIP_CG_MIXED;
   const int max_it = o.max_iterations;
   if (max_it < 0) return fail(TBSLA_HIP_ERR_INVALID, "max_iterations < 0");
+  if (o.spmv_passes < 0 || o.spmv_passes > 2)
+    return fail(TBSLA_HIP_ERR_INVALID, "spmv_passes must be 0, 1 or 2");
   if (o.max_blocks_per_system < 0 || o.max_blocks_per_system > kMaxBlocksPerSystem)
     return fail(TBSLA_HIP_ERR_INVALID, "max_blocks_per_system must be 0..1024");
   // workgroups per system: the dot-product partials every workgroup of a
@@ -1883,6 +2009,11 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   long long dv_len = 0;
   unsigned long long build = 0;
   const bool dia = split_diag_only(m, &d_off, &d_v, &ndiag, &dv_len, &build) && m->ln_col > 0;
+  // K1 on CSR rows in two passes: asked for, or picked for large blocks,
+  // whose gathers of z and p miss the caches (a random column then costs
+  // one line fill instead of two)
+  const bool two = !dia && (o.spmv_passes == 2 ||
+                            (o.spmv_passes == 0 && m->stored >= kTwoPassEntries));
   if (dia) {
     a.doff = d_off;
     a.dv64 = d_v;
@@ -1939,6 +2070,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   p.verify = a.verify;
   p.stall = o.stall_window;
   p.dia = dia;
+  p.two = two;
   p.gdia = gdia;
   p.gkd = gndiag <= 4 ? 4 : 8;
   p.n = n;
@@ -2011,6 +2143,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   key.verify = p.verify;
   key.dia = p.dia ? 1 : 0;
   key.gdia = p.gdia ? p.gkd : 0;
+  key.two = p.two ? 1 : 0;
   // the graph set of this plan, captured and instantiated once; the least
   // recently used set is destroyed when kGraphSets are held (the stream is
   // idle between solves, so no set is in flight then)

@@ -414,3 +414,51 @@ def test_diagonal_layout_mixed_copy_not_reused_across_handles():
         assert np.linalg.norm(rhs - A @ x) < 1e-10 * np.linalg.norm(rhs) * 1.0001
         del g
         gc.collect()
+
+
+@pytest.mark.parametrize("precision,pre", [
+    (T.CG_FP64, T.PRECOND_NONE), (T.CG_MIXED, T.PRECOND_NONE),
+    (T.CG_FP64, T.PRECOND_JACOBI), (T.CG_MIXED, T.PRECOND_JACOBI),
+])
+def test_two_pass_k1_same_bits(sym, precision, pre):
+    """K1 in two passes (p' written, then gathered alone) gives the iterates
+    of the one-kernel form bit for bit, at every lane count."""
+    o, g = sym
+    n = o.geom["ln_row"]
+    rhs = 2 * O.unit_vector(23, n) - 1
+    rep = 50 if precision == T.CG_MIXED else 0
+    for L in (1, 4, 16):
+        out = [g.cg(rhs, tolerance=1e-10, relative=True, precision=precision,
+                    preconditioner=pre, replace_every=rep, lanes_per_row=L, spmv_passes=k)
+               for k in (1, 2)]
+        (x1, it1, r1), (x2, it2, r2) = out
+        assert it1 == it2 and r1 == r2
+        assert np.array_equal(x1, x2)
+    if precision != T.CG_FP64:
+        return
+    # early iterates of the two-pass form against the oracle's PCG
+    for k in (1, 5):
+        x, _, _ = g.cg(rhs, max_iterations=k, tolerance=1e-30, relative=True,
+                       preconditioner=pre, spmv_passes=2)
+        xo, _, _ = O.pcg(o, rhs, max_iterations=k, tolerance=1e-30,
+                         kind=1 if pre == T.PRECOND_JACOBI else 0)
+        assert rel_max(x, xo) < 1e-12
+
+
+def test_two_pass_k1_batched_same_bits():
+    sizes = [(4096, 8), (20000, 16), (9000, 20)]
+    parts, bd, offs = make_batch(sizes, seed=3)
+    rhs = 2 * O.unit_vector(8, int(offs[-1])) - 1
+    xs = []
+    for k in (1, 2):
+        opt = bd.cg_options(tolerance=1e-10, relative=True, lanes_per_row=8, spmv_passes=k)
+        x, res = bd.cg_batched(offs, rhs, opt)
+        xs.append((x, [(r.iterations, r.res_norm, r.status) for r in res]))
+    assert xs[0][1] == xs[1][1]
+    assert np.array_equal(xs[0][0], xs[1][0])
+
+
+def test_spmv_passes_rejected_out_of_range(sym):
+    o, g = sym
+    with pytest.raises(T.TbslaHipError):
+        g.cg(np.ones(o.geom["ln_row"]), spmv_passes=3)
