@@ -110,6 +110,23 @@ def host_cpu():
     return model, len(os.sched_getaffinity(0))
 
 
+def physical_cores():
+    """Distinct physical cores (package, core id) among this process's CPUs,
+    or None when the topology is not readable."""
+    cores = set()
+    try:
+        for cpu in os.sched_getaffinity(0):
+            base = f"/sys/devices/system/cpu/cpu{cpu}/topology/"
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+    except OSError:
+        return None
+    return len(cores) or None
+
+
 def cpu_threads(requested=None):
     """Threads for the CPU baseline: the CPUs of this process's affinity,
     capped by OMP_NUM_THREADS (the host's CPU share) when it is set."""
@@ -126,7 +143,9 @@ def cpu_baseline(n, c, q, stored, iters, reps, threads):
     import oracle
     model, avail = host_cpu()
     exe = oracle.REF_BENCH
-    host = f"{model}, {threads} of {avail} logical CPUs in this process's affinity"
+    phys = physical_cores()
+    host = (f"{model}, {threads} threads on {avail} logical CPUs in this process's affinity"
+            + (f" ({phys} physical cores)" if phys else ""))
     if os.path.exists(exe):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                    OMP_PLACES="cores")
@@ -518,7 +537,7 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic
     step_done("csr spmv (%.1f GB/s)" % gbs)
-    cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 20, 10,
+    cpu = None if args.no_cpu else cpu_baseline(N, C, Q, g.stored, 100, 10,
                                                 cpu_threads(args.cpu_threads))
     if cpu is not None:
         step_done("cpu baseline")
