@@ -179,7 +179,7 @@ constexpr int kPprRemote = 8;                   // remote X rows in flight toget
 constexpr int kPprHist = 128;                   // offsets [-64, 63] in the histogram
 
 struct PprArgs {
-  long long n;  // rows = columns of the square block
+  int n;  // rows = columns of the square block (int, as the reference's ln_row)
   int f_col;
   int old_lds;  // every tile's own rows lie in its window: old X values from the ring
   const int *rowptr, *colidx;
@@ -192,7 +192,7 @@ struct PprArgs {
   const PprState *st;
   double *p_err, *p_sum;
   int dlo;
-  long long ntiles;
+  int ntiles;
   int last;  // index of the last stored entry (a dummy entry when the block has none)
 };
 
@@ -342,11 +342,11 @@ struct PprSlot {
 struct PprWindow {
   int wlo, whi;
 };
-__device__ __forceinline__ PprWindow ppr_window(const PprArgs &a, long long r) {
-  const long long wbase = (r / kPprTile) * kPprTile + a.dlo;
+__device__ __forceinline__ PprWindow ppr_window(const PprArgs &a, int r) {
+  const int wbase = (r / kPprTile) * kPprTile + a.dlo;
   PprWindow w;
-  w.wlo = (int)min(max(wbase, 0LL), a.n);
-  w.whi = (int)max((long long)w.wlo, min(wbase + kPprWin, a.n));
+  w.wlo = min(max(wbase, 0), a.n);
+  w.whi = max(w.wlo, min(wbase + kPprWin, a.n));
   return w;
 }
 
@@ -354,17 +354,17 @@ __device__ __forceinline__ PprWindow ppr_window(const PprArgs &a, long long r) {
 // t1; row q of the wavefront is row wave + kPprWaves (q mod kPprRowsPerWave)
 // of its (q / kPprRowsPerWave)-th tile.
 struct PprWalk {
-  long long t0, t1, tstep;
+  int t0, t1, tstep;
   int wave;
 };
 __device__ __forceinline__ int ppr_row_at(const PprArgs &a, const PprWalk &wk, int q) {
-  const long long t = wk.t0 + (long long)(q / kPprRowsPerWave) * wk.tstep;
-  const long long r = t * kPprTile + wk.wave + kPprWaves * (q % kPprRowsPerWave);
-  return (t < wk.t1 && r < a.n) ? (int)r : -1;
+  const int t = wk.t0 + (q / kPprRowsPerWave) * wk.tstep;
+  const int r = t * kPprTile + wk.wave + kPprWaves * (q % kPprRowsPerWave);
+  return (t < wk.t1 && r < a.n) ? r : -1;
 }
 
-// The stages issue the same loads whatever the row (a missing row or an
-// unused slot reads row 0 or the row's own X row, both valid), so the
+// The stages issue the same loads whatever the row (a missing row reads row
+// 0; an unused remote slot reads X row 0, which stays cached), so the
 // compiler counts the loads in flight exactly and waits only for the ones a
 // stage consumes.
 //
@@ -380,7 +380,8 @@ __device__ __forceinline__ void ppr_se(const PprArgs &a, int r, int lane, PprSlo
   const int beg = __builtin_amdgcn_readlane(s.rp, 0);
   const int len = r < 0 ? 0 : __builtin_amdgcn_readlane(s.rp, 1) - beg;
   const bool in = lane < len;
-  const int j = min(beg + lane, a.last);
+  // lanes past the row's end read its last entry (no line beyond the row's)
+  const int j = min(beg + min(lane, max(len - 1, 0)), a.last);
   const int c = a.colidx[j];
   const double v = a.vals[j];
   s.old = a.old_lds ? 0.0 : a.X[(size_t)max(r, 0) * kWave + lane];
@@ -404,7 +405,7 @@ __device__ __forceinline__ void ppr_sr(const PprArgs &a, int r, int lane, PprSlo
 #pragma unroll
   for (int k = 0; k < S; k++) {
     const int idx = m ? __builtin_ctzll(m) : 0;
-    const int c = m ? __builtin_amdgcn_readlane(s.col, idx) : rr;
+    const int c = m ? __builtin_amdgcn_readlane(s.col, idx) : 0;  // unused slot: X row 0 (cached)
     m &= m - 1;
     s.xr[k] = a.X[(size_t)c * kWave + lane];
   }
@@ -499,27 +500,27 @@ static_assert(kPprTile * kPprPieces % kPprBlock == 0, "whole pieces per thread")
 // one 16-byte piece of an X row (a vector type, so the prefetched pieces
 // stay in registers)
 typedef double pprpiece_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bool ring_row(const PprArgs &a, bool tile_ok, long long j, long long jhi) {
+__device__ __forceinline__ bool ring_row(const PprArgs &a, bool tile_ok, int j, int jhi) {
   return tile_ok && j >= 0 && j < a.n && j < jhi;
 }
-__device__ __forceinline__ void ring_load(const PprArgs &a, bool tile_ok, long long j0, long long jhi,
+__device__ __forceinline__ void ring_load(const PprArgs &a, bool tile_ok, int j0, int jhi,
                                           pprpiece_t (&pre)[kPprRingLoads]) {
   const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
 #pragma unroll
   for (int k = 0; k < kPprRingLoads; k++) {
     const int p = (int)threadIdx.x + k * kPprBlock;
-    const long long j = j0 + p / kPprPieces;
-    pre[k] = X2[ring_row(a, tile_ok, j, jhi) ? j * kPprPieces + p % kPprPieces : 0];
+    const int j = j0 + p / kPprPieces;
+    pre[k] = X2[ring_row(a, tile_ok, j, jhi) ? (size_t)j * kPprPieces + p % kPprPieces : 0];
   }
 }
-__device__ __forceinline__ void ring_store(const PprArgs &a, bool tile_ok, long long j0, long long jhi,
+__device__ __forceinline__ void ring_store(const PprArgs &a, bool tile_ok, int j0, int jhi,
                                            const pprpiece_t (&pre)[kPprRingLoads], double *win) {
   pprpiece_t *w2 = reinterpret_cast<pprpiece_t *>(win);
 #pragma unroll
   for (int k = 0; k < kPprRingLoads; k++) {
     const int p = (int)threadIdx.x + k * kPprBlock;
-    const long long j = j0 + p / kPprPieces;
-    if (ring_row(a, tile_ok, j, jhi)) w2[ring_slot((int)j) * kPprPieces + p % kPprPieces] = pre[k];
+    const int j = j0 + p / kPprPieces;
+    if (ring_row(a, tile_ok, j, jhi)) w2[ring_slot(j) * kPprPieces + p % kPprPieces] = pre[k];
   }
 }
 
@@ -546,8 +547,8 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   const int src_l = a.src[lane];
   const double tele = S_l * (1 - a.beta);
   double e = 0.0, sm = 0.0;
-  const long long nb = gridDim.x, b = blockIdx.x;
-  const long long per = (a.ntiles + nb - 1) / nb;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = (a.ntiles + nb - 1) / nb;
   PprWalk wk;
   wk.wave = wave;
   wk.t0 = min(a.ntiles, b * per);
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   const int nq = (int)((wk.t1 - wk.t0) * kPprRowsPerWave);
   // the first tile's window [base0, base0 + kPprWin) in two loads of
   // kPprTile rows, then the next tile's new rows into registers
-  const long long base0 = wk.t0 * kPprTile + a.dlo;
+  const int base0 = wk.t0 * kPprTile + a.dlo;
   const bool any = wk.t0 < wk.t1;
   pprpiece_t pre[kPprRingLoads];
   ring_load(a, any, base0, base0 + kPprWin, pre);
@@ -569,8 +570,8 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   ppr_prologue<NS, DE, DR, S, 0>(a, wk, st, lane);
   for (int q0 = 0; q0 < nq; q0 += NS) {
     if (q0 > 0 && q0 % kPprRowsPerWave == 0) {
-      const long long t = wk.t0 + q0 / kPprRowsPerWave;
-      const long long jn = t * kPprTile + a.dlo + (kPprWin - kPprTile);  // tile t's new rows
+      const int t = wk.t0 + q0 / kPprRowsPerWave;
+      const int jn = t * kPprTile + a.dlo + (kPprWin - kPprTile);  // tile t's new rows
       __syncthreads();  // tile t - 1's rows are done with the departing rows
       ring_store(a, true, jn, jn + kPprTile, pre, win);
       __syncthreads();
@@ -765,13 +766,23 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   unsigned int *bins = nullptr;
   PprState *st = nullptr;
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    hipFree(buf);
-    hipFree(part);
-    hipFree(S);
-    hipFree(src);
-    hipFree(st);
-    hipFree(bins);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    (void)hipFree(part);
+    (void)hipFree(S);
+    (void)hipFree(src);
+    (void)hipFree(st);
+    (void)hipFree(bins);
+  };
+  // the first failing HIP call of the set-up and iteration loop
+  hipError_t herr = hipSuccess;
+  auto note = [&](hipError_t e) {
+    if (herr == hipSuccess && e != hipSuccess) herr = e;
+  };
+  auto failed = [&](const char *where) {
+    cleanup();
+    return fail(TBSLA_HIP_ERR_HIP, std::string("personalised PageRank, ") + where + ": " +
+                                       hipGetErrorString(herr));
   };
   int rc;
   if ((rc = device_alloc((void **)&buf, (size_t)n * kWave * 8)) ||
@@ -785,12 +796,14 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   // where the band lies: the offsets col - row of every 16th row; the window
   // covers the kPprReach + 1 consecutive offsets holding the most entries
   std::vector<unsigned int> hb(kPprHist, 0);
-  hipMemsetAsync(bins, 0, kPprHist * sizeof(unsigned int), s);
+  note(hipMemsetAsync(bins, 0, kPprHist * sizeof(unsigned int), s));
   if (m->stored > 0)
     k_ppr_offsets<<<grid_for((n + 15) / 16), kBlock, 0, s>>>(n, m->f_col, m->rowptr, m->colidx,
                                                              bins);
-  hipMemcpyAsync(hb.data(), bins, kPprHist * sizeof(unsigned int), hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
+  note(hipGetLastError());
+  note(hipMemcpyAsync(hb.data(), bins, kPprHist * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  note(hipStreamSynchronize(s));
+  if (herr != hipSuccess) return failed("band histogram");
   int dlo = 0;
   unsigned long long best = 0;
   for (int lo = 0; lo + kPprReach < kPprHist; lo++) {
@@ -803,13 +816,15 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   }
   std::vector<double> hS(kWave, (double)n);  // X starts at 1: column sums n
   PprState hst{0, 0, 0.0};
-  hipMemcpyAsync(S, hS.data(), kWave * 8, hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(src, h_sources, kWave * 4, hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(st, &hst, sizeof(hst), hipMemcpyHostToDevice, s);
+  note(hipMemcpyAsync(S, hS.data(), kWave * 8, hipMemcpyHostToDevice, s));
+  note(hipMemcpyAsync(src, h_sources, kWave * 4, hipMemcpyHostToDevice, s));
+  note(hipMemcpyAsync(st, &hst, sizeof(hst), hipMemcpyHostToDevice, s));
   k_fill_d<<<stride_grid(n * kWave), kBlock, 0, s>>>(d_X, n * kWave, 1.0);
+  note(hipGetLastError());
+  if (herr != hipSuccess) return failed("set-up");
   double *Xb[2] = {d_X, buf};
   PprArgs pa{};
-  pa.n = n;
+  pa.n = (int)n;
   pa.f_col = m->f_col;
   pa.rowptr = m->rowptr;
   // the kernel reads entry min(j, last) whatever the row's length: with no
@@ -826,7 +841,7 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
   pa.p_sum = part + (size_t)nb * kWave;
   pa.dlo = dlo;
   pa.old_lds = dlo <= 0 && dlo + kPprWin >= kPprTile;
-  pa.ntiles = ntiles;
+  pa.ntiles = (int)ntiles;
   int launched = 0;
   while (launched < max_iterations) {
     const int batch = std::min(16, max_iterations - launched);
@@ -836,23 +851,21 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
       k_ppr_pipe<4, 2, 1, 4><<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      cleanup();
-      return fail(TBSLA_HIP_ERR_HIP, std::string("personalised PageRank: ") + hipGetErrorString(e));
-    }
-    hipMemcpyAsync(&hst, st, sizeof(hst), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    note(hipGetLastError());
+    note(hipMemcpyAsync(&hst, st, sizeof(hst), hipMemcpyDeviceToHost, s));
+    note(hipStreamSynchronize(s));
+    if (herr != hipSuccess) return failed("iterations");
     if (hst.done) break;
   }
   double *X = Xb[hst.iters % 2];
   k_col_partial<<<nbc, kBlock, 0, s>>>(X, n, part);
   k_col_total<<<1, kBlock, 0, s>>>(part, (int)nbc, S);
   k_col_divide<<<stride_grid(n * kWave), kBlock, 0, s>>>(X, n * kWave, S);
-  if (X != d_X) hipMemcpyAsync(d_X, X, (size_t)n * kWave * 8, hipMemcpyDeviceToDevice, s);
-  hipError_t e = hipGetLastError();
+  note(hipGetLastError());
+  if (X != d_X) note(hipMemcpyAsync(d_X, X, (size_t)n * kWave * 8, hipMemcpyDeviceToDevice, s));
+  note(hipStreamSynchronize(s));
+  if (herr != hipSuccess) return failed("normalisation");
   cleanup();
-  if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
   if (nb_iterations) *nb_iterations = hst.iters;
   if (max_error) *max_error = hst.max_err;
   return TBSLA_HIP_SUCCESS;
