@@ -462,3 +462,19 @@ def test_spmv_passes_rejected_out_of_range(sym):
     o, g = sym
     with pytest.raises(T.TbslaHipError):
         g.cg(np.ones(o.geom["ln_row"]), spmv_passes=3)
+
+
+@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
+def test_two_pass_k1_factored_same_bits(sym, precision):
+    """M = G G^T with K1 in one and in two passes: the same bits (z comes from
+    the preconditioner's buffer in both)."""
+    o, g = sym
+    n = o.geom["ln_row"]
+    G = to_gpu(lower_factor(o))
+    rhs = 2 * O.unit_vector(29, n) - 1
+    rep = 50 if precision == T.CG_MIXED else 0
+    out = [g.cg(rhs, tolerance=1e-10, relative=True, precision=precision, replace_every=rep,
+                preconditioner=T.PRECOND_FACTORED, factor=G, spmv_passes=k) for k in (1, 2)]
+    (x1, it1, r1), (x2, it2, r2) = out
+    assert it1 == it2 and r1 == r2
+    assert np.array_equal(x1, x2)
