@@ -40,7 +40,7 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
   if (m->scratch) {
     // the previous scratch may still be read by queued work
     TBSLA_HIP_CHECK(hipDeviceSynchronize());
-    hipFree(m->scratch);
+    (void)hipFree(m->scratch);
     m->scratch = nullptr;
     m->scratch_bytes = 0;
   }
@@ -53,12 +53,12 @@ int ensure_scratch(tbsla_hip_matrix_s *m, size_t bytes) {
 void release_arrays(tbsla_hip_matrix_s *m) {
   cg_cache_release(m);  // holds graphs that name these arrays
   split_release(m);     // a copy of the entries in another order
-  if (m->rowptr || m->colidx || m->rowidx || m->values || m->coo_rowptr) hipDeviceSynchronize();
-  hipFree(m->rowptr);
-  hipFree(m->colidx);
-  hipFree(m->rowidx);
-  hipFree(m->values);
-  hipFree(m->coo_rowptr);
+  if (m->rowptr || m->colidx || m->rowidx || m->values || m->coo_rowptr) (void)hipDeviceSynchronize();
+  (void)hipFree(m->rowptr);
+  (void)hipFree(m->colidx);
+  (void)hipFree(m->rowidx);
+  (void)hipFree(m->values);
+  (void)hipFree(m->coo_rowptr);
   m->coo_rowptr = nullptr;
   m->coo_row0 = m->coo_nrows = 0;
   m->rowptr = m->colidx = m->rowidx = nullptr;
@@ -237,7 +237,7 @@ int tbsla_hip_create(tbsla_hip_matrix_t *out, int format) {
 int tbsla_hip_destroy(tbsla_hip_matrix_t m) {
   if (!m) return TBSLA_HIP_SUCCESS;
   release_arrays(m);
-  hipFree(m->scratch);
+  (void)hipFree(m->scratch);
   delete m;
   return TBSLA_HIP_SUCCESS;
 }
@@ -486,8 +486,8 @@ int tbsla_hip_time_spmv(tbsla_hip_matrix_t m, double *d_r, const double *d_v, in
   TBSLA_HIP_CHECK(hipEventRecord(e1, s));
   TBSLA_HIP_CHECK(hipEventSynchronize(e1));
   TBSLA_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return st;
 }
 

@@ -1377,8 +1377,8 @@ struct Buf {
   int ensure(size_t count) {
     if (count <= n && p) return TBSLA_HIP_SUCCESS;
     if (p) {
-      hipDeviceSynchronize();
-      hipFree(p);
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
     }
     p = nullptr;
     n = 0;
@@ -1387,7 +1387,7 @@ struct Buf {
     return TBSLA_HIP_SUCCESS;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
   }
@@ -1459,7 +1459,7 @@ struct CgCache {
     void destroy() {
       for (auto &x : exec)
         if (x) {
-          hipGraphExecDestroy(x);
+          (void)hipGraphExecDestroy(x);
           x = nullptr;
         }
     }
@@ -1470,18 +1470,18 @@ struct CgCache {
   std::vector<int> host_blocks;
 
   void drop_factor() {
-    hipFree(t_rp);
-    hipFree(t_ci);
-    hipFree(t_v64);
-    hipFree(g_v32);
-    hipFree(t_v32);
+    (void)hipFree(t_rp);
+    (void)hipFree(t_ci);
+    (void)hipFree(t_v64);
+    (void)hipFree(g_v32);
+    (void)hipFree(t_v32);
     t_rp = t_ci = nullptr;
     t_v64 = nullptr;
     g_v32 = t_v32 = nullptr;
     f32_len = 0;
   }
   ~CgCache() {
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     for (auto &g : graphs) g.destroy();
     Buf<double> *bufs[] = {&b, &x, &r0, &r1, &p0, &p1, &z, &ap, &t64, &dinv, &ax,
                            &ppap, &prr0, &prr1, &prz0, &prz1, &gd64};
@@ -1496,15 +1496,15 @@ struct CgCache {
     gdoff.release();
     gbad.release();
     args.release();
-    if (h_args) hipHostFree(h_args);
-    if (h_com) hipHostFree(h_com);
-    if (h_probe) hipHostFree(h_probe);
+    if (h_args) (void)hipHostFree(h_args);
+    if (h_com) (void)hipHostFree(h_com);
+    if (h_probe) (void)hipHostFree(h_probe);
     for (auto &e : ev)
-      if (e) hipEventDestroy(e);
-    if (own) hipStreamDestroy(own);
+      if (e) (void)hipEventDestroy(e);
+    if (own) (void)hipStreamDestroy(own);
     drop_factor();
-    hipFree(a_v32);
-    hipFree(dv32);
+    (void)hipFree(a_v32);
+    (void)hipFree(dv32);
   }
 };
 
@@ -1549,7 +1549,7 @@ void cg_cache_release(tbsla_hip_matrix_s *m) {
   if (!m->cg_cache) return;
   CgCache *c = static_cast<CgCache *>(m->cg_cache);
   m->cg_cache = nullptr;
-  if (c->own) hipStreamSynchronize(c->own);
+  if (c->own) (void)hipStreamSynchronize(c->own);
   c->drop_factor();  // arrays derived from the old factor
   {
     std::lock_guard<std::mutex> g(spare_mutex());
@@ -1891,7 +1891,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     const size_t need = (size_t)std::max(m->stored, 1LL);
     if (c.a_v32_len < need) {
       TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-      hipFree(c.a_v32);
+      (void)hipFree(c.a_v32);
       c.a_v32 = nullptr;
       c.a_v32_len = 0;
       TBSLA_TRY(device_alloc((void **)&c.a_v32, need * 4));
@@ -1928,9 +1928,9 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
       tbsla_hip_matrix_s *GT = o.factor_t;
       if (!GT) {
         TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-        hipFree(c.t_rp);
-        hipFree(c.t_ci);
-        hipFree(c.t_v64);
+        (void)hipFree(c.t_rp);
+        (void)hipFree(c.t_ci);
+        (void)hipFree(c.t_v64);
         c.t_rp = c.t_ci = nullptr;
         c.t_v64 = nullptr;
         TBSLA_TRY(csr_transpose_arrays(G, &c.t_rp, &c.t_ci, &c.t_v64, s));
@@ -1938,8 +1938,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
       if (mixed) {
         if (c.f32_len < (size_t)std::max(G->stored, 1LL)) {
           TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-          hipFree(c.g_v32);
-          hipFree(c.t_v32);
+          (void)hipFree(c.g_v32);
+          (void)hipFree(c.t_v32);
           c.g_v32 = c.t_v32 = nullptr;
           c.f32_len = 0;
           TBSLA_TRY(device_alloc((void **)&c.g_v32, (size_t)std::max(G->stored, 1LL) * 4));
@@ -2021,7 +2021,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     if (mixed) {
       if (c.dv32_len < (size_t)dv_len) {
         TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-        hipFree(c.dv32);
+        (void)hipFree(c.dv32);
         c.dv32 = nullptr;
         c.dv32_len = 0;
         TBSLA_TRY(device_alloc((void **)&c.dv32, (size_t)dv_len * 4));
@@ -2172,7 +2172,7 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
           e = hipStreamEndCapture(s, &graph);
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&fresh.exec[v], graph, nullptr, nullptr, 0);
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         if (e != hipSuccess) {
           fresh.destroy();  // no partial set is ever kept
           return fail(TBSLA_HIP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -2242,8 +2242,8 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
   TBSLA_HIP_CHECK(hipMemcpyAsync(&h, c.com.p, sizeof h, hipMemcpyDeviceToHost, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   int dev = 0, khz = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
   for (int k = 0; k < nsys; k++) {
     const CgSeg &S = hs[k];
     tbsla_hip_cg_result &r = res[k];

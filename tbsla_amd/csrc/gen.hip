@@ -486,11 +486,11 @@ extern "C" int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, 
   long long total = 0;
   int st = counts_to_offsets(m, offs, ln_row, total, s);
   if (st != TBSLA_HIP_SUCCESS) {
-    hipFree(offs);
+    (void)hipFree(offs);
     return st;
   }
   if (total > INT32_MAX) {
-    hipFree(offs);
+    (void)hipFree(offs);
     return fail(TBSLA_HIP_ERR_RANGE, "block holds more than 2^31-1 entries");
   }
 
@@ -532,7 +532,7 @@ extern "C" int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, 
         TBSLA_HIP_LAUNCH_CHECK();
       }
     }
-    hipFree(offs);
+    (void)hipFree(offs);
   } else if (fmt == TBSLA_HIP_ELL) {
     m->nnz = total;
     m->max_col = width;
@@ -548,7 +548,7 @@ extern "C" int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, 
         TBSLA_HIP_LAUNCH_CHECK();
       }
     }
-    hipFree(offs);
+    (void)hipFree(offs);
   } else if (fmt == TBSLA_HIP_DENSE) {
     if (b.nv > 0) {
       m->nnz = total;
@@ -563,9 +563,9 @@ extern "C" int tbsla_hip_fill_cqmat(tbsla_hip_matrix_t m, int n_row, int n_col, 
         TBSLA_HIP_LAUNCH_CHECK();
       }
     }
-    hipFree(offs);
+    (void)hipFree(offs);
   } else {
-    hipFree(offs);
+    (void)hipFree(offs);
     return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
   }
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
@@ -626,7 +626,7 @@ extern "C" int tbsla_hip_fill_cdiag(tbsla_hip_matrix_t m, int n_row, int n_col, 
   long long total = 0;
   int st = counts_to_offsets(m, offs, ln_row, total, s);
   if (st != TBSLA_HIP_SUCCESS) {
-    hipFree(offs);
+    (void)hipFree(offs);
     return st;
   }
   m->nnz = total;
@@ -645,9 +645,9 @@ extern "C" int tbsla_hip_fill_cdiag(tbsla_hip_matrix_t m, int n_row, int n_col, 
     if (fmt == TBSLA_HIP_CSR)
       m->rowptr = offs;
     else
-      hipFree(offs);
+      (void)hipFree(offs);
   } else if (fmt == TBSLA_HIP_ELL) {
-    hipFree(offs);
+    (void)hipFree(offs);
     m->max_col = cdiag == 0 ? 1 : 2;
     if (total > 0) {
       size_t slots = (size_t)ln_row * m->max_col;
@@ -659,7 +659,7 @@ extern "C" int tbsla_hip_fill_cdiag(tbsla_hip_matrix_t m, int n_row, int n_col, 
       TBSLA_HIP_LAUNCH_CHECK();
     }
   } else if (fmt == TBSLA_HIP_DENSE) {
-    hipFree(offs);
+    (void)hipFree(offs);
     if (total > 0) {
       size_t cells = (size_t)ln_row * (size_t)m->ln_col;
       m->stored = (long long)cells;
@@ -671,7 +671,7 @@ extern "C" int tbsla_hip_fill_cdiag(tbsla_hip_matrix_t m, int n_row, int n_col, 
       TBSLA_HIP_LAUNCH_CHECK();
     }
   } else {
-    hipFree(offs);
+    (void)hipFree(offs);
     return fail(TBSLA_HIP_ERR_INVALID, "unknown format");
   }
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));

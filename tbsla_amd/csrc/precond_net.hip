@@ -762,8 +762,8 @@ struct DBuf {
   int ensure(size_t count) {
     if (p && count <= n) return TBSLA_HIP_SUCCESS;
     if (p) {
-      hipDeviceSynchronize();
-      hipFree(p);
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
     }
     p = nullptr;
     n = 0;
@@ -772,7 +772,7 @@ struct DBuf {
     return TBSLA_HIP_SUCCESS;
   }
   ~DBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
 };
 
@@ -1046,7 +1046,7 @@ extern "C" int tbsla_hip_precond_net_create(tbsla_hip_precond_net_t *out,
 
 extern "C" int tbsla_hip_precond_net_destroy(tbsla_hip_precond_net_t net) {
   if (!net) return TBSLA_HIP_SUCCESS;
-  hipDeviceSynchronize();
+  (void)hipDeviceSynchronize();
   delete net;
   return TBSLA_HIP_SUCCESS;
 }

@@ -338,8 +338,8 @@ int counts_to_rowptr(const int *cnt, int *rowptr, long long n, long long *total,
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) st = fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
   }
-  hipStreamSynchronize(s);
-  hipFree(ws);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(ws);
   if (st) return st;
   if (t > INT32_MAX) return fail(TBSLA_HIP_ERR_RANGE, "more than 2^31-1 entries");
   int ti = (int)t;
@@ -359,10 +359,10 @@ int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **co
   double *v = nullptr;
   unsigned long long *keys = nullptr;
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    hipFree(cnt);
-    hipFree(pay);
-    hipFree(keys);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(cnt);
+    (void)hipFree(pay);
+    (void)hipFree(keys);
   };
   int st;
   if ((st = device_alloc((void **)&cnt, (size_t)(nc + 1) * 4)) ||
@@ -372,12 +372,12 @@ int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **co
       (st = device_alloc((void **)&pay, (size_t)std::max(m, 1LL) * 4)) ||
       (st = device_alloc((void **)&keys, (size_t)std::max(m, 1LL) * 8))) {
     cleanup();
-    hipFree(rp);
-    hipFree(ci);
-    hipFree(v);
+    (void)hipFree(rp);
+    (void)hipFree(ci);
+    (void)hipFree(v);
     return st;
   }
-  hipMemsetAsync(cnt, 0, (size_t)(nc + 1) * 4, s);
+  (void)hipMemsetAsync(cnt, 0, (size_t)(nc + 1) * 4, s);
   const int rb = bits_of(std::max(nr, 1LL));
   const int cbits = bits_of(std::max(nc, 1LL));
   if (m > 0) {
@@ -394,9 +394,9 @@ int csr_transpose_arrays(const tbsla_hip_matrix_s *src, int **rowptr_t, int **co
   }
   cleanup();
   if (st) {
-    hipFree(rp);
-    hipFree(ci);
-    hipFree(v);
+    (void)hipFree(rp);
+    (void)hipFree(ci);
+    (void)hipFree(v);
     return st;
   }
   *rowptr_t = rp;
@@ -426,7 +426,7 @@ extern "C" int tbsla_hip_fill_poisson3d(tbsla_hip_matrix_t m, int nx, int ny, in
   k_poisson_count<<<sgrid(n), kBlock, 0, s>>>(g, n, cnt);
   long long total = 0;
   int st = counts_to_rowptr(cnt, m->rowptr, n, &total, s);
-  hipFree(cnt);
+  (void)hipFree(cnt);
   if (st) return st;
   TBSLA_TRY(device_alloc((void **)&m->colidx, (size_t)total * 4));
   TBSLA_TRY(device_alloc((void **)&m->values, (size_t)total * 8));
@@ -478,7 +478,7 @@ extern "C" int tbsla_hip_fill_lognormal_field(double *d_kappa, int nx, int ny, i
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(d_kappa, tmp, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
   return TBSLA_HIP_SUCCESS;
 }
@@ -516,11 +516,11 @@ extern "C" int tbsla_hip_fill_sym_cqmat(tbsla_hip_matrix_t m, int n, int c, doub
   unsigned long long *keys = nullptr;
   int *pay = nullptr, *cnt = nullptr, *start = nullptr;
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    hipFree(keys);
-    hipFree(pay);
-    hipFree(cnt);
-    hipFree(start);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(keys);
+    (void)hipFree(pay);
+    (void)hipFree(cnt);
+    (void)hipFree(start);
     tbsla_hip_destroy(B);
   };
   const long long m2 = std::max(2 * mb, 1LL);
@@ -531,7 +531,7 @@ extern "C" int tbsla_hip_fill_sym_cqmat(tbsla_hip_matrix_t m, int n, int c, doub
     cleanup();
     return st;
   }
-  hipMemsetAsync(cnt, 0, (size_t)(n + 1) * 4, s);
+  (void)hipMemsetAsync(cnt, 0, (size_t)(n + 1) * 4, s);
   k_sym_keys<<<sgrid(n), kBlock, 0, s>>>(n, mb, B->rowptr, B->colidx, cb, sentinel, keys, pay, cnt);
   if (2 * mb > 1) st = radix_sort_pairs(keys, pay, 2 * mb, 2 * cb, s);
   long long total = 0;

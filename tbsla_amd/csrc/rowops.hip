@@ -457,7 +457,7 @@ unsigned lane_grid(long long n, int L) {
 template <typename T>
 struct Buf {
   T *p = nullptr;
-  ~Buf() { if (p) hipFree(p); }
+  ~Buf() { if (p) (void)hipFree(p); }
   int alloc(long long n) { return device_alloc((void **)&p, (size_t)std::max(n, 1LL) * sizeof(T)); }
 };
 

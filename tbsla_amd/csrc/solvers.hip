@@ -450,7 +450,7 @@ unsigned stride_grid(long long n) {
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
-  ~DevBuf() { if (p) hipFree(p); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
   int alloc(long long n) { return device_alloc((void **)&p, (size_t)std::max(n, 1LL) * sizeof(T)); }
 };
 
@@ -479,7 +479,7 @@ extern "C" int tbsla_hip_values_changed(tbsla_hip_matrix_t m, void *stream) {
   // the COO/SCOO row pointer is derived from the row array: rebuilt (and the
   // triplets' order checked again) at the next SpMV
   if (m->coo_rowptr) {
-    hipFree(m->coo_rowptr);
+    (void)hipFree(m->coo_rowptr);
     m->coo_rowptr = nullptr;
   }
   if (m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO) m->row_sorted = 1;
@@ -751,11 +751,11 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   auto cleanup = [&]() {
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (own) {
-      hipStreamSynchronize(own);
-      hipStreamDestroy(own);
+      (void)hipStreamSynchronize(own);
+      (void)hipStreamDestroy(own);
     }
   };
   bool graphs = m->split == nullptr;

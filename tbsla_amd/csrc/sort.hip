@@ -198,11 +198,11 @@ int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int ke
   long long *d_total = nullptr;
   int st = TBSLA_HIP_SUCCESS;
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    hipFree(keys_alt);
-    hipFree(pay_alt);
-    hipFree(hist);
-    hipFree(ws);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(keys_alt);
+    (void)hipFree(pay_alt);
+    (void)hipFree(hist);
+    (void)hipFree(ws);
   };
   if ((st = device_alloc((void **)&keys_alt, (size_t)n * 8)) ||
       (st = device_alloc((void **)&pay_alt, (size_t)n * 4)) ||
@@ -226,8 +226,8 @@ int radix_sort_pairs(unsigned long long *keys, int *payload, long long n, int ke
     std::swap(pin, pout);
   }
   if (st == TBSLA_HIP_SUCCESS && kin != keys) {
-    hipMemcpyAsync(keys, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
-    hipMemcpyAsync(payload, pin, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(keys, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(payload, pin, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
   }
   hipError_t e = hipGetLastError();
   cleanup();
@@ -247,8 +247,8 @@ static int coo_order(const tbsla_hip_matrix_s *coo, int *perm, hipStream_t s) {
   k_coo_keys<<<grid_for(n), kBlock, 0, s>>>(coo->rowidx, coo->colidx, n, col_bits, keys);
   k_iota<<<grid_for(n), kBlock, 0, s>>>(perm, n);
   int st = radix_sort_pairs(keys, perm, n, col_bits + row_bits, s);
-  hipStreamSynchronize(s);
-  hipFree(keys);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(keys);
   return st;
 }
 
@@ -327,8 +327,8 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
       k_iota<<<grid_for(n), kBlock, 0, s>>>(perm, n);
       // key = row only (col_bits 0 and row OR row == row)
       int st = radix_sort_pairs(keys, perm, n, bits_for(std::max(coo->n_row, 1)), s);
-      hipStreamSynchronize(s);
-      hipFree(keys);
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(keys);
       if (st) return st;
     }
     const size_t cells = (size_t)rows * (size_t)dst->ln_col;
@@ -341,8 +341,8 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
     dst->stored = (long long)cells;
     dst->has_arrays = 1;
     TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-    hipFree(perm);
-    hipFree(rowptr);
+    (void)hipFree(perm);
+    (void)hipFree(rowptr);
     return TBSLA_HIP_SUCCESS;
   }
 
@@ -355,8 +355,8 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
     k_gather_sorted<<<grid_for(n), kBlock, 0, s>>>(perm, coo->colidx, coo->values, n, scol, sval);
     TBSLA_HIP_LAUNCH_CHECK();
   }
-  hipStreamSynchronize(s);
-  hipFree(perm);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(perm);
   if (format == TBSLA_HIP_CSR) {
     dst->rowptr = rowptr;
     dst->colidx = scol;
@@ -377,7 +377,7 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
   int width = 0;
   TBSLA_HIP_CHECK(hipMemcpyAsync(&width, d_w, 4, hipMemcpyDeviceToHost, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-  hipFree(lens);
+  (void)hipFree(lens);
   dst->max_col = width;
   const size_t slots = (size_t)rows * (size_t)width;
   TBSLA_TRY(device_alloc((void **)&dst->colidx, std::max<size_t>(slots, 1) * 4));
@@ -389,8 +389,8 @@ extern "C" int tbsla_hip_convert(tbsla_hip_matrix_t dst, int format, tbsla_hip_m
   dst->stored = (long long)slots;
   dst->has_arrays = 1;
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-  hipFree(rowptr);
-  hipFree(scol);
-  hipFree(sval);
+  (void)hipFree(rowptr);
+  (void)hipFree(scol);
+  (void)hipFree(sval);
   return TBSLA_HIP_SUCCESS;
 }

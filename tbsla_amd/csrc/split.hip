@@ -83,25 +83,25 @@ struct SplitCsr {
   unsigned short *c16 = nullptr, *q16 = nullptr, *d16 = nullptr, *rs16 = nullptr;
   int *sst = nullptr, *pst = nullptr, *pb = nullptr;
   ~SplitCsr() {
-    hipDeviceSynchronize();
-    hipFree(c16);
-    hipFree(q16);
-    hipFree(d16);
-    hipFree(rs16);
-    hipFree(sst);
-    hipFree(pst);
-    hipFree(pb);
-    hipFree(lrp);
-    hipFree(lci);
-    hipFree(lv);
-    hipFree(sr);
-    hipFree(sc);
-    hipFree(sv);
-    hipFree(boff);
-    hipFree(P);
-    hipFree(R);
-    hipFree(doff);
-    hipFree(dv);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(c16);
+    (void)hipFree(q16);
+    (void)hipFree(d16);
+    (void)hipFree(rs16);
+    (void)hipFree(sst);
+    (void)hipFree(pst);
+    (void)hipFree(pb);
+    (void)hipFree(lrp);
+    (void)hipFree(lci);
+    (void)hipFree(lv);
+    (void)hipFree(sr);
+    (void)hipFree(sc);
+    (void)hipFree(sv);
+    (void)hipFree(boff);
+    (void)hipFree(P);
+    (void)hipFree(R);
+    (void)hipFree(doff);
+    (void)hipFree(dv);
   }
 };
 
@@ -747,7 +747,7 @@ static int pick_block_rows(const int *rp, long long n, hipStream_t s, int *h_out
                   hipMemcpyAsync(hb.data(), d, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s) ==
                       hipSuccess &&
                   hipStreamSynchronize(s) == hipSuccess;
-  hipFree(d);
+  (void)hipFree(d);
   if (!ok) return fail(TBSLA_HIP_ERR_HIP, "diagonal layout: block sizes");
   for (int h = 4096; h >= kWave && *h_out == 0; h /= 2) {
     const long long step = h / kWave;
@@ -779,18 +779,18 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
   void *ws = nullptr;
   int tot32[2] = {0, 0};  // rptr[n] and lrp[n], copied from here before the stream syncs
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    hipFree(lcnt);
-    hipFree(rcnt);
-    hipFree(rptr);
-    hipFree(rc);
-    hipFree(pay);
-    hipFree(scount);
-    hipFree(rv);
-    hipFree(keys);
-    hipFree(ws);
-    hipFree(hist);
-    hipFree(dmap);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(lcnt);
+    (void)hipFree(rcnt);
+    (void)hipFree(rptr);
+    (void)hipFree(rc);
+    (void)hipFree(pay);
+    (void)hipFree(scount);
+    (void)hipFree(rv);
+    (void)hipFree(keys);
+    (void)hipFree(ws);
+    (void)hipFree(hist);
+    (void)hipFree(dmap);
   };
   auto failed = [&](int st) {
     cleanup();
@@ -816,7 +816,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     if ((st = device_alloc((void **)&hist, (size_t)nb * 4)) ||
         (st = device_alloc((void **)&dmap, (size_t)nb * 4)))
       return failed(st);
-    hipMemsetAsync(hist, 0, (size_t)nb * 4, s);
+    (void)hipMemsetAsync(hist, 0, (size_t)nb * 4, s);
     const long long step = std::max(1LL, n / 262144);  // rows sampled: at most 262144
     k_diag_hist<<<256, kBlock, 0, s>>>(n, step, m->f_col, window, src, m->colidx, m->values, hist);
     if (hipMemcpyAsync(h.data(), hist, (size_t)nb * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -841,9 +841,9 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     if ((st = device_alloc((void **)&sp->doff, offs.size() * 4)) ||
         (st = device_alloc((void **)&sp->dv, dia_bytes)))
       return failed(st);
-    hipMemcpyAsync(dmap, map.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(sp->doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s);
-    hipMemsetAsync(sp->dv, 0, dia_bytes, s);
+    (void)hipMemcpyAsync(dmap, map.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(sp->doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s);
+    (void)hipMemsetAsync(sp->dv, 0, dia_bytes, s);
   }
   k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, src, m->colidx, m->values, dmap,
                                             lcnt, rcnt);
@@ -855,7 +855,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
         hipStreamSynchronize(s) != hipSuccess)
       return failed(fail(TBSLA_HIP_ERR_HIP, "split layout: scan"));
     tot32[k] = (int)tot[k];
-    hipMemcpyAsync(ptrs[k] + n, &tot32[k], 4, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(ptrs[k] + n, &tot32[k], 4, hipMemcpyHostToDevice, s);
   }
   sp->nl = tot[0];
   sp->nr = tot[1];
@@ -878,7 +878,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
       (st = device_alloc((void **)&keys, (size_t)nr1 * 8)) ||
       (st = device_alloc((void **)&pay, (size_t)nr1 * 4)))
     return failed(st);
-  hipMemsetAsync(scount, 0, (size_t)(std::max(slices, s_lds) + 1) * 4, s);
+  (void)hipMemsetAsync(scount, 0, (size_t)(std::max(slices, s_lds) + 1) * 4, s);
   const int rb = bits_of(std::max(n, 1LL));
   k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, H > 0 ? kSliceCols : 0,
                                            rb, src, m->colidx, m->values, dmap, sp->ndiag, sp->lrp,

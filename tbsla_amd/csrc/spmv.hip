@@ -301,7 +301,7 @@ int coo_rowptr(tbsla_hip_matrix_s *m, long long n, bool *ok, hipStream_t s) {
   TBSLA_TRY(device_alloc((void **)&rp, (size_t)(nrows + 1) * 4));
   int st = device_alloc((void **)&flag, 4);
   if (st) {
-    hipFree(rp);
+    (void)hipFree(rp);
     return st;
   }
   hipError_t e = hipMemsetAsync(flag, 0, 4, s);
@@ -313,13 +313,13 @@ int coo_rowptr(tbsla_hip_matrix_s *m, long long n, bool *ok, hipStream_t s) {
   int unsorted = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&unsorted, flag, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(flag);
+  (void)hipFree(flag);
   if (e != hipSuccess) {
-    hipFree(rp);
+    (void)hipFree(rp);
     return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
   }
   if (unsorted) {
-    hipFree(rp);
+    (void)hipFree(rp);
     m->row_sorted = 0;
     return TBSLA_HIP_SUCCESS;
   }
