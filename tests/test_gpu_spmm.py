@@ -42,8 +42,8 @@ def stochastic_pair(n, c, q, seed):
 
 @pytest.mark.parametrize("n", [20000, 70000])
 def test_ppr_columns_match_single_vector_oracle(n):
-    """n = 70000 takes the XCD-grouped row walk (a full grid of workgroups);
-    n = 20000 the one-slot-per-row walk of a smaller grid."""
+    """n = 70000: 1094 tiles over 768 workgroups (runs of two tiles, the ring
+    refilled once per workgroup); n = 20000: 313 tiles, one per workgroup."""
     o, g = stochastic_pair(n, 16, 0.1, 2)
     rng = np.random.default_rng(1)
     src = rng.choice(n, 64, replace=False).astype(np.int32)
@@ -81,3 +81,61 @@ def test_ppr_fixed_point_direct_solve():
         exact = np.linalg.solve(np.eye(n) - 0.85 * A, e)
         exact /= exact.sum()
         assert np.abs(X[:, j] - exact).sum() < 1e-12
+
+
+def ppr_against_oracle(n, rp, ci, va, src, iters=12):
+    geom = dict(n_row=n, n_col=n, f_row=0, f_col=0, ln_row=n, ln_col=n, pr=0, pc=0, NR=1, NC=1,
+                nnz=len(ci))
+    g = T.Matrix.from_arrays(T.CSR, geom, rp, ci, va)
+    o = O.csr_from_arrays(n, n, rp, ci, va)
+    X, it, _ = g.personalized_page_rank(src, 0.85, 1e-30, iters)
+    assert it == iters
+    worst = 0.0
+    for j in range(0, 64, 5):
+        b, ito, _ = o.page_rank_source(int(src[j]), 0.85, 1e-30, iters)
+        assert ito == iters
+        worst = max(worst, float(np.max(np.abs(X[:, j] - b)) / np.max(np.abs(b))))
+    assert worst < 1e-12, worst
+
+
+def test_ppr_long_rows_and_many_remote_entries():
+    """Rows of 100 entries (past the first 64-entry chunk) with 30 % rewired
+    columns (more remote rows than the pipeline's slots), n = 5000."""
+    n = 5000
+    gen = O.fill_cqmat(O.CSR, n, n, 100, 0.3, 4, value_seed=4)
+    gen.make_stochastic()
+    rp, ci, va = gen.arrays()
+    src = np.random.default_rng(3).choice(n, 64, replace=False).astype(np.int32)
+    ppr_against_oracle(n, rp, ci, va, src)
+
+
+def test_ppr_band_off_the_diagonal():
+    """Every row's entries sit 40..49 columns right of the diagonal (wrapping
+    at the end), so a tile's own rows lie outside its LDS window: the old
+    values come from HBM.  Also a row with no entries."""
+    n = 3000
+    rows, cols = [], []
+    for i in range(n):
+        if i == 1234:
+            continue
+        for k in range(10):
+            rows.append(i)
+            cols.append((i + 40 + k) % n)
+    rows, cols = np.array(rows), np.array(cols)
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp)
+    va = O.unit_vector(6, len(cols)) + 0.1
+    src = np.random.default_rng(4).choice(n, 64, replace=False).astype(np.int32)
+    ppr_against_oracle(n, rp, cols, va, src)
+
+
+@pytest.mark.parametrize("n", [64, 100, 130])
+def test_ppr_small_blocks(n):
+    """One or three tiles: a partial last tile and a ring larger than the
+    block."""
+    gen = O.fill_cqmat(O.CSR, n, n, 6, 0.2, 2, value_seed=2)
+    gen.make_stochastic()
+    rp, ci, va = gen.arrays()
+    src = np.random.default_rng(n).choice(n, 64, replace=False).astype(np.int32)
+    ppr_against_oracle(n, rp, ci, va, src)
