@@ -339,11 +339,14 @@ int tbsla_hip_page_rank_max_norm(tbsla_hip_matrix_t m, double beta, double epsil
  * stream of the library's own when `stream` is NULL).  Instance k follows a
  * tbsla_hip_page_rank of its block alone bit for bit when both SpMVs use
  * the same row kernel (tbsla_hip_set_csr_kernel).  d_b receives every
- * instance's vector (ln_row doubles); h_iterations / h_errors (nullable,
- * ninst entries) the per-instance counts and final errors. */
+ * instance's vector (ln_row doubles); h_iterations / h_errors / h_time_us
+ * (nullable, ninst entries) the per-instance counts, final errors and solve
+ * times (device wall clock from the first iteration to the test that ended
+ * the instance, microseconds; -1 if it did not end). */
 int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, const int *h_row_offsets,
                                 double beta, double epsilon, int max_iterations, double *d_b,
-                                int *h_iterations, double *h_errors, void *stream);
+                                int *h_iterations, double *h_errors, double *h_time_us,
+                                void *stream);
 
 enum { TBSLA_HIP_CG_FP64 = 0, TBSLA_HIP_CG_MIXED = 1 };
 enum { TBSLA_HIP_PRECOND_NONE = 0, TBSLA_HIP_PRECOND_JACOBI = 1, TBSLA_HIP_PRECOND_FACTORED = 2 };

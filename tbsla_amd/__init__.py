@@ -119,7 +119,8 @@ SIGNATURES = [
                                       C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_cg", C.c_int, [_vp, _vp, _vp, C.c_void_p, _ip, C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_page_rank_batched", C.c_int, [_vp, C.c_int, _ip, C.c_double, C.c_double, C.c_int,
-                                              _vp, _ip, C.POINTER(C.c_double), _vp]),
+                                              _vp, _ip, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), _vp]),
     ("tbsla_hip_precond_net_param_count", C.c_int, []),
     ("tbsla_hip_precond_net_create", C.c_int, [C.POINTER(C.c_void_p), C.c_ulonglong]),
     ("tbsla_hip_precond_net_destroy", C.c_int, [_vp]),
@@ -544,18 +545,22 @@ class Matrix:
                                                  C.byref(it), C.byref(err), None))
         return db.to_numpy(np.float64, n), it.value, err.value
 
-    def page_rank_batched(self, row_offsets, beta=0.85, epsilon=1e-10, max_iterations=10000):
+    def page_rank_batched(self, row_offsets, beta=0.85, epsilon=1e-10, max_iterations=10000,
+                          times=False):
         """PageRank of each diagonal block [row_offsets[k], row_offsets[k+1]);
-        returns (b for all rows, iterations per instance, final errors)."""
+        returns (b for all rows, iterations per instance, final errors), and
+        the per-instance solve times in microseconds when `times`."""
         offs = np.ascontiguousarray(row_offsets, np.int32)
         k = len(offs) - 1
         n = self.geometry().ln_row
         db = DeviceBuffer(max(n, 1) * 8)
-        its, errs = np.zeros(k, np.int32), np.zeros(k)
+        its, errs, tus = np.zeros(k, np.int32), np.zeros(k), np.zeros(k)
+        dp = C.POINTER(C.c_double)
         check(lib().tbsla_hip_page_rank_batched(
             self.h, k, offs.ctypes.data_as(_ip), beta, epsilon, max_iterations, db.ptr,
-            its.ctypes.data_as(_ip), errs.ctypes.data_as(C.POINTER(C.c_double)), None))
-        return db.to_numpy(np.float64, n), its, errs
+            its.ctypes.data_as(_ip), errs.ctypes.data_as(dp), tus.ctypes.data_as(dp), None))
+        out = (db.to_numpy(np.float64, n), its, errs)
+        return out + (tus,) if times else out
 
     def spmm(self, B) -> np.ndarray:
         """MatrixCSR::dense_multiply: C = A B (B row-major ln_col x k)."""

@@ -75,13 +75,16 @@ def page_rank_bench(count: int = 512, seed: int = 0, beta: float = 0.85,
     del dx, dy
     bd.page_rank_batched(offs, beta, epsilon, 10000)
     t0 = time.perf_counter()
-    _, its, errs = bd.page_rank_batched(offs, beta, epsilon, 10000)
+    _, its, errs, tus = bd.page_rank_batched(offs, beta, epsilon, 10000, times=True)
     ms = (time.perf_counter() - t0) * 1e3
     spmv_bytes = 12 * g.stored + 4 * (g.ln_row + 1) + 16 * g.ln_row
     return {"instances": count, "rows": int(offs[-1]), "nnz": g.stored,
             "csr_kernel": bd.get_csr_kernel(), "csr_layout": bd.get_csr_layout()["layout"],
             "build_s": round(build_s, 2), "batch_ms": round(ms, 3),
             "instances_per_s": round(count / (ms * 1e-3), 1),
+            "per_instance_us_p50": round(float(np.percentile(tus, 50)), 1),
+            "per_instance_us_p99": round(float(np.percentile(tus, 99)), 1),
+            "per_instance_us_max": round(float(tus.max()), 1),
             "iterations_p50": float(np.median(its)), "iterations_max": int(its.max()),
             "max_error": float(errs.max()),
             "ms_per_iteration": round(ms / max(int(its.max()), 1), 4),

@@ -333,12 +333,16 @@ struct PrInstance {
   int done;     // converged or at the iteration cap
   long long row0, n;
   int part0, nb;
+  long long t_done;  // device wall clock at the test that finished the instance
 };
 
 struct PrCommon {
   int ndone;     // instances finished
   int all_done;  // every instance finished: the SpMV row kernel returns at once
+  long long t0;  // device wall clock before the first iteration
 };
+
+__global__ void k_prb_start(PrCommon *__restrict__ com) { com->t0 = wall_clock64(); }
 
 __global__ __launch_bounds__(kBlock) void k_prb_update(PrInstance *__restrict__ inst,
                                                        const int *__restrict__ block_inst,
@@ -390,6 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_prb_finalize(PrInstance *__restrict_
     I.iters += 1;
     if (e < epsilon || I.iters >= max_iterations) {
       I.done = 1;
+      I.t_done = wall_clock64();
       done_flags[blockIdx.x] = 1;  // the SpMV skips the instance's rows from now on
       if (atomicAdd(&com->ndone, 1) + 1 == ninst) com->all_done = 1;
     }
@@ -671,7 +676,7 @@ extern "C" int tbsla_hip_page_rank_max_norm(tbsla_hip_matrix_t m, double beta, d
 extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, const int *h_row_offsets,
                                            double beta, double epsilon, int max_iterations,
                                            double *d_b, int *h_iterations, double *h_errors,
-                                           void *stream) {
+                                           double *h_time_us, void *stream) {
   if (!m || !h_row_offsets || !d_b) return fail(TBSLA_HIP_ERR_INVALID, "null argument");
   if (m->format != TBSLA_HIP_CSR)
     return fail(TBSLA_HIP_ERR_UNSUPPORTED, "batched PageRank needs a CSR block");
@@ -696,6 +701,7 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
     I.n = n;
     I.part0 = (int)hb.size();
     I.nb = (int)red_grid(n);
+    I.t_done = 0;
     for (int b = 0; b < I.nb; b++) hb.push_back(k);
   }
   const int nblocks = (int)hb.size();
@@ -715,7 +721,7 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   TBSLA_TRY(d_offs.alloc(ninst + 1));
   TBSLA_TRY(done_flags.alloc(ninst));
   TBSLA_TRY(com.alloc(1));
-  const PrCommon hc{0, 0};
+  const PrCommon hc{0, 0, 0};
   TBSLA_HIP_CHECK(hipMemcpyAsync(inst.p, hi.data(), sizeof(PrInstance) * ninst, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(binst.p, hb.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, s));
   TBSLA_HIP_CHECK(hipMemcpyAsync(com.p, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
@@ -781,6 +787,7 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
     }
   }
   PrCommon h{};
+  k_prb_start<<<1, 1, 0, q>>>(com.p);
   int launched = 0;
   while (launched < max_iterations) {
     hipError_t e = hipSuccess;
@@ -810,9 +817,17 @@ extern "C" int tbsla_hip_page_rank_batched(tbsla_hip_matrix_t m, int ninst, cons
   if (e == hipSuccess) e = hipStreamSynchronize(q);
   cleanup();
   if (e != hipSuccess) return fail(TBSLA_HIP_ERR_HIP, hipGetErrorString(e));
+  int dev = 0, khz = 0;
+  if (h_time_us &&
+      (hipGetDevice(&dev) != hipSuccess ||
+       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess))
+    khz = 0;
   for (int k = 0; k < ninst; k++) {
     if (h_iterations) h_iterations[k] = hi[k].iters;
     if (h_errors) h_errors[k] = hi[k].err;
+    if (h_time_us)
+      h_time_us[k] = (hi[k].t_done > 0 && khz > 0) ? (double)(hi[k].t_done - h.t0) * 1000.0 / khz
+                                                   : -1.0;
   }
   return TBSLA_HIP_SUCCESS;
 }

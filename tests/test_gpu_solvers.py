@@ -257,7 +257,10 @@ def test_page_rank_batched_equals_single():
     sizes = [(4096, 8), (777, 31), (20000, 16), (3, 2), (9000, 20), (50000, 32)]
     parts, bd, offs = stochastic_parts(sizes)
     bd.set_csr_kernel(8, 1, 1, 1)
-    b, its, errs = bd.page_rank_batched(offs, 0.85, 1e-10, 500)
+    b, its, errs, tus = bd.page_rank_batched(offs, 0.85, 1e-10, 500, times=True)
+    # solve times: positive, and an instance that needs more iterations ends later
+    assert np.all(tus > 0)
+    assert tus[np.argmax(its)] >= tus[np.argmin(its)]
     for k, (g, (n, c)) in enumerate(zip(parts, sizes)):
         g.set_csr_kernel(8, 1, 1, 1)
         bs, it, err = g.page_rank(0.85, 1e-10, 500)
