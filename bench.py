@@ -299,20 +299,18 @@ def formats_section(iters: int = 20):
         if fmt == T.ELL:
             nbytes = 12 * g.ln_row * g.max_col + 8 * g.ln_col + 8 * g.ln_row
         else:
-            # the timed kernel reads the column and value arrays and a row
-            # pointer it derives once from the row array (kept with the
-            # block), then writes y over the zeroed result: 12 bytes per
-            # entry, not the format's 16
+            # row-sorted triplets are read through a row pointer derived once
+            # from the row array (kept with the block): the CSR bytes of the
+            # same entries, 12 per entry, not the format's 16
             rows = g.n_row if fmt == T.COO else g.ln_row
             nbytes = 12 * g.stored + 4 * (rows + 1) + 8 * g.ln_col + 8 * rows
         extra = {}
         if fmt != T.ELL:
             extra = {"bytes_format": 16 * g.stored + 8 * g.ln_col + 8 * g.ln_row}
-        if fmt == T.ELL:
-            # its column-major kernel, or the split / diagonal layout when one
-            # is at least 3 % faster
-            m.autotune(dy, dx, 5)
-            extra = {"layout": m.get_csr_layout()["layout"]}
+        # ELL's column-major kernel or COO / SCOO's row kernel, or the split /
+        # diagonal layout when one is at least 3 % faster
+        m.autotune(dy, dx, 5)
+        extra["layout"] = m.get_csr_layout()["layout"]
         out[name] = {"bytes": nbytes, **extra, **timed(m, nbytes)}
         del m
     m = T.Matrix(T.DENSE).fill_cqmat(N, N, C, Q, 0, 0, 0, 256, 256, value_seed=0)

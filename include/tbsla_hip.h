@@ -501,9 +501,15 @@ int tbsla_hip_set_csr_kernel(tbsla_hip_matrix_t m, int lanes, int rows, int vec,
                              int nontemporal);
 int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *vec,
                              int *nontemporal);
-/* CSR / ELL execution layout (no counterpart in the reference; results
- * within rounding of the row-major sum; ELL padding slots count as
- * entries).  TBSLA_HIP_CSR_ROWS: the CSR arrays as they are.
+/* CSR / ELL / COO / SCOO execution layout (no counterpart in the
+ * reference; results within rounding of the row-major sum; ELL padding
+ * slots count as entries).  COO and SCOO take it over the rows from their
+ * first to their last triplet's row through a row pointer derived from the
+ * row array, and only when the triplets are sorted by row (else
+ * TBSLA_HIP_ERR_UNSUPPORTED); their diagonals are measured from the global
+ * row (COO) or the row less f_row (SCOO), as CSR's, and the layout serves
+ * their SpMV (Ax accumulates through its own path).  DENSE has none.
+ * TBSLA_HIP_CSR_ROWS: the arrays as they are.
  * TBSLA_HIP_CSR_SPLIT: entries with |col - row| <= window stay in row order;
  * the others ("remote") are grouped by column slice (slices a multiple of 8,
  * at most 64) and multiplied by one XCD per slice, whose L2 then holds that
@@ -519,8 +525,9 @@ int tbsla_hip_get_csr_kernel(tbsla_hip_matrix_t m, int *lanes, int *rows, int *v
  * block of 64..4096 rows holds at most 14336 remote entries (slices is then
  * unused); otherwise through L2 slices as the split layout.
  * Inf and NaN in x reach the same rows as in the row-major sum.  Both copy
- * the entries once, are rebuilt by normalize_cols and dropped when the
- * arrays change; a block without columns cannot take the diagonal layout. */
+ * the entries once, are rebuilt by normalize_cols and values_changed and
+ * dropped when the arrays change; a block without columns cannot take the
+ * diagonal layout. */
 enum { TBSLA_HIP_CSR_ROWS = 0, TBSLA_HIP_CSR_SPLIT = 1, TBSLA_HIP_CSR_DIAG = 2 };
 int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int slices,
                              void *stream);
@@ -541,8 +548,9 @@ int tbsla_hip_get_csr_remote_form(tbsla_hip_matrix_t m, int *block_rows);
  * with 3 x `iters` launches before the choice) and keeps the fastest; a
  * split or diagonal layout is kept only when it is at least 3 % faster than
  * the row layout.
- * ELL compares its column-major kernel with the split and diagonal layouts
- * by the same rule; other formats have a single schedule.  *best_us
+ * ELL compares its column-major kernel, and row-sorted COO / SCOO their row
+ * kernel, with the split and diagonal layouts by the same rule; DENSE has a
+ * single schedule.  *best_us
  * receives its time per SpMV. */
 int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int iters,
                        float *best_us, void *stream);

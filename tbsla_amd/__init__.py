@@ -450,9 +450,11 @@ class Matrix:
         the SpMV kernel and execution layout (the CG engine then reads the
         diagonal layout when it holds every entry); returns us per SpMV."""
         g = self.geometry()
-        dv = DeviceBuffer(max(g.ln_col, 1) * 8)
+        # COO reads x by the global column and writes y by the global row
+        nx = g.n_col if self.fmt == COO else g.ln_col
+        dv = DeviceBuffer(max(nx, 1) * 8)
         dr = DeviceBuffer(max(g.n_row if self.fmt == COO else g.ln_row, 1) * 8)
-        check(lib().tbsla_hip_fill_unit_vector(dv.ptr, g.ln_col, 1, None))
+        check(lib().tbsla_hip_fill_unit_vector(dv.ptr, nx, 1, None))
         return self.autotune(dr, dv, iters)
 
     def autotune(self, dr: DeviceBuffer, dv: DeviceBuffer, iters: int = 10,

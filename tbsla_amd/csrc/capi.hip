@@ -547,18 +547,26 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
   if (!m || iters < 1) return fail(TBSLA_HIP_ERR_INVALID, "bad argument");
   float ms = 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (m->format == TBSLA_HIP_ELL && m->has_arrays && m->max_col > 0 && m->ln_row > 0) {
-    // ELL: its column-major kernel, or the split or diagonal layout (8 or
-    // 16 slices)
+  const bool coo = m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO;
+  if ((m->format == TBSLA_HIP_ELL && m->has_arrays && m->max_col > 0 && m->ln_row > 0) ||
+      (coo && m->has_arrays && m->stored > 0 && m->row_sorted)) {
+    // ELL: its column-major kernel; COO / SCOO: the row kernel over their
+    // triplets' row pointer; or the split or diagonal layout (8 or 16 slices)
     split_release(m);
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
     TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
+    if (coo && !m->coo_rowptr) {  // the timing found the triplets unsorted
+      if (best_us) *best_us = ms * 1000.0f / iters;
+      return TBSLA_HIP_SUCCESS;
+    }
     float best = ms;
     const float ell_ms = ms;
     int best_slices = 0, best_layout = 0, best_window = 0;
     for (int layout : {TBSLA_HIP_CSR_SPLIT, TBSLA_HIP_CSR_DIAG})
       for (int slices : {16, 8}) {
-        if (layout == TBSLA_HIP_CSR_DIAG && m->ln_col == 0) break;
+        if (layout == TBSLA_HIP_CSR_DIAG && (coo ? (m->format == TBSLA_HIP_COO ? m->n_col : m->ln_col)
+                                                 : m->ln_col) == 0)
+          break;
         const int window = layout == TBSLA_HIP_CSR_DIAG ? kDiagWindow : 64;
         TBSLA_TRY(split_build(m, window, slices, s, layout));
         TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));

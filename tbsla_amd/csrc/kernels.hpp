@@ -50,6 +50,10 @@ int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *lay
 // rows per block of the diagonal layout's LDS-slice remote form, or 0 (no
 // layout, no remote entries, or the L2-slice form, whose slice count matters)
 int split_lds_block_rows(const tbsla_hip_matrix_s *m);
+// The row pointer of a COO / SCOO block's n row-sorted triplets over their
+// row range (built once and kept with the block); *ok false when the
+// triplets are not sorted by row
+int coo_rowptr(tbsla_hip_matrix_s *m, long long n, bool *ok, hipStream_t s);
 // The diagonal layout's arrays when it holds every entry (no remote part):
 // offsets, tiled values (dia_at), count, values' length, and the build's
 // id (unique in the process: a copy made from one build is stale for any

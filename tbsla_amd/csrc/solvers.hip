@@ -487,7 +487,17 @@ extern "C" int tbsla_hip_values_changed(tbsla_hip_matrix_t m, void *stream) {
     (void)hipFree(m->coo_rowptr);
     m->coo_rowptr = nullptr;
   }
-  if (m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO) m->row_sorted = 1;
+  if (m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO) {
+    m->row_sorted = 1;
+    // a derived layout over the triplets' rows is rebuilt from the new row
+    // pointer, or dropped when the triplets are no longer sorted by row
+    if (m->split) {
+      const long long n = (m->format == TBSLA_HIP_SCOO && m->nnz == 0) ? 0 : m->stored;
+      bool ok = false;
+      if (m->has_arrays && n > 0) TBSLA_TRY(coo_rowptr(m, n, &ok, s));
+      if (!ok) split_release(m);
+    }
+  }
   return values_changed(m, s);
 }
 

@@ -53,6 +53,7 @@
 //                                      block; row starts within the block
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <vector>
 
 #include "common.hpp"
@@ -126,7 +127,32 @@ struct RowSrc {
   }
 };
 
-// Where an entry of row r goes.  Split layout (dmap null): near-diagonal
+// The rows a derived layout is built over and launched on: a CSR or ELL
+// block's own rows, or a row-sorted COO / SCOO block's triplet rows through
+// its derived row pointer (rows coo_row0 ..; COO indexes x by the global
+// column, SCOO by the column less f_col, as their SpMVs do).  Diagonals are
+// offsets c - (r + r0) of the x index c from the layout row r shifted by r0,
+// the row of x that layout row 0 lines up with: 0 for CSR and ELL, the
+// global row coo_row0 for COO, coo_row0 - f_row for SCOO, so that a grid
+// block's diagonals are the whole matrix's.
+struct LayoutRows {
+  long long n;   // rows
+  long long nc;  // entries of x
+  int f_col;     // subtracted from a column index to index x
+  const int *rp;
+  int ell_w;
+  long long r0;
+};
+LayoutRows layout_rows(const tbsla_hip_matrix_s *m) {
+  if (m->format == TBSLA_HIP_COO || m->format == TBSLA_HIP_SCOO) {
+    const bool coo = m->format == TBSLA_HIP_COO;
+    return {m->coo_nrows, coo ? m->n_col : m->ln_col, coo ? 0 : m->f_col, m->coo_rowptr, 0,
+            coo ? m->coo_row0 : (long long)m->coo_row0 - m->f_row};
+  }
+  return {m->ln_row, m->ln_col, m->f_col, m->rowptr, m->format == TBSLA_HIP_ELL ? m->max_col : 0, 0};
+}
+
+// Where an entry of (shifted) row r goes.  Split layout (dmap null): near-diagonal
 // (returns 0) or remote (-1).  Diagonal layout: the slot of diagonal k
 // (returns k + 1) when c - r is a stored diagonal, the value is not zero and
 // the row's slot on it is still free (`used`), else remote; so a zero slot
@@ -142,8 +168,8 @@ __device__ inline int place_of(long long c, long long r, double v, int window,
   return k + 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, int window,
-                                                        RowSrc src, const int *__restrict__ ci,
+__global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, long long r0,
+                                                        int window, RowSrc src, const int *__restrict__ ci,
                                                         const double *__restrict__ v,
                                                         const int *__restrict__ dmap,
                                                         int *__restrict__ lcnt,
@@ -156,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, 
     for (int k = 0; k < cnt; k++) {
       const long long j = src.at(r, k);
       const long long c = (long long)max(ci[j] - f_col, 0);
-      if (place_of(c, r, v[j], window, dmap, used) >= 0) l++;
+      if (place_of(c, r + r0, v[j], window, dmap, used) >= 0) l++;
       else q++;
     }
     lcnt[r] = l;
@@ -164,12 +190,12 @@ __global__ __launch_bounds__(kBlock) void k_split_count(long long n, int f_col, 
   }
 }
 
-// entries per diagonal offset c - r in [-window, window] (nonzero values)
+// entries per diagonal offset c - (r + r0) in [-window, window] (nonzero values)
 // over the sampled rows r = 0, step, 2 step, ...; offsets within
 // kLdsHalf of the diagonal are counted in LDS and added once per workgroup
 constexpr int kLdsHalf = 4095;
 __global__ __launch_bounds__(kBlock) void k_diag_hist(long long n, long long step, int f_col,
-                                                      int window, RowSrc src,
+                                                      long long r0, int window, RowSrc src,
                                                       const int *__restrict__ ci,
                                                       const double *__restrict__ v,
                                                       int *__restrict__ hist) {
@@ -183,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_hist(long long n, long long ste
     const int cnt = src.count(r);
     for (int k = 0; k < cnt; k++) {
       const long long j = src.at(r, k);
-      const long long d = (long long)max(ci[j] - f_col, 0) - r;
+      const long long d = (long long)max(ci[j] - f_col, 0) - (r + r0);
       if (d < -window || d > window || !(v[j] != 0.0)) continue;
       if (d >= -kLdsHalf && d <= kLdsHalf) atomicAdd(lh + (d + kLdsHalf), 1);
       else atomicAdd(hist + (d + window), 1);
@@ -259,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_dia_rows(long long n, long long ncol
 // (slice, row) sort key: slice c / slice_cols when slice_cols > 0 (LDS
 // slices), else c slices / n_col
 __global__ __launch_bounds__(kBlock) void k_split_fill(
-    long long n, long long n_col, int f_col, int window, int slices, int slice_cols, int rb,
-    RowSrc src,
+    long long n, long long n_col, int f_col, long long r0, int window, int slices, int slice_cols,
+    int rb, RowSrc src,
     const int *__restrict__ ci, const double *__restrict__ v, const int *__restrict__ dmap,
     int ndiag, const int *__restrict__ lrp, const int *__restrict__ rptr, int *__restrict__ lci,
     double *__restrict__ lv, double *__restrict__ dv, int *__restrict__ rc,
@@ -276,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fill(
       // ELL clamps a negative index to 0 (MatrixELL.cpp:198-199)
       const long long c = (long long)max(ci[j] - f_col, 0);
       const double val = v[j];
-      const int at = place_of(c, r, val, window, dmap, used);
+      const int at = place_of(c, r + r0, val, window, dmap, used);
       if (at > 0) {
         dv[dia_at(r, at - 1, ndiag)] = val;
       } else if (at == 0) {
@@ -675,7 +701,7 @@ int split_diag_only(const tbsla_hip_matrix_s *m, const int **doff, const double 
   *doff = sp->doff;
   *dv = sp->dv;
   *ndiag = sp->ndiag;
-  *dv_len = (long long)sp->ndiag * ((m->ln_row + kWave - 1) / kWave) * kWave;
+  *dv_len = (long long)sp->ndiag * ((layout_rows(m).n + kWave - 1) / kWave) * kWave;
   return 1;
 }
 
@@ -690,12 +716,14 @@ void split_release(tbsla_hip_matrix_s *m) {
 int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *done,
                  hipStream_t s) {
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  const LayoutRows lr = layout_rows(m);
+  const long long n = lr.n, ncol = std::max(lr.nc, 1LL);
   if (sp->lds_h > 0) {
     k_slice_products<<<(unsigned)(sp->lds_s * sp->lds_g), kBigBlock, 0, s>>>(
-        m->ln_col, sp->lds_s, sp->lds_r, sp->lds_g, sp->sst, sp->pst, sp->c16, sp->q16, sp->sv, x,
+        lr.nc, sp->lds_s, sp->lds_r, sp->lds_g, sp->sst, sp->pst, sp->c16, sp->q16, sp->sv, x,
         sp->P, done);
     k_dia_rows_remote<<<(unsigned)sp->lds_r, kBigBlock, 0, s>>>(
-        m->ln_row, std::max(m->ln_col, 1), sp->ndiag, sp->lds_h, sp->doff, sp->dv, x, sp->pb,
+        n, ncol, sp->ndiag, sp->lds_h, sp->doff, sp->dv, x, sp->pb,
         sp->d16, sp->rs16, sp->P, y, done);
     TBSLA_HIP_LAUNCH_CHECK();
     return TBSLA_HIP_SUCCESS;
@@ -714,18 +742,18 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
   if (sp->layout == TBSLA_HIP_CSR_DIAG) {
     // no remote entries: nothing to combine
     if (sp->nr > 0)
-      k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(m->ln_row, sp->slices, sp->nblk,
+      k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(n, sp->slices, sp->nblk,
                                                              sp->boff, sp->sr, sp->P, sp->R, done);
-    k_dia_rows<<<sgrid(m->ln_row / 2), kBlock, 0, s>>>(m->ln_row, std::max(m->ln_col, 1), sp->ndiag,
+    k_dia_rows<<<sgrid(n / 2), kBlock, 0, s>>>(n, ncol, sp->ndiag,
                                                    sp->doff, sp->dv, x,
                                                    sp->nr > 0 ? sp->R : nullptr, y, done);
     TBSLA_HIP_LAUNCH_CHECK();
     return TBSLA_HIP_SUCCESS;
   }
-  k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(m->ln_row, sp->slices, sp->nblk, sp->boff,
+  k_remote_combine<<<(unsigned)sp->nblk, kBlock, 0, s>>>(n, sp->slices, sp->nblk, sp->boff,
                                                          sp->sr, sp->P, sp->R, done);
   // the near-diagonal entries with the block's CSR schedule, plus R[row]
-  CsrRowsArgs a{m->ln_row, sp->lrp, sp->lci, sp->lv, x, 0, y, done, sp->R};
+  CsrRowsArgs a{n, sp->lrp, sp->lci, sp->lv, x, 0, y, done, sp->R};
   const int L = pick_csr_lanes(m);
   csr_rows_launch<true>(a, pick_csr_vec(m), L, pick_csr_rows(m, L), pick_csr_nt(m), s);
   TBSLA_HIP_LAUNCH_CHECK();
@@ -761,8 +789,12 @@ static int pick_block_rows(const int *rp, long long n, hipStream_t s, int *h_out
 
 int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, int layout) {
   split_release(m);
-  const long long n = m->ln_row, nc = std::max(m->ln_col, 1);
+  const LayoutRows lr = layout_rows(m);
+  const long long n = lr.n, nc = std::max(lr.nc, 1LL);
+  const int f_col = lr.f_col;
   const bool diag = layout == TBSLA_HIP_CSR_DIAG;
+  if (diag && (lr.r0 + window > INT_MAX || lr.r0 - window < INT_MIN))
+    return fail(TBSLA_HIP_ERR_UNSUPPORTED, "diagonal layout: offsets beyond 32 bits");
   const int s_lds = (int)((nc + kSliceCols - 1) / kSliceCols);  // LDS slices of x
   static std::atomic<unsigned long long> builds{0};
   auto *sp = new SplitCsr();
@@ -806,7 +838,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
       (st = device_alloc(&ws, scan_workspace_bytes(n + 1) + 64)))
     return failed(st);
   long long *d_total = reinterpret_cast<long long *>(static_cast<char *>(ws) + scan_workspace_bytes(n + 1));
-  RowSrc src{m->rowptr, m->format == TBSLA_HIP_ELL ? m->max_col : 0, n};
+  RowSrc src{lr.rp, lr.ell_w, n};
   if (diag) {
     // the stored diagonals: offsets within the window holding at least a
     // quarter of the rows' worth of nonzero entries (at most 64, the
@@ -818,7 +850,8 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
       return failed(st);
     (void)hipMemsetAsync(hist, 0, (size_t)nb * 4, s);
     const long long step = std::max(1LL, n / 262144);  // rows sampled: at most 262144
-    k_diag_hist<<<256, kBlock, 0, s>>>(n, step, m->f_col, window, src, m->colidx, m->values, hist);
+    k_diag_hist<<<256, kBlock, 0, s>>>(n, step, f_col, lr.r0, window, src, m->colidx, m->values,
+                                       hist);
     if (hipMemcpyAsync(h.data(), hist, (size_t)nb * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return failed(fail(TBSLA_HIP_ERR_HIP, "diagonal layout: histogram"));
@@ -836,7 +869,7 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     std::vector<int> offs(std::max<size_t>(cand.size(), 1), 0);
     for (size_t k = 0; k < cand.size(); k++) {
       map[cand[k]] = (int)k;
-      offs[k] = cand[k] - window;
+      offs[k] = (int)(cand[k] - window + lr.r0);  // the kernels read x at r + offs[k]
     }
     if ((st = device_alloc((void **)&sp->doff, offs.size() * 4)) ||
         (st = device_alloc((void **)&sp->dv, dia_bytes)))
@@ -845,8 +878,8 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     (void)hipMemcpyAsync(sp->doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s);
     (void)hipMemsetAsync(sp->dv, 0, dia_bytes, s);
   }
-  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, m->f_col, window, src, m->colidx, m->values, dmap,
-                                            lcnt, rcnt);
+  k_split_count<<<sgrid(n), kBlock, 0, s>>>(n, f_col, lr.r0, window, src, m->colidx, m->values,
+                                            dmap, lcnt, rcnt);
   long long tot[2] = {0, 0};
   int *cnts[2] = {lcnt, rcnt}, *ptrs[2] = {sp->lrp, rptr};
   for (int k = 0; k < 2; k++) {
@@ -880,8 +913,8 @@ int split_build(tbsla_hip_matrix_s *m, int window, int slices, hipStream_t s, in
     return failed(st);
   (void)hipMemsetAsync(scount, 0, (size_t)(std::max(slices, s_lds) + 1) * 4, s);
   const int rb = bits_of(std::max(n, 1LL));
-  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, m->f_col, window, slices, H > 0 ? kSliceCols : 0,
-                                           rb, src, m->colidx, m->values, dmap, sp->ndiag, sp->lrp,
+  k_split_fill<<<sgrid(n), kBlock, 0, s>>>(n, nc, f_col, lr.r0, window, slices,
+                                           H > 0 ? kSliceCols : 0, rb, src, m->colidx, m->values, dmap, sp->ndiag, sp->lrp,
                                            rptr, sp->lci, sp->lv, sp->dv, rc, rv, keys, pay, scount);
   if (H > 0) {
     const int S = s_lds, R = (int)((n + H - 1) / H);
@@ -949,11 +982,20 @@ using namespace tbsla_hip;
 extern "C" int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int window, int slices,
                                         void *stream) {
   if (!m) return fail(TBSLA_HIP_ERR_INVALID, "null handle");
-  if (m->format != TBSLA_HIP_CSR && m->format != TBSLA_HIP_ELL)
-    return fail(TBSLA_HIP_ERR_UNSUPPORTED, "CSR and ELL blocks only");
+  if (m->format == TBSLA_HIP_DENSE)
+    return fail(TBSLA_HIP_ERR_UNSUPPORTED, "CSR, ELL, COO and SCOO blocks only");
   if (layout == TBSLA_HIP_CSR_ROWS) {
     split_release(m);
     return TBSLA_HIP_SUCCESS;
+  }
+  const bool coo = m->format == TBSLA_HIP_COO, scoo = m->format == TBSLA_HIP_SCOO;
+  if (coo || scoo) {
+    // over the triplets' rows through their row pointer (as their SpMV)
+    const long long n = (scoo && m->nnz == 0) ? 0 : m->stored;
+    if (!m->has_arrays || n <= 0) return fail(TBSLA_HIP_ERR_INVALID, "empty block");
+    bool ok = false;
+    if (m->row_sorted) TBSLA_TRY(coo_rowptr(m, n, &ok, static_cast<hipStream_t>(stream)));
+    if (!ok) return fail(TBSLA_HIP_ERR_UNSUPPORTED, "triplets not sorted by row");
   }
   if (layout != TBSLA_HIP_CSR_SPLIT && layout != TBSLA_HIP_CSR_DIAG)
     return fail(TBSLA_HIP_ERR_INVALID, "unknown layout");
@@ -961,9 +1003,10 @@ extern "C" int tbsla_hip_set_csr_layout(tbsla_hip_matrix_t m, int layout, int wi
     return fail(TBSLA_HIP_ERR_INVALID, "window >= 0, slices a multiple of 8 up to 64");
   if (layout == TBSLA_HIP_CSR_DIAG && window > kMaxDiagWindow)
     return fail(TBSLA_HIP_ERR_INVALID, "diagonal layout: window at most 1048576");
-  if (layout == TBSLA_HIP_CSR_DIAG && m->ln_col == 0)  // its kernel reads x for every slot
+  const LayoutRows lr = layout_rows(m);
+  if (layout == TBSLA_HIP_CSR_DIAG && lr.nc == 0)  // its kernel reads x for every slot
     return fail(TBSLA_HIP_ERR_INVALID, "diagonal layout: block without columns");
-  if (!m->has_arrays || m->ln_row == 0 || (m->format == TBSLA_HIP_ELL && m->max_col == 0))
+  if (!m->has_arrays || lr.n == 0 || (m->format == TBSLA_HIP_ELL && m->max_col == 0))
     return fail(TBSLA_HIP_ERR_INVALID, "empty block");
   return split_build(m, window, slices, static_cast<hipStream_t>(stream), layout);
 }

@@ -418,6 +418,8 @@ int spmv_launch(tbsla_hip_matrix_s *m, double *d_r, const double *d_v, int vect_
       const int col_off = coo ? 0 : -m->f_col;
       bool rows_ok = false;
       if (zero_first && m->row_sorted) TBSLA_TRY(coo_rowptr(m, n, &rows_ok, s));
+      if (rows_ok && m->split)  // a derived layout over the triplets' rows
+        return split_launch(m, d_r + ((long long)m->coo_row0 + row_off), d_v, nullptr, s);
       if (rows_ok) {
         // y is zero: the row kernel's stores over the triplets' row range
         // equal the accumulation.  V from the mean row length, L the

@@ -173,6 +173,98 @@ def test_split_bad_arguments():
         g.set_csr_layout(T.CSR_DIAG, (1 << 20) + 1, 8)
     with pytest.raises(T.TbslaHipError):
         g.set_csr_layout(3, 64, 8)
-    e = T.Matrix(T.COO).fill_cqmat(100, 100, 4, 0.1, 0)
+    e = T.Matrix(T.DENSE).fill_cqmat(100, 100, 4, 0.1, 0)
     with pytest.raises(T.TbslaHipError):
         e.set_csr_layout(T.CSR_SPLIT, 64, 8)
+
+
+def coo_x(o, seed):
+    """x as the block's SpMV reads it: COO by the global column (n_col
+    entries), SCOO by the column less f_col (ln_col entries)."""
+    geo = o.geom
+    return O.unit_vector(seed, geo["n_col"] if o.fmt == O.COO else geo["ln_col"]) - 0.25
+
+
+@pytest.mark.parametrize("layout", [T.CSR_SPLIT, T.CSR_DIAG])
+@pytest.mark.parametrize("fmt", [T.COO, T.SCOO])
+@pytest.mark.parametrize("grid", [(0, 0, 1, 1), (1, 1, 2, 2), (1, 0, 2, 2), (2, 1, 3, 3)])
+def test_coo_layouts_match_oracle(fmt, layout, grid):
+    """COO and SCOO blocks in the derived layouts over their triplets' rows
+    (rows from the first to the last row holding a triplet; y is zero
+    elsewhere): SpMV within 1e-12 of the oracle and bitwise repeatable; on a
+    block of the matrix's diagonal the stored diagonals are the matrix's (the
+    rows shifted by the block's first row), so few entries stay remote; Ax
+    (accumulating for COO and SCOO, MatrixCOO.cpp / MatrixSCOO.cpp) keeps
+    its own path; a refill drops the layout."""
+    n = 30000
+    o = O.fill_cqmat(fmt, n, n, 16, 0.1, 0, *grid, value_seed=6)
+    g = T.Matrix(fmt).fill_cqmat(n, n, 16, 0.1, 0, *grid, value_seed=6)
+    x = coo_x(o, 4)
+    before = g.spmv(x)
+    g.set_csr_layout(layout, 65536 if layout == T.CSR_DIAG else 64, 8)
+    lay = g.get_csr_layout()
+    assert lay["layout"] == layout
+    y1, y2 = g.spmv(x), g.spmv(x)
+    ref = o.spmv(x)
+    assert rel_max(y1, ref) < 1e-12
+    assert np.array_equal(y1, y2)
+    if layout == T.CSR_DIAG and grid[0] == grid[1]:
+        assert lay["remote"] < 0.2 * o.geom["nnz"], lay
+    r = O.unit_vector(9, ref.size)
+    assert rel_max(g.Ax(r.copy(), x), o.Ax(r.copy(), x)) < 1e-12
+    g.set_csr_layout(T.CSR_ROWS)
+    assert rel_max(g.spmv(x), before) < 1e-15
+    g.set_csr_layout(layout, 65536 if layout == T.CSR_DIAG else 64, 8)
+    g.fill_cqmat(n, n, 16, 0.1, 0, *grid, value_seed=6)
+    assert g.get_csr_layout()["layout"] == T.CSR_ROWS
+
+
+@pytest.mark.parametrize("fmt", [T.COO, T.SCOO])
+def test_coo_layout_gaps_values_changed_and_unsorted(fmt):
+    """Row-sorted triplets with empty rows before, between and after them
+    (integer data: exact) in the diagonal layout; new values written through
+    the device arrays and announced by values_changed rebuild the layout;
+    triplets out of row order refuse a layout (UNSUPPORTED) and keep their
+    atomic path."""
+    rng = np.random.default_rng(3)
+    n = 4000
+    rows = np.sort(rng.choice(np.arange(500, 3500), 30000)).astype(np.int32)
+    rows = rows[(rows < 1500) | (rows > 1800)]
+    cols = np.clip(rows + rng.integers(-3, 4, rows.size), 0, n - 1).astype(np.int32)
+    cols[::17] = rng.integers(0, n, cols[::17].size)
+    vals = rng.integers(-3, 4, rows.size).astype(np.float64)
+    geom = dict(n_row=n, n_col=n, f_row=0, f_col=0, ln_row=n, ln_col=n, pr=0, pc=0, NR=1, NC=1,
+                nnz=int(rows.size))
+    g = T.Matrix.from_arrays(fmt, geom, rows, cols, vals)
+    x = np.arange(n, dtype=np.float64) - 1000.0
+    g.set_csr_layout(T.CSR_DIAG, 64, 8)
+    assert g.get_csr_layout()["layout"] == T.CSR_DIAG
+    assert np.array_equal(g.spmv(x), O.coo_from_arrays(n, n, rows, cols, vals).spmv(x))
+    # new values through the device arrays
+    _, _, dvals = g.device_arrays()
+    vals2 = vals * 2.0 - 1.0
+    T.check(T.lib().tbsla_hip_memcpy_htod(dvals, vals2.ctypes.data, vals2.nbytes, None))
+    g.values_changed()
+    assert g.get_csr_layout()["layout"] == T.CSR_DIAG
+    assert np.array_equal(g.spmv(x), O.coo_from_arrays(n, n, rows, cols, vals2).spmv(x))
+    perm = rng.permutation(rows.size)
+    u = T.Matrix.from_arrays(fmt, geom, rows[perm], cols[perm], vals[perm])
+    with pytest.raises(T.TbslaHipError):
+        u.set_csr_layout(T.CSR_DIAG, 64, 8)
+    assert u.get_csr_layout()["layout"] == T.CSR_ROWS
+    ref = O.coo_from_arrays(n, n, rows, cols, vals).spmv(x)
+    assert rel_max(u.spmv(x), ref) < 1e-14
+
+
+@pytest.mark.parametrize("fmt", [T.COO, T.SCOO])
+def test_coo_autotune_keeps_results(fmt):
+    """The autotuner on a COO / SCOO block (n = 200000, c = 24, q = 0.1)
+    leaves a layout whose SpMV is within 1e-12 of the oracle."""
+    n = 200000
+    o = O.fill_cqmat(fmt, n, n, 24, 0.1, 0, value_seed=2)
+    g = T.Matrix(fmt).fill_cqmat(n, n, 24, 0.1, 0, value_seed=2)
+    us = g.tune_layout(3)
+    assert us > 0
+    assert g.get_csr_layout()["layout"] in (T.CSR_ROWS, T.CSR_SPLIT, T.CSR_DIAG)
+    x = coo_x(o, 1)
+    assert rel_max(g.spmv(x), o.spmv(x)) < 1e-12

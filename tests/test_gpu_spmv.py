@@ -277,7 +277,7 @@ def test_spmv_config2_sampled_rows(fmt):
         close(y[geo["f_row"]:geo["f_row"] + geo["ln_row"]], blk.spmv(x))
 
 
-@pytest.mark.parametrize("fmt", [T.CSR, T.ELL])
+@pytest.mark.parametrize("fmt", [T.CSR, T.ELL, T.COO, T.SCOO])
 def test_spmv_config2_diag_layout_sampled_rows(fmt):
     """The same matrix in the diagonal layout (32 stored diagonals, the
     13.4 M rewired entries multiplied through LDS column slices): the rows of
