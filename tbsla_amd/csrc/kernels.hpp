@@ -50,6 +50,19 @@ int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *lay
 // rows per block of the diagonal layout's LDS-slice remote form, or 0 (no
 // layout, no remote entries, or the L2-slice form, whose slice count matters)
 int split_lds_block_rows(const tbsla_hip_matrix_s *m);
+// PageRank's SpMV and update in one pass of the diagonal layout's LDS-slice
+// form (CSR / ELL blocks with that form only; split_pr_blocks gives its
+// workgroup count, the number of partials, or 0): b = (beta A b_t + S (1 -
+// beta) / n_col) / S over the block's rows with b_t = x, as k_pr_update.
+struct PrFuse {
+  double *b;
+  const double *S;
+  double beta, n_col;
+  double *p_err, *p_sum;
+};
+int split_pr_blocks(const tbsla_hip_matrix_s *m);
+int split_launch_pr(tbsla_hip_matrix_s *m, const double *bt, const PrFuse &pf, const int *done,
+                    hipStream_t s);
 // The row pointer of a COO / SCOO block's n row-sorted triplets over their
 // row range (built once and kept with the block); *ok false when the
 // triplets are not sorted by row

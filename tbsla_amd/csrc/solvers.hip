@@ -581,12 +581,18 @@ extern "C" int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double eps
       length = std::min(be, re) - (rs + r_start);
     }
   }
+  // a square block on the diagonal in the diagonal layout's LDS-slice form:
+  // SpMV and update in one pass (split_launch_pr), one partial per row block
+  const int fused = (r_start == 0 && s_start == 0 && length == ln_col && ln_row == ln_col)
+                        ? split_pr_blocks(m)
+                        : 0;
+  const long long np = std::max<long long>(kRedBlocks, fused);  // partials of each kind
   DevBuf<double> bufA, bufB, y, part;
   DevBuf<SolverState> st;
   TBSLA_TRY(bufA.alloc(ln_col));
   TBSLA_TRY(bufB.alloc(ln_col));
-  TBSLA_TRY(y.alloc(ylen));
-  TBSLA_TRY(part.alloc(2 * kRedBlocks + 8));
+  if (!fused) TBSLA_TRY(y.alloc(ylen));
+  TBSLA_TRY(part.alloc(2 * np + 8));
   TBSLA_TRY(st.alloc(1));
   TBSLA_TRY(launch_fill(bufA.p, ln_col, 1.0, s));  // b = 1 (Matrix.cpp:549-552)
   SolverState h{};
@@ -599,11 +605,16 @@ extern "C" int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double eps
     const int batch = std::min(16, max_iterations - launched);
     for (int i = 0; i < batch; i++, launched++) {
       double *bt = buf[launched % 2], *bn = buf[(launched + 1) % 2];
-      TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, s, &st.p->done));
-      k_pr_update<<<nb, kBlock, 0, s>>>(ln_col, y.p, bt, bn, r_start, s_start, length, beta,
-                                        (double)m->n_col, st.p, part.p, part.p + kRedBlocks);
-      k_pr_finalize<<<1, kBlock, 0, s>>>(st.p, part.p, part.p + kRedBlocks, (int)nb, m->NC,
-                                         epsilon);
+      if (fused) {
+        const PrFuse pf{bn, &st.p->S, beta, (double)m->n_col, part.p, part.p + np};
+        TBSLA_TRY(split_launch_pr(m, bt, pf, &st.p->done, s));
+      } else {
+        TBSLA_TRY(spmv_launch(m, y.p, bt, 0, true, s, &st.p->done));
+        k_pr_update<<<nb, kBlock, 0, s>>>(ln_col, y.p, bt, bn, r_start, s_start, length, beta,
+                                          (double)m->n_col, st.p, part.p, part.p + np);
+      }
+      k_pr_finalize<<<1, kBlock, 0, s>>>(st.p, part.p, part.p + np, fused ? fused : (int)nb,
+                                         m->NC, epsilon);
     }
     TBSLA_HIP_LAUNCH_CHECK();
     TBSLA_HIP_CHECK(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -613,8 +624,8 @@ extern "C" int tbsla_hip_page_rank(tbsla_hip_matrix_t m, double beta, double eps
   double *b = buf[h.iters % 2];
   // pagerank_norma_end (Matrix.cpp:149-171): b /= sum(b)
   k_partial_sum<<<nb, kBlock, 0, s>>>(b, ln_col, part.p);
-  k_total<<<1, kBlock, 0, s>>>(part.p, (int)nb, part.p + 2 * kRedBlocks);
-  k_divide<<<stride_grid(ln_col), kBlock, 0, s>>>(b, ln_col, part.p + 2 * kRedBlocks);
+  k_total<<<1, kBlock, 0, s>>>(part.p, (int)nb, part.p + 2 * np);
+  k_divide<<<stride_grid(ln_col), kBlock, 0, s>>>(b, ln_col, part.p + 2 * np);
   TBSLA_HIP_LAUNCH_CHECK();
   TBSLA_HIP_CHECK(hipMemcpyAsync(d_b, b, (size_t)ln_col * 8, hipMemcpyDeviceToDevice, s));
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));

@@ -588,19 +588,39 @@ __global__ __launch_bounds__(kBigBlock) void k_slice_products(
   }
 }
 
+// PageRank's update of one row (Matrix.cpp:123-141, as k_pr_update), each
+// operation rounded on its own
+__device__ __forceinline__ double pr_value(const PrFuse &pf, double y, double S, double t_sum) {
+  TBSLA_NO_FMA
+  double v = pf.beta * y + t_sum;
+  return v / S;
+}
+
 // y[r] = the diagonal sum of row r (as k_dia_rows) + the sum of its remote
 // products, for the h rows of block blockIdx.  The block's products
 // P[pb[q] .. pb[q + 1]) are first scattered into LDS in row-major order
 // (d16); each row then sums its own run in CSR order.  Sixteen wavefronts
 // take 2048 rows per pass, two tiles of kWave rows each, a lane two
-// adjacent rows.
+// adjacent rows.  With PR, PageRank's update of the row replaces the store
+// of y: b[r] = (beta y[r] + S (1 - beta) / n_col) / S with b_t = x and S its
+// sum, and the workgroup's partials of |b - b_t| and of sum(b) go to
+// p_err / p_sum[blockIdx.x] (lanes' partials summed by the wavefront's fixed
+// tree, then the wavefronts in order).
+template <bool PR>
 __global__ __launch_bounds__(kBigBlock) void k_dia_rows_remote(
     long long n, long long ncol, int ndiag, int h, const int *__restrict__ doff,
     const double *__restrict__ dv, const double *__restrict__ x, const int *__restrict__ pb,
     const unsigned short *__restrict__ d16, const unsigned short *__restrict__ rs16,
-    const double *__restrict__ P, double *__restrict__ y, const int *__restrict__ done) {
+    const double *__restrict__ P, double *__restrict__ y, const int *__restrict__ done,
+    PrFuse pf) {
   __shared__ double buf[kBlockCap];
   if (done && *done) return;
+  double S = 0.0, t_sum = 0.0, pe = 0.0, ps = 0.0;
+  if constexpr (PR) {
+    TBSLA_NO_FMA
+    S = *pf.S;
+    t_sum = S * ((1 - pf.beta) / pf.n_col);
+  }
   const long long q = blockIdx.x;
   const long long r0 = q * h;
   {
@@ -660,8 +680,42 @@ __global__ __launch_bounds__(kBigBlock) void k_dia_rows_remote(
       double t0 = 0.0, t1 = 0.0;
       for (int i = a; i < b; i++) t0 += buf[i];
       for (int i = b; i < c; i++) t1 += buf[i];
-      if (r < n) y[r] = s0 + t0;
-      if (r + 1 < n) y[r + 1] = s1 + t1;
+      if constexpr (PR) {
+        if (r < n) {
+          const double v = pr_value(pf, s0 + t0, S, t_sum);
+          pf.b[r] = v;
+          pe += fabs(v - x[r]);
+          ps += v;
+        }
+        if (r + 1 < n) {
+          const double v = pr_value(pf, s1 + t1, S, t_sum);
+          pf.b[r + 1] = v;
+          pe += fabs(v - x[r + 1]);
+          ps += v;
+        }
+      } else {
+        if (r < n) y[r] = s0 + t0;
+        if (r + 1 < n) y[r + 1] = s1 + t1;
+      }
+    }
+  }
+  if constexpr (PR) {
+    pe = group_sum<kWave>(pe);
+    ps = group_sum<kWave>(ps);
+    __syncthreads();  // every row is done with buf
+    if (lane == 0) {
+      buf[wave] = pe;
+      buf[kBigBlock / kWave + wave] = ps;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double te = 0.0, ts = 0.0;
+      for (int w = 0; w < kBigBlock / kWave; w++) {
+        te += buf[w];
+        ts += buf[kBigBlock / kWave + w];
+      }
+      pf.p_err[blockIdx.x] = te;
+      pf.p_sum[blockIdx.x] = ts;
     }
   }
 }
@@ -678,6 +732,28 @@ int bits_of(long long v) {
 }
 
 }  // namespace
+
+int split_pr_blocks(const tbsla_hip_matrix_s *m) {
+  const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  const bool own_rows = m->format == TBSLA_HIP_CSR || m->format == TBSLA_HIP_ELL;
+  return sp && own_rows && sp->lds_h > 0 && m->ln_row > 0 ? sp->lds_r : 0;
+}
+
+int split_launch_pr(tbsla_hip_matrix_s *m, const double *bt, const PrFuse &pf, const int *done,
+                    hipStream_t s) {
+  if (split_pr_blocks(m) == 0) return fail(TBSLA_HIP_ERR_INVALID, "no LDS-slice diagonal layout");
+  const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
+  const LayoutRows lr = layout_rows(m);
+  const long long n = lr.n, ncol = std::max(lr.nc, 1LL);
+  k_slice_products<<<(unsigned)(sp->lds_s * sp->lds_g), kBigBlock, 0, s>>>(
+      lr.nc, sp->lds_s, sp->lds_r, sp->lds_g, sp->sst, sp->pst, sp->c16, sp->q16, sp->sv, bt,
+      sp->P, done);
+  k_dia_rows_remote<true><<<(unsigned)sp->lds_r, kBigBlock, 0, s>>>(
+      n, ncol, sp->ndiag, sp->lds_h, sp->doff, sp->dv, bt, sp->pb, sp->d16, sp->rs16, sp->P,
+      nullptr, done, pf);
+  TBSLA_HIP_LAUNCH_CHECK();
+  return TBSLA_HIP_SUCCESS;
+}
 
 int split_params(const tbsla_hip_matrix_s *m, int *window, int *slices, int *layout) {
   const SplitCsr *sp = static_cast<const SplitCsr *>(m->split);
@@ -722,9 +798,9 @@ int split_launch(tbsla_hip_matrix_s *m, double *y, const double *x, const int *d
     k_slice_products<<<(unsigned)(sp->lds_s * sp->lds_g), kBigBlock, 0, s>>>(
         lr.nc, sp->lds_s, sp->lds_r, sp->lds_g, sp->sst, sp->pst, sp->c16, sp->q16, sp->sv, x,
         sp->P, done);
-    k_dia_rows_remote<<<(unsigned)sp->lds_r, kBigBlock, 0, s>>>(
+    k_dia_rows_remote<false><<<(unsigned)sp->lds_r, kBigBlock, 0, s>>>(
         n, ncol, sp->ndiag, sp->lds_h, sp->doff, sp->dv, x, sp->pb,
-        sp->d16, sp->rs16, sp->P, y, done);
+        sp->d16, sp->rs16, sp->P, y, done, PrFuse{});
     TBSLA_HIP_LAUNCH_CHECK();
     return TBSLA_HIP_SUCCESS;
   }

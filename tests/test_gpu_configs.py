@@ -90,6 +90,19 @@ def test_cfg2_page_rank(stochastic_cfg2):
     A = csr_scipy(g)
     v = 0.85 * (A @ b) + 0.15 / N2
     assert np.abs(v / v.sum() - b).sum() < 1e-9
+    # the diagonal layout's LDS-slice form: each iteration one fused pass
+    say("cfg2 page_rank: diagonal layout, fused update")
+    g.set_csr_layout(T.CSR_DIAG, 65536, 16)
+    try:
+        assert g.get_csr_layout()["lds_block_rows"] > 0
+        bo, _, _ = o.page_rank(0.85, 0.0, 3)
+        bg, itg, _ = g.page_rank(0.85, 0.0, 3)
+        assert itg == 3 and rel_max(bg, bo) < 1e-12
+        bd, itd, errd = g.page_rank(0.85, 1e-10, 10000)
+        assert errd < 1e-10 and itd == 65
+        assert rel_max(bd, b) < 1e-12
+    finally:
+        g.set_csr_layout(T.CSR_ROWS)
 
 
 @pytest.mark.parametrize("pc", [0, 1])

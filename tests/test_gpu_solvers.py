@@ -133,6 +133,35 @@ def test_page_rank_max_norm_known_answer_and_geometry():
         blk.page_rank_max_norm()
 
 
+@pytest.mark.parametrize("fmt", [T.CSR, T.ELL])
+def test_page_rank_fused_in_diagonal_layout(fmt):
+    """A block in the diagonal layout's LDS-slice form runs each PageRank
+    iteration as one pass (the update inside the diagonal kernel, one
+    partial per row block): same iteration count as the oracle, iterate
+    within a relative 1e-12, within 1e-13 of the row layout's, and the same
+    bits on a second solve."""
+    n, c, q = 100000, 16, 0.1
+    o = O.fill_cqmat(O.CSR, n, n, c, q, 0, value_seed=0)
+    o.make_stochastic()
+    bo, ito, _ = o.page_rank(0.85, 1e-10, 10000)
+    g = T.Matrix(fmt).fill_cqmat(n, n, c, q, 0, value_seed=0)
+    g.make_stochastic()
+    b_rows, it_rows, _ = g.page_rank(0.85, 1e-10, 10000)
+    g.set_csr_layout(T.CSR_DIAG, 65536, 16)
+    assert g.get_csr_layout()["lds_block_rows"] > 0
+    b1, it1, e1 = g.page_rank(0.85, 1e-10, 10000)
+    b2, it2, e2 = g.page_rank(0.85, 1e-10, 10000)
+    assert it1 == ito == it_rows, (it1, ito, it_rows)
+    assert rel_max(b1, bo) < 1e-12
+    assert rel_max(b1, b_rows) < 1e-13
+    assert np.array_equal(b1, b2) and e1 == e2 and it2 == it1
+    assert abs(b1.sum() - 1.0) < 1e-12
+    # a fixed number of iterations (epsilon 0)
+    bk, itk, _ = g.page_rank(0.85, 0.0, 7)
+    bok, itok, _ = o.page_rank(0.85, 0.0, 7)
+    assert itk == itok == 7 and rel_max(bk, bok) < 1e-12
+
+
 def test_page_rank_max_iterations_and_determinism():
     n = 30000
     g = T.Matrix(T.CSR).fill_cqmat(n, n, 8, 0.3, 2, value_seed=4)
