@@ -445,13 +445,17 @@ def test_two_pass_k1_same_bits(sym, precision, pre):
         assert rel_max(x, xo) < 1e-12
 
 
-def test_two_pass_k1_batched_same_bits():
+@pytest.mark.parametrize("precision", [T.CG_FP64, T.CG_MIXED])
+def test_two_pass_k1_batched_same_bits(precision):
+    """A batch with K1 in one and in two passes: the same bits, in FP64 and
+    in mixed precision (with residual replacement)."""
     sizes = [(4096, 8), (20000, 16), (9000, 20)]
     parts, bd, offs = make_batch(sizes, seed=3)
     rhs = 2 * O.unit_vector(8, int(offs[-1])) - 1
     xs = []
     for k in (1, 2):
-        opt = bd.cg_options(tolerance=1e-10, relative=True, lanes_per_row=8, spmv_passes=k)
+        opt = bd.cg_options(tolerance=1e-10, relative=True, lanes_per_row=8, spmv_passes=k,
+                            precision=precision, replace_every=50 if precision else 0)
         x, res = bd.cg_batched(offs, rhs, opt)
         xs.append((x, [(r.iterations, r.res_norm, r.status) for r in res]))
     assert xs[0][1] == xs[1][1]
