@@ -595,6 +595,364 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
   }
 }
 
+// ---- four rows per wavefront ----------------------------------------------
+//
+// The same step with a wavefront taking four rows at a time, one per group
+// of kQuadLanes = 16 lanes (a DPP row), lane t of a group holding columns
+// 4t .. 4t + 3.  A row's first 32 entries sit two per lane (entries t and
+// t + 16); entry K reaches its group's lanes by DPP row_newbcast (one
+// v_mov_dpp per 32-bit half, no readlane and no scalar register), so a band
+// entry costs two 16-byte LDS reads, three broadcasts, one address and four
+// fused multiply-adds for the four rows' 64 columns at once.  Remote
+// entries are found per group from the wavefront's ballots and fetched by
+// lane shuffles.  The pipeline, ring, window and sums are those of
+// k_ppr_pipe; entries past a row's first 32 go chunk by chunk, unpipelined.
+constexpr int kQuadLanes = 16;
+constexpr int kQuadRows = kWave / kQuadLanes;  // rows per wavefront step
+constexpr int kQuadEnt = 2 * kQuadLanes;       // entries per row in the pipeline
+
+// lane K of each group of 16, in every lane of the group
+template <int K>
+__device__ __forceinline__ int qbc(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+}
+template <int K>
+__device__ __forceinline__ double qbc_d(double v) {
+  return __hiloint2double(qbc<K>(__double2hiint(v)), qbc<K>(__double2loint(v)));
+}
+
+// a quad slot: four rows (row q of each group), this lane's part
+template <int S>
+struct QuadSlot {
+  int beg, end;            // the group's row extent
+  int col[2];              // entries t, t + 16 of the row: column, -1 past the row
+  double val[2];
+  pprpiece_t old[2];       // X[r, 4t .. 4t + 3] (when not read from the ring)
+  pprpiece_t xr[S][2];     // the group's first S remote entries: X rows, this lane's columns
+  double vr[S];            // and values (0 past the count)
+};
+
+// row of group g at quad step qs: the row q = 4 qs + g of the wavefront's sequence
+__device__ __forceinline__ int quad_row(const PprArgs &a, const PprWalk &wk, int qs, int g) {
+  return ppr_row_at(a, wk, kQuadRows * qs + g);
+}
+
+// stage P
+template <int S>
+__device__ __forceinline__ void quad_sp(const PprArgs &a, int r, QuadSlot<S> &s) {
+  const int rr = max(r, 0);
+  const int b = a.rowptr[rr], e = a.rowptr[rr + 1];
+  s.beg = b;
+  s.end = r < 0 ? b : e;
+}
+
+// stage E: the first 32 entries (two per lane) and, unless OLD (every row's
+// old values in the ring), the old values
+template <int S, bool OLD>
+__device__ __forceinline__ void quad_se(const PprArgs &a, int r, int t, QuadSlot<S> &s) {
+  const int len = s.end - s.beg;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int i = t + h * kQuadLanes;
+    const int j = min(s.beg + min(i, max(len - 1, 0)), a.last);
+    const int c = a.colidx[j];
+    const double v = a.vals[j];
+    s.col[h] = i < len ? c - a.f_col : -1;
+    s.val[h] = i < len ? v : 0.0;
+  }
+  if constexpr (!OLD) {
+    const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
+    const size_t o = (size_t)max(r, 0) * kPprPieces + 2 * t;
+    s.old[0] = X2[o];
+    s.old[1] = X2[o + 1];
+  }
+}
+
+// the group's remote entries among the lane's two (bit t: entry t, bit
+// 16 + t: entry 16 + t), from the wavefront's ballots
+__device__ __forceinline__ unsigned quad_remote_mask(const PprArgs &a, int c0, int c1, int wlo,
+                                                     int whi, int g) {
+  const unsigned long long m0 = __ballot(c0 >= 0 && c0 < a.n && (c0 < wlo || c0 >= whi));
+  const unsigned long long m1 = __ballot(c1 >= 0 && c1 < a.n && (c1 < wlo || c1 >= whi));
+  return (unsigned)((m0 >> (kQuadLanes * g)) & 0xFFFFu) |
+         ((unsigned)((m1 >> (kQuadLanes * g)) & 0xFFFFu) << 16);
+}
+
+// entry idx (0 .. 31) of the group's row: column and value, by shuffles
+__device__ __forceinline__ void quad_pick(const int (&col)[2], const double (&val)[2], int g,
+                                          int idx, int &c, double &v) {
+  const int src = kQuadLanes * g + (idx & (kQuadLanes - 1));
+  const int c0 = __shfl(col[0], src), c1 = __shfl(col[1], src);
+  const double v0 = __shfl(val[0], src), v1 = __shfl(val[1], src);
+  c = idx < kQuadLanes ? c0 : c1;
+  v = idx < kQuadLanes ? v0 : v1;
+}
+
+// stage R: the X rows of the group's first S remote entries
+template <int S>
+__device__ __forceinline__ void quad_sr(const PprArgs &a, int r, int t, int g, QuadSlot<S> &s) {
+  const PprWindow w = ppr_window(a, max(r, 0));
+  unsigned m = quad_remote_mask(a, s.col[0], s.col[1], w.wlo, w.whi, g);
+  const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    const bool has = m != 0;
+    const int idx = has ? __builtin_ctz(m) : 0;
+    m &= m - 1;
+    int c;
+    double v;
+    quad_pick(s.col, s.val, g, idx, c, v);
+    const size_t o = (size_t)(has ? c : 0) * kPprPieces + 2 * t;  // unused: X row 0
+    s.xr[k][0] = X2[o];
+    s.xr[k][1] = X2[o + 1];
+    s.vr[k] = has ? v : 0.0;
+  }
+}
+
+// the band walk over one lane's entries (h = 0: entries 0 .. 15, h = 1:
+// 16 .. 31 of each group), K = 0 .. 15 in storage order
+template <int K>
+__device__ __forceinline__ void quad_walk(int off, double v, const char *win_b, int t,
+                                          double (&acc)[4]) {
+  const int o = qbc<K>(off) + 32 * t;
+  const double vk = qbc_d<K>(v);
+  const pprpiece_t x0 = *reinterpret_cast<const pprpiece_t *>(win_b + o);
+  const pprpiece_t x1 = *reinterpret_cast<const pprpiece_t *>(win_b + o + 16);
+  acc[0] = ppr_madd(acc[0], vk, x0.x);
+  acc[1] = ppr_madd(acc[1], vk, x0.y);
+  acc[2] = ppr_madd(acc[2], vk, x1.x);
+  acc[3] = ppr_madd(acc[3], vk, x1.y);
+  if constexpr (K + 1 < kQuadLanes) quad_walk<K + 1>(off, v, win_b, t, acc);
+}
+
+// byte offset in the ring of entry column c's X row (the zero row outside
+// the window), and the value the walk multiplies
+__device__ __forceinline__ void quad_band(int c, double val, int wlo, int whi, int &off,
+                                          double &v) {
+  const bool band = c >= wlo && c < whi;
+  off = (band ? ring_slot(c) : kPprWin) * kWave * 8;
+  v = band ? val : 0.0;
+}
+
+// remote products of entries whose group bits are in m, one at a time
+__device__ __forceinline__ void quad_remote_rest(const PprArgs &a, unsigned m, const int (&col)[2],
+                                                 const double (&val)[2], int t, int g,
+                                                 double (&acc)[4]) {
+  const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
+  while (__ballot(m != 0)) {
+    const bool has = m != 0;
+    const int idx = has ? __builtin_ctz(m) : 0;
+    m &= m - 1;
+    int c;
+    double v;
+    quad_pick(col, val, g, idx, c, v);
+    const size_t o = (size_t)(has ? c : 0) * kPprPieces + 2 * t;
+    const pprpiece_t x0 = X2[o], x1 = X2[o + 1];
+    v = has ? v : 0.0;
+    acc[0] = ppr_madd(acc[0], v, x0.x);
+    acc[1] = ppr_madd(acc[1], v, x0.y);
+    acc[2] = ppr_madd(acc[2], v, x1.x);
+    acc[3] = ppr_madd(acc[3], v, x1.y);
+  }
+}
+
+// stage C: the four rows' sums -- band entries of the first 32 in storage
+// order, the first S remote ones, the other remote ones, then entries past
+// the first 32 sixteen at a time (band, then remote) -- and their update
+template <int S, bool OLD>
+__device__ __forceinline__ void quad_sc(const PprArgs &a, int r, int t, int g, const QuadSlot<S> &s,
+                                        const double *__restrict__ win, double (&e)[4],
+                                        double (&sm)[4]) {
+  TBSLA_NO_FMA
+  const PprWindow w = ppr_window(a, max(r, 0));
+  const char *win_b = reinterpret_cast<const char *>(win);
+  const int len = s.end - s.beg;
+  // the longest of the four rows (every lane)
+  int ml = len;
+  ml = max(ml, __shfl_xor(ml, kQuadLanes));
+  ml = max(ml, __shfl_xor(ml, 2 * kQuadLanes));
+  ml = __builtin_amdgcn_readfirstlane(ml);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    int off;
+    double v;
+    quad_band(s.col[0], s.val[0], w.wlo, w.whi, off, v);
+    quad_walk<0>(off, v, win_b, t, acc);
+    if (ml > kQuadLanes) {
+      quad_band(s.col[1], s.val[1], w.wlo, w.whi, off, v);
+      quad_walk<0>(off, v, win_b, t, acc);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    acc[0] = ppr_madd(acc[0], s.vr[k], s.xr[k][0].x);
+    acc[1] = ppr_madd(acc[1], s.vr[k], s.xr[k][0].y);
+    acc[2] = ppr_madd(acc[2], s.vr[k], s.xr[k][1].x);
+    acc[3] = ppr_madd(acc[3], s.vr[k], s.xr[k][1].y);
+  }
+  {
+    unsigned m = quad_remote_mask(a, s.col[0], s.col[1], w.wlo, w.whi, g);
+#pragma unroll
+    for (int k = 0; k < S; k++) m &= m - 1;  // the ones stage R fetched
+    quad_remote_rest(a, m, s.col, s.val, t, g, acc);
+  }
+  // entries past the first 32, sixteen per chunk
+  for (int c0 = kQuadEnt; c0 < ml; c0 += kQuadLanes) {
+    const int i = c0 + t;
+    const int j = min(s.beg + min(i, max(len - 1, 0)), a.last);
+    const int cj = a.colidx[j];
+    const double vj = a.vals[j];
+    const int col[2] = {i < len ? cj - a.f_col : -1, -1};
+    const double val[2] = {i < len ? vj : 0.0, 0.0};
+    int off;
+    double v;
+    quad_band(col[0], val[0], w.wlo, w.whi, off, v);
+    quad_walk<0>(off, v, win_b, t, acc);
+    const unsigned m = quad_remote_mask(a, col[0], col[1], w.wlo, w.whi, g);
+    quad_remote_rest(a, m, col, val, t, g, acc);
+  }
+  if (r < 0) return;
+  double old[4];
+  if constexpr (OLD) {
+    const pprpiece_t *o2 = reinterpret_cast<const pprpiece_t *>(win + ring_slot(r) * kWave);
+    const pprpiece_t p0 = o2[2 * t], p1 = o2[2 * t + 1];
+    old[0] = p0.x;
+    old[1] = p0.y;
+    old[2] = p1.x;
+    old[3] = p1.y;
+  } else {
+    old[0] = s.old[0].x;
+    old[1] = s.old[0].y;
+    old[2] = s.old[1].x;
+    old[3] = s.old[1].y;
+  }
+  // the columns' sums and sources (read here, not held: registers)
+  const pprpiece_t *S2 = reinterpret_cast<const pprpiece_t *>(a.S);
+  const pprpiece_t sa = S2[2 * t], sb = S2[2 * t + 1];
+  const int4 sr = reinterpret_cast<const int4 *>(a.src)[t];
+  const double S4[4] = {sa.x, sa.y, sb.x, sb.y};
+  const int src4[4] = {sr.x, sr.y, sr.z, sr.w};
+  double tv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double u = a.beta * acc[i] + ((r + a.f_col == src4[i]) ? S4[i] * (1 - a.beta) : 0.0);
+    u = u / S4[i];
+    tv[i] = u;
+    e[i] += fabs(u - old[i]);
+    sm[i] += u;
+  }
+  pprpiece_t *Xn2 = reinterpret_cast<pprpiece_t *>(a.Xn);
+  const size_t o = (size_t)r * kPprPieces + 2 * t;
+  Xn2[o] = pprpiece_t{tv[0], tv[1]};
+  Xn2[o + 1] = pprpiece_t{tv[2], tv[3]};
+}
+
+// One group of NS quad steps (NS quad steps = one tile when NS = 4): for
+// step qs0 + K, stage C, then R of step + DR, E of step + DE, P of step + D0.
+template <int NS, int DE, int DR, int S, bool OLD, int K>
+__device__ __forceinline__ void quad_group_step(const PprArgs &a, const PprWalk &wk, int qs0,
+                                                QuadSlot<S> (&st)[NS],
+                                                const double *__restrict__ win, int t, int g,
+                                                double (&e)[4], double (&sm)[4]) {
+  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
+  const int qs = qs0 + K;
+  // C first: the remote rows R fetches next are then live only until the
+  // next step's C, whose band walk covers their latency
+  quad_sc<S, OLD>(a, quad_row(a, wk, qs, g), t, g, st[K], win, e, sm);
+  quad_sr(a, quad_row(a, wk, qs + DR, g), t, g, st[(K + DR) % NS]);
+  quad_se<S, OLD>(a, quad_row(a, wk, qs + DE, g), t, st[(K + DE) % NS]);
+  quad_sp(a, quad_row(a, wk, qs + D0, g), st[(K + D0) % NS]);
+  if constexpr (K + 1 < NS)
+    quad_group_step<NS, DE, DR, S, OLD, K + 1>(a, wk, qs0, st, win, t, g, e, sm);
+}
+
+template <int NS, int DE, int DR, int S, bool OLD, int K>
+__device__ __forceinline__ void quad_prologue(const PprArgs &a, const PprWalk &wk,
+                                              QuadSlot<S> (&st)[NS], int t, int g) {
+  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
+  if constexpr (K < D0) {
+    quad_sp(a, quad_row(a, wk, K, g), st[K]);
+  } else if constexpr (K < D0 + DE) {
+    quad_se<S, OLD>(a, quad_row(a, wk, K - D0, g), t, st[K - D0]);
+  } else if constexpr (K < D0 + DE + DR) {
+    quad_sr(a, quad_row(a, wk, K - D0 - DE, g), t, g, st[K - D0 - DE]);
+  }
+  if constexpr (K + 1 < D0 + DE + DR) quad_prologue<NS, DE, DR, S, OLD, K + 1>(a, wk, st, t, g);
+}
+
+constexpr int kQuadStepsPerTile = kPprRowsPerWave / kQuadRows;
+
+template <int NS, int DE, int DR, int S, bool OLD>
+__global__ __launch_bounds__(kPprBlock, 3) void k_ppr_quad(PprArgs a) {
+  TBSLA_NO_FMA
+  static_assert(kQuadStepsPerTile % NS == 0, "a tile starts a group");
+  static_assert(kPprWin > kPprTile && kPprWin <= 2 * kPprTile, "the first window is two loads");
+  if (a.st->done) return;
+  __shared__ __attribute__((aligned(16))) double win[(kPprWin + 1) * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & (kQuadLanes - 1), g = lane / kQuadLanes;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  if (threadIdx.x < kWave) win[kPprWin * kWave + threadIdx.x] = 0.0;  // before the first barrier
+  double e[4] = {0.0, 0.0, 0.0, 0.0}, sm[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = (a.ntiles + nb - 1) / nb;
+  PprWalk wk;
+  wk.wave = wave;
+  wk.t0 = min(a.ntiles, b * per);
+  wk.t1 = min(a.ntiles, wk.t0 + per);
+  wk.tstep = 1;
+  const int nqs = (wk.t1 - wk.t0) * kQuadStepsPerTile;
+  const int base0 = wk.t0 * kPprTile + a.dlo;
+  const bool any = wk.t0 < wk.t1;
+  pprpiece_t pre[kPprRingLoads];
+  ring_load(a, any, base0, base0 + kPprWin, pre);
+  ring_store(a, any, base0, base0 + kPprWin, pre, win);
+  ring_load(a, any, base0 + kPprTile, base0 + kPprWin, pre);
+  ring_store(a, any, base0 + kPprTile, base0 + kPprWin, pre, win);
+  ring_load(a, wk.t0 + 1 < wk.t1, base0 + kPprWin, base0 + kPprWin + kPprTile, pre);
+  __syncthreads();
+  QuadSlot<S> st[NS];
+  quad_prologue<NS, DE, DR, S, OLD, 0>(a, wk, st, t, g);
+  for (int qs0 = 0; qs0 < nqs; qs0 += NS) {
+    if (qs0 > 0 && qs0 % kQuadStepsPerTile == 0) {
+      const int tt = wk.t0 + qs0 / kQuadStepsPerTile;
+      const int jn = tt * kPprTile + a.dlo + (kPprWin - kPprTile);  // tile tt's new rows
+      __syncthreads();  // tile tt - 1's rows are done with the departing rows
+      ring_store(a, true, jn, jn + kPprTile, pre, win);
+      __syncthreads();
+      ring_load(a, tt + 1 < wk.t1, jn + kPprTile, jn + 2 * kPprTile, pre);
+    }
+    quad_group_step<NS, DE, DR, S, OLD, 0>(a, wk, qs0, st, win, t, g, e, sm);
+  }
+  // per-column partials: the four groups (fixed xor tree), then the
+  // wavefronts in order
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    e[i] += __shfl_xor(e[i], kQuadLanes);
+    e[i] += __shfl_xor(e[i], 2 * kQuadLanes);
+    sm[i] += __shfl_xor(sm[i], kQuadLanes);
+    sm[i] += __shfl_xor(sm[i], 2 * kQuadLanes);
+  }
+  __syncthreads();
+  if (g == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      win[wave * kWave + 4 * t + i] = e[i];
+      win[(kPprWaves + wave) * kWave + 4 * t + i] = sm[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    double te = 0.0, ts = 0.0;
+    for (int v = 0; v < kPprWaves; v++) {
+      te += win[v * kWave + lane];
+      ts += win[(kPprWaves + v) * kWave + lane];
+    }
+    a.p_err[(size_t)b * kWave + lane] = te;
+    a.p_sum[(size_t)b * kWave + lane] = ts;
+  }
+}
+
 // Histogram of the entries' offsets col - row in [-64, 63] over every 16th
 // row (the window's placement): per-workgroup bins in LDS, then added to the
 // global bins.
@@ -848,7 +1206,9 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     for (int i = 0; i < batch; i++, launched++) {
       pa.X = Xb[launched % 2];
       pa.Xn = Xb[(launched + 1) % 2];
-      k_ppr_pipe<4, 2, 1, 4><<<nb, kPprBlock, 0, s>>>(pa);
+      // four rows per wavefront when every row's old values are in the ring
+      if (pa.old_lds) k_ppr_quad<4, 2, 1, 3, true><<<nb, kPprBlock, 0, s>>>(pa);
+      else k_ppr_pipe<4, 2, 1, 4><<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     note(hipGetLastError());
