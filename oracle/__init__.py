@@ -1,7 +1,7 @@
 """CPU oracle for TBSLA's sparse hot path -- test infrastructure only.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package; it is the checker, never the product.
+``bench.py`` may import this package; it is the oracle, never the product.
 
 ``liboracle.so`` (``tbsla_oracle.c``) is a plain-C restatement of the
 reference routines; every C function cites the reference file:line it

@@ -1,4 +1,4 @@
-// ref_bench.cpp -- times the reference's own CPU SpMV (checker / CPU baseline only).
+// ref_bench.cpp -- times the reference's own CPU SpMV (oracle and CPU baseline only).
 //
 // TEST INFRASTRUCTURE.  Linked by oracle/Makefile against the reference's
 // src/cpp sources compiled where they lie; bench.py runs the binary for its

@@ -1,4 +1,4 @@
-// ref_shim.cpp -- C entry points over the compiled reference library (checker only).
+// ref_shim.cpp -- C entry points over the compiled reference library (oracle only).
 //
 // TEST INFRASTRUCTURE.  Built by oracle/Makefile together with the reference's
 // own src/cpp sources (compiled where they lie, outputs in oracle/_ref/), and
@@ -68,7 +68,7 @@ Matrix *make(int fmt) {
   return nullptr;
 }
 
-// The reference prints progress on std::cout; keep the checker's stdout clean.
+// The reference prints progress on std::cout; keep the oracle's stdout clean.
 struct Quiet {
   std::streambuf *old;
   std::ostringstream sink;

@@ -1,7 +1,7 @@
 /*
  * tbsla_oracle.c -- CPU restatement of TBSLA's sparse hot path.
  *
- * TEST INFRASTRUCTURE.  This is the checker of the HIP library: tests/, the
+ * TEST INFRASTRUCTURE.  This is the oracle of the HIP library: tests/, the
  * smoke check and the CPU-baseline leg of bench.py load it (liboracle.so);
  * nothing in the product links, loads or calls it.
  *

@@ -1,7 +1,7 @@
 /*
  * tbsla_oracle.h -- CPU restatement of TBSLA's sparse hot path (test infrastructure).
  *
- * This header belongs to the checker, not to the product: only tests/, the
+ * This header belongs to the oracle, not to the product: only tests/, the
  * smoke check and the CPU-baseline leg of bench.py load liboracle.so.  The
  * HIP library never links it.
  *
