@@ -614,7 +614,7 @@ constexpr int kQuadEnt = 2 * kQuadLanes;       // entries per row in the pipelin
 // lane K of each group of 16, in every lane of the group
 template <int K>
 __device__ __forceinline__ int qbc(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 template <int K>
 __device__ __forceinline__ double qbc_d(double v) {
