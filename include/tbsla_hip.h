@@ -378,8 +378,9 @@ typedef struct {
                             forms sum G^T's and G's rows in ascending column, so they give
                             bitwise the same iterates */
   int max_blocks_per_system; /* workgroups of the iteration's kernels per system, at most
-                            (1..1024); 0 picks 1024 for one system and 256 when
-                            tbsla_hip_cg_batched solves more than one.  Every workgroup reads all of its system's
+                            (1..1024); 0 picks one per 512 rows, 256 to 1024, for one
+                            system and 256 when tbsla_hip_cg_batched solves more
+                            than one.  Every workgroup reads all of its system's
                             dot-product partials, so a batch, whose systems fill the GPU
                             together, takes fewer; a system solved alone and in a batch
                             gives bitwise the same iterates when both use the same value

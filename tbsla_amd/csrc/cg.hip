@@ -44,6 +44,7 @@ namespace {
 constexpr double kNearZero = 1.0e-10;  // CG.cpp:35
 constexpr int kMaxBlocksPerSystem = 1024;
 constexpr int kBatchBlocksPerSystem = 256;  // default for a batch of systems
+constexpr long long kRowsPerBlockLone = 512;  // default rows per workgroup of a lone system
 constexpr long long kTwoPassEntries = 1LL << 24;  // K1 in two passes from this many entries
 
 enum SegDone { kRunning = 0, kConverged = 1, kPending = 2, kBreakdown = 3, kInactive = 5 };
@@ -1829,10 +1830,13 @@ int cg_solve_on(tbsla_hip_matrix_s *m, int nsys, const int *h_rows, const int *h
     return fail(TBSLA_HIP_ERR_INVALID, "max_blocks_per_system must be 0..1024");
   // workgroups per system: the dot-product partials every workgroup of a
   // system reads grow with their square, which a lone system's parallelism
-  // pays for and a batch's does not
+  // pays for and a batch's does not.  A lone system: one workgroup per 512
+  // rows, 256 to 1024 (at 64^3, 512 workgroups beat 1024 and 256)
+  const long long lone = std::min<long long>(kMaxBlocksPerSystem,
+                                             std::max<long long>(256, n / kRowsPerBlockLone));
   const long long cap = o.max_blocks_per_system > 0 ? o.max_blocks_per_system
                         : nsys > 1                  ? kBatchBlocksPerSystem
-                                                    : kMaxBlocksPerSystem;
+                                                    : lone;
 
   if (!m->cg_cache) m->cg_cache = take_cache();
   CgCache &c = *static_cast<CgCache *>(m->cg_cache);

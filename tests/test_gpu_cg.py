@@ -222,7 +222,7 @@ def test_batched_equals_single(pre):
     opt = bd.cg_options(tolerance=1e-10, relative=True, preconditioner=pre, lanes_per_row=8)
     x, res = bd.cg_batched(offs, rhs, opt)
     for k, p in enumerate(parts):
-        # the batch's default of 256 workgroups per system (1024 alone)
+        # the batch's default of 256 workgroups per system (one per 512 rows, 256 to 1024, alone)
         xs, its, rs = p.cg(rhs[offs[k]:offs[k + 1]], tolerance=1e-10, relative=True,
                            preconditioner=pre, lanes_per_row=8, max_blocks_per_system=256)
         assert res[k].status == T.CG_CONVERGED
