@@ -605,13 +605,18 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
 // entry costs two 16-byte LDS reads, three broadcasts, one address and four
 // fused multiply-adds for the four rows' 64 columns at once.  Remote
 // entries are found per group from the wavefront's ballots and fetched by
-// lane shuffles.  The pipeline, ring, window and sums are those of
-// k_ppr_pipe; entries past a row's first 32 go chunk by chunk, unpipelined.
+// lane shuffles.  The tiles, ring, window and sums are those of
+// k_ppr_pipe, and so are the four pipeline stages, but each step runs its
+// sum first (quad_group_step); entries past a row's first 32 go sixteen at
+// a time, unpipelined.
 constexpr int kQuadLanes = 16;
 constexpr int kQuadRows = kWave / kQuadLanes;  // rows per wavefront step
 constexpr int kQuadEnt = 2 * kQuadLanes;       // entries per row in the pipeline
 
-// lane K of each group of 16, in every lane of the group
+// lane K of each group of 16, in every lane of the group (row_newbcast reads
+// a lane of the lane's own row, so every lane has a source; bound_ctrl set
+// lets the compiler give the broadcast straight to the instruction using it,
+// as the address add does)
 template <int K>
 __device__ __forceinline__ int qbc(int v) {
   return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
