@@ -651,8 +651,7 @@ __device__ __forceinline__ void quad_sp(const PprArgs &a, int r, QuadSlot<S> &s)
   s.end = r < 0 ? b : e;
 }
 
-// stage E: the first 32 entries (two per lane) and, unless OLD (every row's
-// old values in the ring), the old values
+// stage E: the first 32 entries (two per lane)
 template <int S, bool OLD>
 __device__ __forceinline__ void quad_se(const PprArgs &a, int r, int t, QuadSlot<S> &s) {
   const int len = s.end - s.beg;
@@ -664,12 +663,6 @@ __device__ __forceinline__ void quad_se(const PprArgs &a, int r, int t, QuadSlot
     const double v = a.vals[j];
     s.col[h] = i < len ? c - a.f_col : -1;
     s.val[h] = i < len ? v : 0.0;
-  }
-  if constexpr (!OLD) {
-    const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
-    const size_t o = (size_t)max(r, 0) * kPprPieces + 2 * t;
-    s.old[0] = X2[o];
-    s.old[1] = X2[o + 1];
   }
 }
 
@@ -693,9 +686,16 @@ __device__ __forceinline__ void quad_pick(const int (&col)[2], const double (&va
   v = idx < kQuadLanes ? v0 : v1;
 }
 
-// stage R: the X rows of the group's first S remote entries
-template <int S>
+// stage R: the X rows of the group's first S remote entries and, unless OLD
+// (every row's old values in the ring), the row's old values
+template <int S, bool OLD>
 __device__ __forceinline__ void quad_sr(const PprArgs &a, int r, int t, int g, QuadSlot<S> &s) {
+  if constexpr (!OLD) {
+    const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
+    const size_t o = (size_t)max(r, 0) * kPprPieces + 2 * t;
+    s.old[0] = X2[o];
+    s.old[1] = X2[o + 1];
+  }
   const PprWindow w = ppr_window(a, max(r, 0));
   unsigned m = quad_remote_mask(a, s.col[0], s.col[1], w.wlo, w.whi, g);
   const pprpiece_t *X2 = reinterpret_cast<const pprpiece_t *>(a.X);
@@ -864,7 +864,7 @@ __device__ __forceinline__ void quad_group_step(const PprArgs &a, const PprWalk 
   // C first: the remote rows R fetches next are then live only until the
   // next step's C, whose band walk covers their latency
   quad_sc<S, OLD>(a, quad_row(a, wk, qs, g), t, g, st[K], win, e, sm);
-  quad_sr(a, quad_row(a, wk, qs + DR, g), t, g, st[(K + DR) % NS]);
+  quad_sr<S, OLD>(a, quad_row(a, wk, qs + DR, g), t, g, st[(K + DR) % NS]);
   quad_se<S, OLD>(a, quad_row(a, wk, qs + DE, g), t, st[(K + DE) % NS]);
   quad_sp(a, quad_row(a, wk, qs + D0, g), st[(K + D0) % NS]);
   if constexpr (K + 1 < NS)
@@ -880,7 +880,7 @@ __device__ __forceinline__ void quad_prologue(const PprArgs &a, const PprWalk &w
   } else if constexpr (K < D0 + DE) {
     quad_se<S, OLD>(a, quad_row(a, wk, K - D0, g), t, st[K - D0]);
   } else if constexpr (K < D0 + DE + DR) {
-    quad_sr(a, quad_row(a, wk, K - D0 - DE, g), t, g, st[K - D0 - DE]);
+    quad_sr<S, OLD>(a, quad_row(a, wk, K - D0 - DE, g), t, g, st[K - D0 - DE]);
   }
   if constexpr (K + 1 < D0 + DE + DR) quad_prologue<NS, DE, DR, S, OLD, K + 1>(a, wk, st, t, g);
 }
@@ -1211,9 +1211,10 @@ extern "C" int tbsla_hip_personalized_page_rank(tbsla_hip_matrix_t m, double bet
     for (int i = 0; i < batch; i++, launched++) {
       pa.X = Xb[launched % 2];
       pa.Xn = Xb[(launched + 1) % 2];
-      // four rows per wavefront when every row's old values are in the ring
+      // old values from the ring (three remote rows ahead per row), or from
+      // X (two, for the registers the old values take)
       if (pa.old_lds) k_ppr_quad<4, 2, 1, 3, true><<<nb, kPprBlock, 0, s>>>(pa);
-      else k_ppr_pipe<4, 2, 1, 4><<<nb, kPprBlock, 0, s>>>(pa);
+      else k_ppr_quad<4, 2, 1, 2, false><<<nb, kPprBlock, 0, s>>>(pa);
       k_ppr_finalize<<<1, kBlock, 0, s>>>(st, part, part + (size_t)nb * kWave, (int)nb, S, epsilon);
     }
     note(hipGetLastError());

@@ -130,6 +130,31 @@ def test_ppr_band_off_the_diagonal():
     ppr_against_oracle(n, rp, cols, va, src)
 
 
+def test_ppr_off_diagonal_band_long_rows_many_remote():
+    """Old values from HBM with rows past the first 32 entries and more
+    remote entries than the kernel fetches ahead: each row holds 24 entries
+    50..73 columns right of the diagonal (wrapping) and 16 random columns,
+    a few rows 90 entries."""
+    n = 4000
+    rng = np.random.default_rng(9)
+    rows, cols = [], []
+    for i in range(n):
+        k_band, k_rand = (24, 16) if i % 97 else (60, 30)
+        for k in range(k_band):
+            rows.append(i)
+            cols.append((i + 50 + k) % n)
+        for c in rng.integers(0, n, k_rand):
+            rows.append(i)
+            cols.append(int(c))
+    rows, cols = np.array(rows), np.array(cols)
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp)
+    va = O.unit_vector(8, len(cols)) + 0.1
+    src = np.random.default_rng(5).choice(n, 64, replace=False).astype(np.int32)
+    ppr_against_oracle(n, rp, cols, va, src)
+
+
 @pytest.mark.parametrize("n", [64, 100, 130])
 def test_ppr_small_blocks(n):
     """One or three tiles: a partial last tile and a ring larger than the
