@@ -158,24 +158,17 @@ __device__ inline void vset(VT &v, int q, double x) {
 // they are loaded into registers while this tile is summed and written over
 // the departing rows at the tile boundary.  Each X row is read from HBM once
 // per step (plus the first window of each workgroup), and a row's old value
-// X[r, :] comes from the ring too.  A wavefront
-// takes whole rows, lane t holding column t.  A row's entries (at most 64 per
-// chunk) are loaded one per lane and broadcast by readlane; each entry's X
-// row comes from LDS when it lies in the window ("band") or from HBM
-// ("remote": rewired entries, about 10% in a cqmat with q = 0.1).  The sum
-// of a chunk is its band entries in storage order, then its remote entries
-// in storage order, each a fused multiply-add.  A
-// wavefront runs its rows through a software pipeline (row pointers, then
-// entries, then the remote X rows, then the sum), so each row's loads are in
-// flight while earlier rows are summed from LDS; the band sum walks the
-// entries by lane index, with no scalar bookkeeping per entry.
+// X[r, :] comes from the ring too when the tile's own rows lie in its window.
+// Each entry's X row comes from LDS when it lies in the window ("band") or
+// from HBM ("remote": rewired entries, about 10% in a cqmat with q = 0.1);
+// a row's sum is its band entries in storage order, then its remote entries
+// in storage order, each a fused multiply-add (k_ppr_quad below).
 constexpr int kPprBlock = 256;                  // threads of a tile workgroup
 constexpr int kPprWaves = kPprBlock / kWave;    // 4 wavefronts
 constexpr int kPprTile = 64;                    // rows per tile
 constexpr int kPprRowsPerWave = kPprTile / kPprWaves;
 constexpr int kPprWin = 96;                     // X rows staged per tile (48 KB)
 constexpr int kPprReach = kPprWin - kPprTile;   // band offsets [dlo, dlo + 32] always staged
-constexpr int kPprRemote = 8;                   // remote X rows in flight together (long rows)
 constexpr int kPprHist = 128;                   // offsets [-64, 63] in the histogram
 
 struct PprArgs {
@@ -196,13 +189,6 @@ struct PprArgs {
   int last;  // index of the last stored entry (a dummy entry when the block has none)
 };
 
-// lane idx's double, the same in every lane (two v_readlane_b32)
-__device__ __forceinline__ double readlane_d(double v, int idx) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), idx);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), idx);
-  return __hiloint2double(hi, lo);
-}
-
 // one product added to a row sum (a fused multiply-add: one VALU
 // instruction per entry and column instead of two; the step is within
 // rounding of the unfused sum)
@@ -210,133 +196,6 @@ __device__ __forceinline__ double ppr_madd(double acc, double v, double x) { ret
 
 // the ring slot of X row j (j >= -kPprWin)
 __device__ __forceinline__ int ring_slot(int j) { return (int)((unsigned)(j + kPprWin) % kPprWin); }
-
-// The chunk's remote entries: the first kPprRemote X rows loaded into xr
-// (their values in vr, zeros past the count); returns the mask of the ones
-// left over.
-__device__ __forceinline__ unsigned long long ppr_remote_issue(const PprArgs &a, int col, double val,
-                                                      bool in, int wlo, int whi, int lane,
-                                                      double (&xr)[kPprRemote],
-                                                      double (&vr)[kPprRemote]) {
-  unsigned long long m = __ballot(in && col >= 0 && col < a.n && (col < wlo || col >= whi));
-#pragma unroll
-  for (int k = 0; k < kPprRemote; k++) {
-    if (m) {
-      const int idx = __builtin_ctzll(m);
-      m &= m - 1;
-      const int c = __builtin_amdgcn_readlane(col, idx);
-      vr[k] = readlane_d(val, idx);
-      xr[k] = a.X[(size_t)c * kWave + lane];
-    } else {
-      vr[k] = 0.0;
-      xr[k] = 0.0;
-    }
-  }
-  return m;
-}
-
-// sum of the chunk's band entries in storage order (X rows from the window)
-__device__ __forceinline__ double ppr_band(int col, double val, bool in, int wlo, int whi,
-                                  const double *__restrict__ win, int lane) {
-  TBSLA_NO_FMA
-  unsigned long long m = __ballot(in && col >= wlo && col < whi);
-  double acc = 0.0;
-  while (m) {
-    double xv[8], vv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (m) {
-        const int idx = __builtin_ctzll(m);
-        m &= m - 1;
-        const int c = __builtin_amdgcn_readlane(col, idx);
-        vv[k] = readlane_d(val, idx);
-        xv[k] = win[ring_slot(c) * kWave + lane];
-      } else {
-        vv[k] = 0.0;  // adds +0: acc (never -0) is unchanged
-        xv[k] = 0.0;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = ppr_madd(acc, vv[k], xv[k]);
-  }
-  return acc;
-}
-
-// The same sum for a row's first chunk (len entries, at most 64), walking
-// the entries in storage order by lane index: a lane's entry outside the
-// window points at the window's zero row with value 0 (adds +0, which
-// leaves acc unchanged), so the walk needs no mask and no scalar
-// bookkeeping -- three readlanes, an LDS read, a multiply and an add per
-// entry.
-__device__ __forceinline__ double ppr_band_walk(int col, double val, int len, int wlo, int whi,
-                                                const double *__restrict__ win,
-                                                int lane) {
-  TBSLA_NO_FMA
-  const bool band = lane < len && col >= wlo && col < whi;
-  const int off = band ? ring_slot(col) * kWave : kPprWin * kWave;  // zero row
-  const double v = band ? val : 0.0;
-  double acc = 0.0;
-#pragma unroll
-  for (int ch = 0; ch < kWave / 8; ch++) {
-    if (ch * 8 >= len) break;
-    double xv[8], vv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int o = __builtin_amdgcn_readlane(off, ch * 8 + k);
-      vv[k] = readlane_d(v, ch * 8 + k);
-      xv[k] = win[o + lane];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = ppr_madd(acc, vv[k], xv[k]);
-  }
-  return acc;
-}
-
-// the remote products after the band (loaded ones, then the left-over ones)
-__device__ __forceinline__ double ppr_remote_add(const PprArgs &a, double acc, int col, double val,
-                                        unsigned long long left, const double (&xr)[kPprRemote],
-                                        const double (&vr)[kPprRemote], int lane) {
-  TBSLA_NO_FMA
-#pragma unroll
-  for (int k = 0; k < kPprRemote; k++) acc = ppr_madd(acc, vr[k], xr[k]);
-  while (left) {
-    const int idx = __builtin_ctzll(left);
-    left &= left - 1;
-    const int c = __builtin_amdgcn_readlane(col, idx);
-    acc = ppr_madd(acc, readlane_d(val, idx), a.X[(size_t)c * kWave + lane]);
-  }
-  return acc;
-}
-
-// entries past a row's first 64: chunk by chunk, band then remote
-__device__ __forceinline__ double ppr_long_tail(const PprArgs &a, int beg, int len, double acc,
-                                                int wlo, int whi,
-                                                const double *__restrict__ win, int lane) {
-  TBSLA_NO_FMA
-  for (int c0 = kWave; c0 < len; c0 += kWave) {
-    const bool in = c0 + lane < len;
-    const int j = in ? beg + c0 + lane : 0;
-    const int col = in ? a.colidx[j] - a.f_col : -1;
-    const double val = in ? a.vals[j] : 0.0;
-    double xr[kPprRemote], vr[kPprRemote];
-    const unsigned long long left = ppr_remote_issue(a, col, val, in, wlo, whi, lane, xr, vr);
-    acc = acc + ppr_band(col, val, in, wlo, whi, win, lane);
-    acc = ppr_remote_add(a, acc, col, val, left, xr, vr, lane);
-  }
-  return acc;
-}
-
-// a row in the pipeline (row q of the wavefront's sequence): its row-pointer
-// pair, its first chunk of entries, one per lane (lane < len), the lane's old
-// X value, and the X rows of its first S remote entries
-template <int S>
-struct PprSlot {
-  int rp;                   // lanes 0, 1: rowptr[r], rowptr[r + 1]
-  int beg, len;
-  int col;                  // lane's entry: column in the block, -1 past the chunk
-  double val, old;
-  double xr[S];             // X rows of the first S remote entries
-};
 
 // the window of row r's tile in the ring: X rows [wlo, whi)
 struct PprWindow {
@@ -363,89 +222,6 @@ __device__ __forceinline__ int ppr_row_at(const PprArgs &a, const PprWalk &wk, i
   return (t < wk.t1 && r < a.n) ? r : -1;
 }
 
-// The stages issue the same loads whatever the row (a missing row reads row
-// 0; an unused remote slot reads X row 0, which stays cached), so the
-// compiler counts the loads in flight exactly and waits only for the ones a
-// stage consumes.
-//
-// stage P: the row-pointer pair
-template <int S>
-__device__ __forceinline__ void ppr_sp(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
-  s.rp = a.rowptr[max(r, 0) + (lane & 1)];
-}
-
-// stage E: the first chunk of entries and the old X values
-template <int S>
-__device__ __forceinline__ void ppr_se(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
-  const int beg = __builtin_amdgcn_readlane(s.rp, 0);
-  const int len = r < 0 ? 0 : __builtin_amdgcn_readlane(s.rp, 1) - beg;
-  const bool in = lane < len;
-  // lanes past the row's end read its last entry (no line beyond the row's)
-  const int j = min(beg + min(lane, max(len - 1, 0)), a.last);
-  const int c = a.colidx[j];
-  const double v = a.vals[j];
-  s.old = a.old_lds ? 0.0 : a.X[(size_t)max(r, 0) * kWave + lane];
-  s.beg = beg;
-  s.len = len;
-  s.col = in ? c - a.f_col : -1;
-  s.val = in ? v : 0.0;
-}
-
-// the chunk's entries outside the row's window (summed from HBM, not LDS)
-__device__ __forceinline__ unsigned long long ppr_remote_mask(const PprArgs &a, int col,
-                                                              const PprWindow &w) {
-  return __ballot(col >= 0 && col < a.n && (col < w.wlo || col >= w.whi));
-}
-
-// stage R: the X rows of the first S remote entries
-template <int S>
-__device__ __forceinline__ void ppr_sr(const PprArgs &a, int r, int lane, PprSlot<S> &s) {
-  const int rr = max(r, 0);
-  unsigned long long m = ppr_remote_mask(a, s.col, ppr_window(a, rr));
-#pragma unroll
-  for (int k = 0; k < S; k++) {
-    const int idx = m ? __builtin_ctzll(m) : 0;
-    const int c = m ? __builtin_amdgcn_readlane(s.col, idx) : 0;  // unused slot: X row 0 (cached)
-    m &= m - 1;
-    s.xr[k] = a.X[(size_t)c * kWave + lane];
-  }
-}
-
-// stage C: the row's sum -- band entries (LDS) in storage order, then remote
-// entries in storage order, then entries past the first 64 chunk by chunk,
-// each a separate multiply and add -- and its update
-template <int S>
-__device__ __forceinline__ void ppr_sc(const PprArgs &a, int r, const PprSlot<S> &s,
-                                       const double *__restrict__ win, int lane, int src_l,
-                                       double S_l, double tele, double &e, double &sm) {
-  TBSLA_NO_FMA
-  if (r < 0) return;
-  const PprWindow w = ppr_window(a, r);
-  double acc = ppr_band_walk(s.col, s.val, s.len, w.wlo, w.whi, win, lane);
-  unsigned long long m = ppr_remote_mask(a, s.col, w);
-#pragma unroll
-  for (int k = 0; k < S; k++) {
-    if (m) {
-      const int idx = __builtin_ctzll(m);
-      m &= m - 1;
-      acc = ppr_madd(acc, readlane_d(s.val, idx), s.xr[k]);
-    }
-  }
-  while (m) {  // remote entries past the first S (rare)
-    const int idx = __builtin_ctzll(m);
-    m &= m - 1;
-    const int c = __builtin_amdgcn_readlane(s.col, idx);
-    acc = ppr_madd(acc, readlane_d(s.val, idx), a.X[(size_t)c * kWave + lane]);
-  }
-  if (s.len > kWave) acc = ppr_long_tail(a, s.beg, s.len, acc, w.wlo, w.whi, win, lane);
-  double tv = a.beta * acc + ((r + a.f_col == src_l) ? tele : 0.0);
-  tv = tv / S_l;
-  const double old = a.old_lds ? win[ring_slot(r) * kWave + lane] : s.old;
-  e += fabs(tv - old);
-  sm += tv;
-  a.Xn[(size_t)r * kWave + lane] = tv;
-}
-
 // Distances of the stages ahead of the row being summed: P at D0 = NS - 1,
 // E at DE, R at DR.
 template <int NS, int DE, int DR>
@@ -453,41 +229,6 @@ struct PprPipe {
   static constexpr int D0 = NS - 1;
   static_assert(D0 > DE && DE > DR && DR >= 1, "each stage after the one it consumes");
 };
-
-// One group of NS rows: for row q0 + K, stage P of row q0 + K + D0, E of
-// q0 + K + DE, R of q0 + K + DR, then C of the row itself.  The slot index
-// is a constant in every call, so the slots live in registers.
-template <int NS, int DE, int DR, int S, int K>
-__device__ __forceinline__ void ppr_group_step(const PprArgs &a, const PprWalk &wk, int q0,
-                                               PprSlot<S> (&st)[NS],
-                                               const double *__restrict__ win, int lane,
-                                               int src_l, double S_l, double tele, double &e,
-                                               double &sm) {
-  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
-  const int q = q0 + K;
-  ppr_sp(a, ppr_row_at(a, wk, q + D0), lane, st[(K + D0) % NS]);
-  ppr_se(a, ppr_row_at(a, wk, q + DE), lane, st[(K + DE) % NS]);
-  ppr_sr(a, ppr_row_at(a, wk, q + DR), lane, st[(K + DR) % NS]);
-  ppr_sc(a, ppr_row_at(a, wk, q), st[K], win, lane, src_l, S_l, tele, e, sm);
-  if constexpr (K + 1 < NS)
-    ppr_group_step<NS, DE, DR, S, K + 1>(a, wk, q0, st, win, lane, src_l, S_l, tele, e, sm);
-}
-
-// before the first group: P of rows 0 .. D0 - 1, E of rows 0 .. DE - 1, R of
-// rows 0 .. DR - 1
-template <int NS, int DE, int DR, int S, int K>
-__device__ __forceinline__ void ppr_prologue(const PprArgs &a, const PprWalk &wk,
-                                             PprSlot<S> (&st)[NS], int lane) {
-  constexpr int D0 = PprPipe<NS, DE, DR>::D0;
-  if constexpr (K < D0) {
-    ppr_sp(a, ppr_row_at(a, wk, K), lane, st[K]);
-  } else if constexpr (K < D0 + DE) {
-    ppr_se(a, ppr_row_at(a, wk, K - D0), lane, st[K - D0]);
-  } else if constexpr (K < D0 + DE + DR) {
-    ppr_sr(a, ppr_row_at(a, wk, K - D0 - DE), lane, st[K - D0 - DE]);
-  }
-  if constexpr (K + 1 < D0 + DE + DR) ppr_prologue<NS, DE, DR, S, K + 1>(a, wk, st, lane);
-}
 
 // The ring's loads: kPprRingLoads 16-byte pieces per thread cover the
 // kPprTile rows a tile adds to the ring.  Rows [j0, j0 + kPprTile) of tile t
@@ -524,77 +265,6 @@ __device__ __forceinline__ void ring_store(const PprArgs &a, bool tile_ok, int j
   }
 }
 
-// A workgroup takes a contiguous run of tiles of kPprTile rows
-// (kPprRowsPerWave per wavefront); a wavefront runs its rows through the four
-// stages above, so a row's row-pointer, entry and remote X loads are in
-// flight while the rows before it are summed from LDS.  NS divides
-// kPprRowsPerWave, so a tile starts a group.  At a tile boundary the
-// workgroup waits for every row of the tile, writes the prefetched rows over
-// the departing ones, waits again, and issues the loads of the next tile's
-// new rows.
-template <int NS, int DE, int DR, int S>
-__global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
-  TBSLA_NO_FMA
-  static_assert(kPprRowsPerWave % NS == 0, "a tile starts a group");
-  static_assert(kPprWin > kPprTile && kPprWin <= 2 * kPprTile, "the first window is two loads");
-  if (a.st->done) return;
-  // the ring, then one row of zeros (read for entries outside the window)
-  __shared__ __attribute__((aligned(16))) double win[(kPprWin + 1) * kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-  if (threadIdx.x < kWave) win[kPprWin * kWave + threadIdx.x] = 0.0;  // before the first barrier
-  const double S_l = a.S[lane];
-  const int src_l = a.src[lane];
-  const double tele = S_l * (1 - a.beta);
-  double e = 0.0, sm = 0.0;
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int per = (a.ntiles + nb - 1) / nb;
-  PprWalk wk;
-  wk.wave = wave;
-  wk.t0 = min(a.ntiles, b * per);
-  wk.t1 = min(a.ntiles, wk.t0 + per);
-  wk.tstep = 1;
-  const int nq = (int)((wk.t1 - wk.t0) * kPprRowsPerWave);
-  // the first tile's window [base0, base0 + kPprWin) in two loads of
-  // kPprTile rows, then the next tile's new rows into registers
-  const int base0 = wk.t0 * kPprTile + a.dlo;
-  const bool any = wk.t0 < wk.t1;
-  pprpiece_t pre[kPprRingLoads];
-  ring_load(a, any, base0, base0 + kPprWin, pre);
-  ring_store(a, any, base0, base0 + kPprWin, pre, win);
-  ring_load(a, any, base0 + kPprTile, base0 + kPprWin, pre);
-  ring_store(a, any, base0 + kPprTile, base0 + kPprWin, pre, win);
-  ring_load(a, wk.t0 + 1 < wk.t1, base0 + kPprWin, base0 + kPprWin + kPprTile, pre);
-  __syncthreads();
-  PprSlot<S> st[NS];
-  ppr_prologue<NS, DE, DR, S, 0>(a, wk, st, lane);
-  for (int q0 = 0; q0 < nq; q0 += NS) {
-    if (q0 > 0 && q0 % kPprRowsPerWave == 0) {
-      const int t = wk.t0 + q0 / kPprRowsPerWave;
-      const int jn = t * kPprTile + a.dlo + (kPprWin - kPprTile);  // tile t's new rows
-      __syncthreads();  // tile t - 1's rows are done with the departing rows
-      ring_store(a, true, jn, jn + kPprTile, pre, win);
-      __syncthreads();
-      ring_load(a, t + 1 < wk.t1, jn + kPprTile, jn + 2 * kPprTile, pre);
-    }
-    ppr_group_step<NS, DE, DR, S, 0>(a, wk, q0, st, win, lane, src_l, S_l, tele, e, sm);
-  }
-  // per-column partials of the workgroup: waves in order
-  __syncthreads();
-  win[wave * kWave + lane] = e;
-  win[(kPprWaves + wave) * kWave + lane] = sm;
-  __syncthreads();
-  if (threadIdx.x < kWave) {
-    double te = 0.0, ts = 0.0;
-    for (int v = 0; v < kPprWaves; v++) {
-      te += win[v * kWave + lane];
-      ts += win[(kPprWaves + v) * kWave + lane];
-    }
-    a.p_err[(size_t)b * kWave + lane] = te;
-    a.p_sum[(size_t)b * kWave + lane] = ts;
-  }
-}
-
 // ---- four rows per wavefront ----------------------------------------------
 //
 // The same step with a wavefront taking four rows at a time, one per group
@@ -605,10 +275,12 @@ __global__ __launch_bounds__(kPprBlock, 3) void k_ppr_pipe(PprArgs a) {
 // entry costs two 16-byte LDS reads, three broadcasts, one address and four
 // fused multiply-adds for the four rows' 64 columns at once.  Remote
 // entries are found per group from the wavefront's ballots and fetched by
-// lane shuffles.  The tiles, ring, window and sums are those of
-// k_ppr_pipe, and so are the four pipeline stages, but each step runs its
-// sum first (quad_group_step); entries past a row's first 32 go sixteen at
-// a time, unpipelined.
+// lane shuffles.  A workgroup takes a contiguous run of tiles; each
+// wavefront runs its rows through four pipeline stages (row extents,
+// entries, remote X rows, sum) so that a row's loads are in flight while
+// earlier rows are summed, each step running its sum first
+// (quad_group_step); entries past a row's first 32 go sixteen at a time,
+// unpipelined.
 constexpr int kQuadLanes = 16;
 constexpr int kQuadRows = kWave / kQuadLanes;  // rows per wavefront step
 constexpr int kQuadEnt = 2 * kQuadLanes;       // entries per row in the pipeline
