@@ -560,7 +560,7 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
       return TBSLA_HIP_SUCCESS;
     }
     float best = ms;
-    const float ell_ms = ms;
+    const float own_ms = ms;  // the format's own kernel
     int best_slices = 0, best_layout = 0, best_window = 0;
     for (int layout : {TBSLA_HIP_CSR_SPLIT, TBSLA_HIP_CSR_DIAG})
       for (int slices : {16, 8}) {
@@ -571,7 +571,7 @@ int tbsla_hip_autotune(tbsla_hip_matrix_t m, double *d_r, const double *d_v, int
         TBSLA_TRY(split_build(m, window, slices, s, layout));
         TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, 1, &ms, stream));
         TBSLA_TRY(tbsla_hip_time_spmv(m, d_r, d_v, iters, &ms, stream));
-        if (ms < best && ms < ell_ms * kDerivedGain) {
+        if (ms < best && ms < own_ms * kDerivedGain) {
           best = ms;
           best_slices = slices;
           best_layout = layout;
