@@ -154,7 +154,11 @@ def evaluate(net, side: int, seeds, methods=METHODS, tol: float = 1e-8, warm: bo
     factors = (T.Matrix(T.CSR), T.Matrix(T.CSR))  # one pair for all problems of this size
     for k, sd in enumerate(seeds):
         A, rhs = poisson_problem(side, sd)
-        A.tune_layout()  # the 7-point matrix: the CG engine reads its diagonals
+        # the 7-point matrix in the diagonal layout: the CG engine then reads
+        # its 7 stored diagonals (no column indices).  The SpMV autotuner's
+        # 3 % rule can leave a 64^3 matrix on its rows, where K1 costs 11.7
+        # instead of 9.5 us.
+        A.set_csr_layout(T.CSR_DIAG, 65536, 8)
         s = Solver(A, rhs, net, factors)
         for m in methods:
             if net is None and m.startswith("learned"):
